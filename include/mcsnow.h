@@ -1,0 +1,200 @@
+/*
+ * mcsnow.h -- C ABI of the MI355X (gfx950) photon random-walk library
+ * (libmcsnow.so, built from monte_carlompi_amd/csrc/).
+ *
+ * Drop-in boundary.  The reference has no FFI; its seam is the Python region
+ *     par_wvls = Parallel(wvls)                                  mc3D:1521
+ *     for i, wvl in enumerate(par_wvls.working_set):             mc3D:1613-1616
+ *         self.photon = i; answer.append(self.monte_carlo3D(wvl))
+ *     all_answers = par_wvls.answer_and_reduce(answer, ...)      mc3D:1618-1619
+ * with the mpi4py scatter/gather of monte_carloMPI/parallelize.py:14-38
+ * (mc3D = monte_carloMPI/monte_carlo3D.py).  These entry points replace that
+ * region: the host uploads the per-photon parameter tables (the scatter),
+ * one call walks every photon on the GPU (the per-rank photon loop), and the
+ * records come back in photon order (the gather + flatten_list, mc3D:1845).
+ *
+ * Conventions: plain C types and pointers only; host buffers are owned by
+ * the caller; device buffers by the context.  Every function returns an
+ * int status (MCS_OK or a negative MCS_ERR_*); mcs_last_error() gives the
+ * text.  No function calls exit() or abort().  A context binds one HIP
+ * device and one stream; calls on one context must not overlap.
+ */
+#ifndef MCSNOW_H
+#define MCSNOW_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MCS_OK 0
+#define MCS_ERR_ARG (-1)        /* invalid argument (NULL, size, NaN, range) */
+#define MCS_ERR_HIP (-2)        /* HIP runtime error                          */
+#define MCS_ERR_NOMEM (-3)      /* device allocation failed                   */
+#define MCS_ERR_STEP_CAP (-4)   /* a photon reached max_steps                 */
+#define MCS_ERR_REPLAY (-5)     /* replay slice too short or not consumed     */
+#define MCS_ERR_STATE (-6)      /* call out of order (e.g. no cases set)      */
+#define MCS_ERR_PHI_UNBOUND (-7) /* replay: final direction on an axis; the
+                                    reference raises UnboundLocalError there */
+
+#define MCS_NATIVE 0            /* FP32 walk, Philox4x32-10 per photon        */
+#define MCS_REPLAY 1            /* FP64 walk consuming the reference's
+                                   uniform-deviate stream (CSR slices)         */
+
+#define MCS_THETA_BINS 18       /* BRF zenith bins over [0, pi/2), 5 deg      */
+#define MCS_PHI_BINS 36         /* BRF azimuth bins over [0, 2 pi), 10 deg    */
+#define MCS_BRF_BINS (MCS_THETA_BINS * MCS_PHI_BINS)
+#define MCS_TALLY_F64 656       /* doubles per case, see mcs_read_tallies     */
+#define MCS_TALLY_U64 16        /* uint64 per case, see mcs_read_counts       */
+
+typedef struct mcs_ctx mcs_ctx;
+
+/* One case: a snowpack + illumination configuration (the scalar part of
+ * MonteCarlo.run's state, mc3D:1504-1512, 1612).  Its photons take their
+ * single-scattering properties from n_bands wavelength bins (rows of the band
+ * table starting at band_begin): the per-photon arrays g, ssa_ice, ssa_imp,
+ * ext_cff_mss, P_ext_imp of mc3D:1577-1588, which depend on the photon only
+ * through its wavelength (rounded to 0.01 um, mc3D:1519-1520). */
+typedef struct {
+    double tau_tot;            /* layer optical depth (> 0)                   */
+    double mux0;               /* initial direction x: sin(theta0)  mc3D:1119 */
+    double muz0;               /* initial direction z: -cos(theta0) mc3D:1121 */
+    double r_lambertian;       /* bottom reflectance in [0, 1]                */
+    int32_t lambertian_bottom; /* 1: Lambertian bottom (mc3D:1420-1459)       */
+    int32_t band_begin;        /* first band row of this case                 */
+    int32_t n_bands;           /* number of band rows (1..MCS_MAX_BANDS)      */
+    int32_t reserved;
+    int64_t n_photons;         /* photon packets of this case (>= 0)          */
+} mcs_case;
+
+#define MCS_MAX_BANDS 64
+
+/* One wavelength bin of a case. */
+typedef struct {
+    double wvn;                /* 1/wvl [1/um]: record field, tally weight    */
+    double g;                  /* HG asymmetry parameter, |g| < 1             */
+    double ssa_ice;            /* single-scattering albedo of ice             */
+    double ssa_imp;            /* single-scattering albedo of the impurity    */
+    double ext_cff;            /* ext_cff_mss * rho_snw [1/m] (> 0)           */
+    double p_ext_imp;          /* P(extinction by impurity), mc3D:1580        */
+    double snow_depth;         /* tau_tot / ext_cff [m] (record field)        */
+    double cdf;                /* cumulative photon fraction up to and
+                                  including this bin (last = 1); used when
+                                  the GPU samples photon wavelengths          */
+} mcs_band;
+
+typedef struct {
+    int32_t mode;              /* MCS_NATIVE or MCS_REPLAY                    */
+    int32_t want_records;      /* 1: keep per-photon records for
+                                  mcs_read_records                            */
+    uint64_t seed;             /* Philox key (native mode)                    */
+    int64_t max_steps;         /* per-photon step cap, 0 = 2^24               */
+    /* optional: band index (within its case) of every photon, all cases
+     * concatenated in case order (sum of n_photons entries).  NULL: native
+     * mode samples bands from the cdf column; replay requires it.           */
+    const uint8_t *photon_band;
+    /* replay: CSR uniform deviates, slice k = photon k's initial triple
+     * (tau0, ssa0, ext0 uniforms, mc3D:1035-1038) then every uniform its
+     * walk draws in the reference's order (mc3D:915-1023, 1245-1246, 1422,
+     * 1453).  replay_off has sum(n_photons) + 1 entries.                    */
+    const double *replay_dev;
+    const int64_t *replay_off;
+    int32_t async;             /* 1: return after enqueueing; mcs_sync()
+                                  waits and reports errors                    */
+    int32_t use_graph;         /* 1: launch through a captured hipGraph       */
+} mcs_walk_args;
+
+/* Launch shape of the persistent walk kernel (autotuned; results do not
+ * depend on it: each photon's Philox stream is keyed by (case, photon)). */
+typedef struct {
+    int32_t block;             /* threads per block: 64..1024, multiple of 64 */
+    int32_t blocks_per_cu;     /* grid = blocks_per_cu * CUs                  */
+    int32_t chunk;             /* photons per work-queue chunk (>= 64)        */
+    int32_t refill_min;        /* idle lanes that trigger a wave refill 1..64 */
+} mcs_launch;
+
+/* Host output arrays (each sum(n_photons) long, photon order; any may be
+ * NULL).  The CSV record of mc3D:1489-1490 / 1626-1636 is
+ * (condition, wvn[band], theta_n, phi_n, n_scat, path_length,
+ *  snow_depth[band]). */
+typedef struct {
+    int8_t *condition;         /* 1 top, 2 bottom diffuse, 3 bottom direct,
+                                  4 absorbed by ice, 5 by impurity, 0 capped  */
+    uint8_t *band;
+    int64_t *n_scat;
+    double *theta_n;
+    double *phi_n;
+    double *path_length;
+} mcs_records;
+
+typedef struct {
+    uint64_t photons;          /* photons walked by the last call             */
+    uint64_t steps;            /* photon-steps: sum over photons of n_scat+1  */
+    uint64_t capped;           /* photons stopped at max_steps                */
+    uint64_t phi_unbound;      /* photons whose final direction lies on an
+                                  axis (phi_n from atan2; replay: error)      */
+    double kernel_ms;          /* device time of the last walk (HIP events on
+                                  the context stream)                         */
+} mcs_stats;
+
+/* replaces: process start + MonteCarlo() construction on every rank (drv:104) */
+int mcs_create(int device, mcs_ctx **out);
+int mcs_destroy(mcs_ctx *ctx);
+const char *mcs_last_error(const mcs_ctx *ctx);
+/* number of compute units of the context's device */
+int mcs_device_cus(const mcs_ctx *ctx, int32_t *out);
+
+/* replaces: Parallel(wvls) scatter of the per-photon parameter arrays
+ * (parallelize.py:28-38, mc3D:1521-1612) */
+int mcs_set_cases(mcs_ctx *ctx, const mcs_case *cases, int32_t n_cases,
+                  const mcs_band *bands, int32_t n_bands);
+
+int mcs_set_launch(mcs_ctx *ctx, const mcs_launch *launch);
+int mcs_get_launch(const mcs_ctx *ctx, mcs_launch *out);
+
+/* replaces: the photon loop of run() (mc3D:1613-1616) calling
+ * monte_carlo3D (mc3D:1111-1490) on every rank */
+int mcs_walk(mcs_ctx *ctx, const mcs_walk_args *args);
+int mcs_sync(mcs_ctx *ctx);
+
+/* replaces: comm.gather + flatten_list (parallelize.py:17-26, mc3D:1618) */
+int mcs_read_records(mcs_ctx *ctx, const mcs_records *out, int64_t n_photons);
+
+/* Per-case float64 tallies (n_cases x MCS_TALLY_F64), the sums that
+ * post_processing.py computes from the CSV (pp:46-100, 125-128, 269-275):
+ *   [0, 648)   sum of wvn over reflected photons (condition 1) binned by
+ *              theta_n (18 x 5 deg, row) and phi_n (36 x 10 deg, column)
+ *   648        Q_down = sum of wvn over all photons
+ *   649..653   sum of wvn by condition 1..5
+ *   654        sum of wvn over capped photons (condition 0)
+ *   655        sum of wvn * path_length over reflected photons [m] */
+int mcs_read_tallies(mcs_ctx *ctx, double *out, int32_t n_cases);
+
+/* Per-case counts (n_cases x MCS_TALLY_U64):
+ *   0 photons, 1..5 photons by condition, 6 capped, 7 phi on an axis,
+ *   8 photon-steps, 9..15 reserved (0) */
+int mcs_read_counts(mcs_ctx *ctx, uint64_t *out, int32_t n_cases);
+
+int mcs_last_stats(const mcs_ctx *ctx, mcs_stats *out);
+
+/* Time each launch shape of `candidates` on the cases currently set (native
+ * mode, records off, `photons_cap` photons per case at most, `repeats`
+ * timed runs each) and keep the fastest.  times_ms (optional, n entries)
+ * receives the median time of each candidate.  No reference counterpart:
+ * the reference has a fixed rank count chosen by the user (README:54). */
+int mcs_autotune(mcs_ctx *ctx, const mcs_launch *candidates, int32_t n,
+                 int64_t photons_cap, int32_t repeats, uint64_t seed,
+                 double *times_ms, mcs_launch *best);
+
+/* Diagnostic: Philox4x32-10 (Salmon et al., SC'11) of n counters (4 x u32
+ * each, row-major) under key (k0, k1), computed on the device; lets tests
+ * check the device generator against known-answer vectors.  No reference
+ * counterpart (the reference draws from numpy's MT19937). */
+int mcs_philox(mcs_ctx *ctx, const uint32_t *ctr, uint32_t k0, uint32_t k1,
+               uint32_t *out, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCSNOW_H */

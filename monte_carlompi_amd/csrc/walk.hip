@@ -1,0 +1,1255 @@
+// walk.hip -- photon random walk in a snow layer for MI355X (gfx950).
+//
+// One photon per lane in a persistent grid.  Work is a queue of chunks
+// (case, photon range); the waves of a block draw photons from the block's
+// current chunk through an LDS counter and refill idle lanes wave-wide
+// (ballot + mbcnt rank, one LDS atomic per refill).  Each chunk owns one of
+// SLOTS LDS tally slots (FP64 BRF histogram 18 x 36 + sums + counts); the wave
+// that finishes a chunk's last photon flushes the slot to the case's global
+// tallies.  Finished photons are compacted per wave (ballot/mbcnt + one LDS
+// atomic) into the chunk's contiguous id range, so record stores of lanes that
+// finish together are contiguous; a second kernel scatters them to photon
+// order.
+//
+// Two walkers share that skeleton:
+//  * NativeWalker: FP32, Philox4x32-10 keyed by the seed, counter
+//    (photon, case, step, stream) -- results do not depend on launch shape;
+//  * ReplayWalker: FP64 with the reference's exact step semantics
+//    (monte_carloMPI/monte_carlo3D.py:1212-1490), consuming the reference's
+//    uniform deviates from CSR slices.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mcsnow.h"
+#include "philox.h"
+
+namespace mcs {
+
+constexpr int SLOTS = 4;
+constexpr int TF = MCS_TALLY_F64;
+constexpr int TU = MCS_TALLY_U64;
+constexpr double PI_D = 3.141592653589793;
+constexpr float TWO_PI_F = 6.283185307179586f;
+constexpr float LN2_F = 0.6931471805599453f;
+constexpr float HALF_PI_F = 1.5707963267948966f;
+
+// tally layout (doubles), see mcs_read_tallies
+constexpr int T_QDOWN = MCS_BRF_BINS;        // 648
+constexpr int T_QCOND = MCS_BRF_BINS + 1;    // 649..653 by condition 1..5
+constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
+constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
+// counts (uint64)
+constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8;
+
+struct DevCase {
+    double tau_tot, mux0, muz0, r_lamb;
+    unsigned long long t_refl;  // reflect iff 32 random bits < t_refl (R * 2^32)
+    float tau_tot_f, mux0_f, muz0_f;
+    int32_t lamb, band_begin, n_bands;
+    int64_t n_photons, gid_base;
+};
+
+struct DevBand {
+    double wvn, g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, cdf;
+    float hg_a, hg_b, hg_c, hg_d, hg_e;  // 1/(2g), 1+g^2, 1-g^2, 1-g, 2g
+    float inv_ext;                       // 1/ext_cff [m]
+    uint32_t t_abs_ice, t_abs_imp;       // absorbed iff bits < (1-ssa) 2^32
+    uint32_t t_imp;                      // impurity iff bits < P_ext_imp 2^32
+    int32_t iso;                         // g == 0
+};
+
+struct KParams {
+    const DevCase *cases;
+    const DevBand *bands;
+    const int32_t *case_chunk0;  // n_cases + 1: first chunk of each case
+    int32_t n_cases;
+    int32_t chunk;
+    uint32_t n_chunks;
+    uint32_t refill_min;
+    uint32_t key0, key1;
+    uint32_t max_steps;
+    int32_t pad;
+    const uint8_t *photon_band;  // nullable
+    const double *rdev;          // replay
+    const int64_t *roff;
+    double *tally_f;
+    unsigned long long *tally_u;
+    unsigned int *chunk_counter;
+    unsigned long long *ctl;     // [0] replay slice errors
+    uint32_t *rec_gid, *rec_nscat, *rec_cb;
+    void *rec_theta, *rec_phi, *rec_path;
+};
+
+// ---------------------------------------------------------------- LDS state
+struct SlotHdr {
+    unsigned long long seqcount;  // (block-local chunk seq << 32) | count
+    unsigned long long gid0;      // global id of the chunk's first photon
+    unsigned int p0;              // first photon (within its case)
+    int case_id;
+    int remaining;                // photons of the chunk not finished
+    int rec_next;                 // records written so far
+    int state;                    // 0 free, 1 live
+    int pad;
+};
+
+struct SlotTally {
+    double f[TF];
+    unsigned long long u[TU];
+};
+
+struct BlockState {
+    unsigned long long cur;  // (seq << 32) | next photon offset in chunk
+    int lock;
+    int exhausted;
+    SlotHdr hdr[SLOTS];
+    SlotTally tal[SLOTS];
+};
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ int lane_rank(unsigned long long mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+__device__ __forceinline__ int bcast(int v, int src) { return __shfl(v, src, 64); }
+__device__ __forceinline__ unsigned long long bcast64(unsigned long long v, int src)
+{
+    return __shfl(v, src, 64);
+}
+
+template <class T>
+__device__ __forceinline__ T lds_load(const T *p)
+{
+    return __atomic_load_n(p, __ATOMIC_RELAXED);
+}
+
+// ----------------------------------------------------------- native walker
+struct NativeWalker {
+    using Real = float;
+    static constexpr int MAX_BLOCK = 1024;
+    float z, mux, muy, muz, tau_acc;
+    uint32_t i, p, c;
+    int band;
+    bool bounce;
+    // per-photon parameters
+    float a, b, cc, d, e, tau_tot, inv_ext;
+    uint32_t t_abs_ice, t_abs_imp, t_imp;
+    unsigned long long t_refl;
+    bool iso, lamb, has_imp;
+    int cond;
+
+    __device__ void start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    {
+        const DevCase &C = P.cases[case_id];
+        int bnd = 0;
+        if (P.photon_band) {
+            bnd = P.photon_band[gid];
+        } else if (C.n_bands > 1) {
+            const u32x4 r = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
+            const double u = ((double)r.x + 0.5) * 2.3283064365386963e-10;
+            bnd = C.n_bands - 1;
+            for (int k = 0; k < C.n_bands - 1; ++k)
+                if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
+        }
+        const DevBand &B = P.bands[C.band_begin + bnd];
+        band = bnd;
+        a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
+        iso = B.iso != 0;
+        inv_ext = B.inv_ext;
+        t_abs_ice = B.t_abs_ice; t_abs_imp = B.t_abs_imp; t_imp = B.t_imp;
+        has_imp = B.t_imp != 0u;
+        tau_tot = C.tau_tot_f;
+        lamb = C.lamb != 0;
+        t_refl = C.t_refl;
+        z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
+        i = 0; p = photon; c = (uint32_t)case_id;
+        bounce = false;
+        cond = 0;
+    }
+
+    // one step; returns true when the photon has left or been absorbed
+    __device__ __forceinline__ bool step(const KParams &P)
+    {
+        ++i;
+        const u32x4 r = philox4x32_10(u32x4{p, c, i, 0u}, P.key0, P.key1);
+        const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(r.z));
+        float mxn = mux, myn = muy, mzn = muz;
+        if (i > 1) {
+            float cost, sint;
+            bool vertical_up = false;
+            if (bounce) {
+                // cosine-weighted hemisphere: cos(theta) = sqrt(u), the pdf
+                // 2 sin cos of the rejection loop in mc3D:1242-1250
+                const float u = u01f(r.x);
+                cost = __builtin_sqrtf(u);
+                sint = __builtin_sqrtf(fmaxf(1.f - u, 0.f));
+                vertical_up = true;
+                bounce = false;
+            } else if (iso) {
+                cost = fmaf(-2.f, u01f(r.x), 1.f);
+                sint = __builtin_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+            } else {
+                const float t = cc / fmaf(e, u01f(r.x), d);
+                cost = fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+                sint = __builtin_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+            }
+            // v_sin/v_cos take the angle in revolutions: 2 pi u -> u
+            const float rev = u01f(r.y);
+            const float sinp = __builtin_amdgcn_sinf(rev);
+            const float cosp = __builtin_amdgcn_cosf(rev);
+            const float s2 = fmaf(-muz, muz, 1.f);
+            if (vertical_up) {
+                mxn = sint * cosp; myn = sint * sinp; mzn = cost;
+            } else if (s2 < 1e-7f) {
+                const float sg = muz > 0.f ? 1.f : -1.f;
+                mxn = sint * cosp; myn = sg * sint * sinp; mzn = sg * cost;
+            } else {
+                const float rs = __builtin_amdgcn_rsqf(s2);
+                const float st = sint * rs;
+                mxn = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+                myn = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+                mzn = fmaf(-s2 * st, cosp, muz * cost);
+            }
+            mux = mxn; muy = myn; muz = mzn;
+        }
+        z = fmaf(dtau, mzn, z);
+        tau_acc += dtau;
+        bool imp = false;
+        if (has_imp) {
+            const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
+            imp = r2.x < t_imp;
+        }
+        if (z > 0.f) {
+            tau_acc -= fminf(z / mzn, dtau);
+            cond = 1;
+            return true;
+        }
+        if (z < -tau_tot) {
+            tau_acc -= fminf((z + tau_tot) / mzn, dtau);
+            z = -tau_tot;
+            if (lamb) {
+                const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
+                if ((unsigned long long)r2.y < t_refl) {
+                    bounce = true;
+                    return false;
+                }
+            }
+            cond = (i == 1) ? 3 : 2;
+            return true;
+        }
+        if (r.w < (imp ? t_abs_imp : t_abs_ice)) {
+            cond = imp ? 5 : 4;
+            return true;
+        }
+        return false;
+    }
+
+    __device__ void result(float &theta, float &phi, float &path, bool &axis) const
+    {
+        theta = acosf(fminf(fmaxf(muz, -1.f), 1.f));
+        path = tau_acc * inv_ext;
+        axis = false;
+        if (i == 1) {
+            phi = 0.f;
+        } else {
+            float ph = atan2f(muy, mux);
+            if (ph < 0.f) ph += TWO_PI_F;
+            phi = ph;
+            axis = (mux == 0.f) || (muy == 0.f);
+        }
+    }
+};
+
+// ----------------------------------------------------------- replay walker
+// FP64, the reference's operation order; contraction off so that every
+// product and sum rounds as numpy's float64 scalars do.
+struct ReplayWalker {
+    using Real = double;
+    static constexpr int MAX_BLOCK = 256;
+    double z, mux, muy, muz, muz2, path;
+    double u_ssa, u_ext, dtau0;
+    double g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, tau_tot, R;
+    int64_t pos, end;
+    uint32_t i;
+    int band;
+    bool bounce, lamb, err;
+    int cond;
+
+    __device__ double draw(const KParams &P)
+    {
+        if (pos >= end) {
+            err = true;
+            return 0.5;
+        }
+        return P.rdev[pos++];
+    }
+
+    __device__ void start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    {
+        (void)photon;
+        const DevCase &C = P.cases[case_id];
+        band = P.photon_band ? P.photon_band[gid] : 0;
+        const DevBand &B = P.bands[C.band_begin + band];
+        g = B.g; ssa_ice = B.ssa_ice; ssa_imp = B.ssa_imp; ext_cff = B.ext_cff;
+        p_ext_imp = B.p_ext_imp;
+        tau_tot = C.tau_tot; R = C.r_lamb; lamb = C.lamb != 0;
+        pos = P.roff[gid];
+        end = P.roff[gid + 1];
+        err = false;
+        dtau0 = -log(draw(P));
+        u_ssa = draw(P);
+        u_ext = draw(P);
+        z = 0.0; mux = C.mux0; muy = 0.0; muz = C.muz0;
+        muz2 = muz * muz;
+        path = 0.0;
+        i = 0;
+        bounce = false;
+        cond = 0;
+    }
+
+    __device__ bool step(const KParams &P)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        double dtau, u_hg = 0.0, u_phi = 0.0;
+        if (i >= 2) {
+            u_hg = draw(P);
+            u_phi = draw(P);
+            dtau = -log(draw(P));
+            u_ssa = draw(P);
+            u_ext = draw(P);
+        } else {
+            dtau = dtau0;
+        }
+        double mxn = mux, myn = muy, mzn = muz;
+        if (i > 1) {
+            double cost, sint;
+            if (bounce) {
+                mux = 0.0; muy = 0.0; muz = 1.0;
+                for (;;) {
+                    const double th = 0.0 + (PI_D / 2) * draw(P);
+                    const double r1 = draw(P);
+                    if (err) { cost = 1.0; sint = 0.0; break; }
+                    if (r1 < 2.0 * sin(th) * cos(th)) {
+                        cost = cos(th);
+                        sint = sqrt(1.0 - cost * cost);
+                        break;
+                    }
+                }
+            } else if (g == 0.0) {
+                cost = 1.0 - 2.0 * u_hg;
+                sint = sqrt(1.0 - cost * cost);
+            } else {
+                const double t = (1.0 - g * g) / ((1.0 - g) + (2.0 * g) * u_hg);
+                cost = (1.0 / (2.0 * g)) * ((1.0 + g * g) - t * t);
+                sint = sqrt(1.0 - cost * cost);
+            }
+            const double ph = u_phi * (2.0 * PI_D);
+            const double cosp = cos(ph), sinp = sin(ph);
+            if (muz == 1.0) {
+                mxn = sint * cosp; myn = sint * sinp; mzn = cost;
+            } else if (muz == -1.0) {
+                mxn = sint * cosp; myn = -sint * sinp; mzn = -cost;
+            } else {
+                const double sq = sqrt(1.0 - muz2);
+                mxn = (sint * (mux * muz * cosp - muy * sinp)) / sq + mux * cost;
+                myn = (sint * (muy * muz * cosp + mux * sinp)) / sq + muy * cost;
+                mzn = -sq * sint * cosp + muz * cost;
+            }
+            bounce = false;
+        }
+        const double z_prev = z;
+        const double z_i = z_prev + dtau * mzn;
+        z = z_i;
+        if (i > 1) {
+            mux = mxn; muy = myn; muz = mzn;
+            muz2 = muz * muz;
+        }
+        path += dtau / ext_cff;
+        const bool ice = u_ext > p_ext_imp;
+        const double ssa_event = ice ? ssa_ice : ssa_imp;
+        if (z_i > 0.0) {
+            path += -((z_i * dtau) / ((z_i - z_prev) * ext_cff));
+            cond = 1;
+            return true;
+        }
+        if (z_i < -tau_tot) {
+            path += -(((z_i + tau_tot) * dtau) / ((z_i - z_prev) * ext_cff));
+            const double dcorr = -(((z_i + tau_tot) / (z_prev - z_i)) * dtau);
+            z = z_i - (mzn * dcorr);
+            const int exit_cond = (i == 1) ? 3 : 2;
+            if (lamb) {
+                if (draw(P) <= R) {
+                    bounce = true;
+                    return err;
+                }
+            }
+            cond = exit_cond;
+            return true;
+        }
+        if (u_ssa >= ssa_event) {
+            cond = ice ? 4 : 5;
+            return true;
+        }
+        return err;
+    }
+
+    __device__ void result(double &theta, double &phi, double &pth, bool &axis) const
+    {
+#pragma clang fp contract(off)
+        theta = acos(muz);
+        pth = path;
+        axis = false;
+        if (i == 1) phi = 0.0;
+        else if (mux > 0 && muy > 0) phi = atan(muy / mux);
+        else if (mux < 0 && muy > 0) phi = atan(muy / mux) + PI_D;
+        else if (mux < 0 && muy < 0) phi = atan(muy / mux) + PI_D;
+        else if (mux > 0 && muy < 0) phi = atan(muy / mux) + 2.0 * PI_D;
+        else { phi = __builtin_nan(""); axis = true; }
+    }
+};
+
+// ------------------------------------------------------------ chunk queue
+// Take chunk `seq` of this block into slot seq % SLOTS (whole wave calls).
+// Returns false when the global queue is empty.
+__device__ bool acquire_chunk(BlockState &bs, uint32_t seq, const KParams &P)
+{
+    const int lane = lane_id();
+    const int slot = seq % SLOTS;
+    unsigned int j = 0;
+    if (lane == 0) j = atomicAdd(P.chunk_counter, 1u);
+    j = (unsigned int)bcast((int)j, 0);
+    if (j >= P.n_chunks) {
+        if (lane == 0) __atomic_store_n(&bs.exhausted, 1, __ATOMIC_RELAXED);
+        return false;
+    }
+    // case of chunk j: last c with case_chunk0[c] <= j
+    int lo = 0, hi = P.n_cases;  // invariant: chunk0[lo] <= j < chunk0[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if ((unsigned int)P.case_chunk0[mid] <= j) lo = mid; else hi = mid;
+    }
+    const DevCase &C = P.cases[lo];
+    const unsigned long long p0 = (unsigned long long)(j - (unsigned int)P.case_chunk0[lo]) * (unsigned)P.chunk;
+    const long long left = C.n_photons - (long long)p0;
+    const int count = (int)(left < P.chunk ? left : P.chunk);
+    SlotTally &T = bs.tal[slot];
+    for (int w = lane; w < TF; w += 64) T.f[w] = 0.0;
+    for (int w = lane; w < TU; w += 64) T.u[w] = 0ull;
+    if (lane == 0) {
+        SlotHdr &H = bs.hdr[slot];
+        H.gid0 = (unsigned long long)C.gid_base + p0;
+        H.p0 = (unsigned int)p0;
+        H.case_id = lo;
+        H.remaining = count;
+        H.rec_next = 0;
+        H.state = 1;
+        __atomic_store_n(&H.seqcount, ((unsigned long long)seq << 32) | (unsigned)count,
+                         __ATOMIC_RELAXED);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    return true;
+}
+
+// Flush slot s to its case's global tallies and free it (whole wave calls).
+__device__ void flush_slot(BlockState &bs, int s, const KParams &P)
+{
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int lane = lane_id();
+    const int cid = bs.hdr[s].case_id;
+    const SlotTally &T = bs.tal[s];
+    double *gf = P.tally_f + (size_t)cid * TF;
+    unsigned long long *gu = P.tally_u + (size_t)cid * TU;
+    for (int w = lane; w < TF; w += 64) {
+        const double v = T.f[w];
+        if (v != 0.0) unsafeAtomicAdd(gf + w, v);
+    }
+    for (int w = lane; w < TU; w += 64) {
+        const unsigned long long v = T.u[w];
+        if (v) atomicAdd(gu + w, v);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __atomic_store_n(&bs.hdr[s].state, 0, __ATOMIC_RELAXED);
+}
+
+template <class W>
+__device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint64_t gid,
+                                             const W &w, typename W::Real th,
+                                             typename W::Real ph, typename W::Real pa)
+{
+    using R = typename W::Real;
+    P.rec_gid[pos] = (uint32_t)gid;
+    P.rec_nscat[pos] = w.i - 1;
+    P.rec_cb[pos] = (uint32_t)w.cond | ((uint32_t)w.band << 8);
+    ((R *)P.rec_theta)[pos] = th;
+    ((R *)P.rec_phi)[pos] = ph;
+    ((R *)P.rec_path)[pos] = pa;
+}
+
+template <class W>
+__global__ void __launch_bounds__(W::MAX_BLOCK) k_walk(const KParams P)
+{
+    __shared__ BlockState bs;
+    const int lane = lane_id();
+    const int wave = threadIdx.x >> 6;
+    // ---- init: zero LDS, wave 0 takes the first chunk
+    {
+        unsigned long long *w64 = reinterpret_cast<unsigned long long *>(&bs);
+        for (int k = threadIdx.x; k < (int)(sizeof(BlockState) / 8); k += blockDim.x) w64[k] = 0ull;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        if (!acquire_chunk(bs, 0u, P)) {
+            if (lane == 0) bs.hdr[0].seqcount = 0ull;  // seq 0, count 0
+        }
+    }
+    __syncthreads();
+
+    W w;
+    bool active = false;
+    int my_slot = 0;
+    uint64_t my_gid = 0;
+    int spins = 0;
+
+    for (;;) {
+        // ---------------- refill idle lanes from the block's chunk
+        unsigned long long idle = __ballot(!active);
+        const int n_idle = __popcll(idle);
+        const bool exhausted = lds_load(&bs.exhausted) != 0;
+        if (n_idle && !exhausted && (n_idle >= (int)P.refill_min || n_idle == 64)) {
+            for (int attempt = 0; attempt < 2; ++attempt) {
+                idle = __ballot(!active);
+                const int n = __popcll(idle);
+                if (n == 0) break;
+                const unsigned long long cur = lds_load(&bs.cur);
+                uint32_t seq = (uint32_t)(cur >> 32);
+                uint32_t off = (uint32_t)cur;
+                unsigned long long meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+                uint32_t count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
+                int got = 0;
+                if (off < count) {
+                    unsigned long long old = 0;
+                    if (lane == 0) old = atomicAdd(&bs.cur, (unsigned long long)n);
+                    old = bcast64(old, 0);
+                    seq = (uint32_t)(old >> 32);
+                    off = (uint32_t)old;
+                    meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+                    count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
+                    got = off < count ? (int)min((uint32_t)n, count - off) : 0;
+                    const int rk = lane_rank(idle);
+                    if (!active && rk < got) {
+                        const int s = seq % SLOTS;
+                        const SlotHdr &H = bs.hdr[s];
+                        const int cid = H.case_id;
+                        const uint32_t photon = H.p0 + off + rk;
+                        my_gid = H.gid0 + off + rk;
+                        my_slot = s;
+                        w.start(P, cid, photon, my_gid);
+                        active = true;
+                    }
+                }
+                if (got == n) break;
+                // current chunk used up: one wave advances the block to the next
+                int lk = 1;
+                if (lane == 0) lk = atomicCAS(&bs.lock, 0, 1);
+                lk = bcast(lk, 0);
+                if (lk != 0) break;
+                bool advanced = false;
+                const unsigned long long now = lds_load(&bs.cur);
+                const uint32_t seq_now = (uint32_t)(now >> 32);
+                if (seq_now == seq && !lds_load(&bs.exhausted)) {
+                    const int ns = (seq_now + 1) % SLOTS;
+                    if (lds_load(&bs.hdr[ns].state) == 0) {
+                        if (acquire_chunk(bs, seq_now + 1, P)) {
+                            if (lane == 0)
+                                __atomic_store_n(&bs.cur, (unsigned long long)(seq_now + 1) << 32,
+                                                 __ATOMIC_RELAXED);
+                            advanced = true;
+                        }
+                    }
+                } else if (seq_now != seq) {
+                    advanced = true;  // someone else advanced: retry the handout
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+                if (lane == 0) __atomic_store_n(&bs.lock, 0, __ATOMIC_RELAXED);
+                if (!advanced) break;
+            }
+        }
+        if (__ballot(active) == 0ull) {
+            if (lds_load(&bs.exhausted)) break;
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > (1 << 30)) break;  // cannot happen: chunks always finish
+            continue;
+        }
+
+        // ---------------- one step of every active photon
+        bool done = false;
+        if (active) {
+            done = w.step(P);
+            if (!done && w.i >= P.max_steps) { w.cond = 0; done = true; }
+        }
+
+        // ---------------- finished photons: record, tally, maybe flush
+        unsigned long long fin = __ballot(done);
+        while (fin) {
+            const int l0 = __ffsll((long long)fin) - 1;
+            const int s = bcast(my_slot, l0);
+            const unsigned long long same = __ballot(done && my_slot == s) & fin;
+            fin &= ~same;
+            const int cnt = __popcll(same);
+            const bool mine = (same >> lane) & 1ull;
+            SlotHdr &H = bs.hdr[s];
+            SlotTally &T = bs.tal[s];
+            int base = 0;
+            if (lane == 0) base = atomicAdd(&H.rec_next, cnt);
+            base = bcast(base, 0);
+            if (mine) {
+                typename W::Real th, ph, pa;
+                bool axis;
+                w.result(th, ph, pa, axis);
+                if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
+                const DevBand &B = P.bands[P.cases[H.case_id].band_begin + w.band];
+                const double wvn = B.wvn;
+                unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
+                atomicAdd(&T.u[U_N], 1ull);
+                atomicAdd(&T.u[U_STEPS], (unsigned long long)w.i);
+                if (w.cond >= 1) {
+                    unsafeAtomicAdd(&T.f[T_QCOND + w.cond - 1], wvn);
+                    atomicAdd(&T.u[w.cond], 1ull);
+                } else {
+                    unsafeAtomicAdd(&T.f[T_QCAP], wvn);
+                    atomicAdd(&T.u[U_CAP], 1ull);
+                }
+                if (axis) atomicAdd(&T.u[U_AXIS], 1ull);
+                if (w.cond == 1) {
+                    const double thd = (double)th, phd = (double)ph;
+                    int jt = (int)(thd * (MCS_THETA_BINS / (PI_D / 2)));
+                    int kp = (int)(phd * (MCS_PHI_BINS / (2 * PI_D)));
+                    jt = jt < 0 ? 0 : (jt > MCS_THETA_BINS - 1 ? MCS_THETA_BINS - 1 : jt);
+                    kp = kp < 0 ? 0 : (kp > MCS_PHI_BINS - 1 ? MCS_PHI_BINS - 1 : kp);
+                    if (phd != phd) kp = 0;
+                    unsafeAtomicAdd(&T.f[jt * MCS_PHI_BINS + kp], wvn);
+                    unsafeAtomicAdd(&T.f[T_QPATH], wvn * (double)pa);
+                }
+                if constexpr (sizeof(typename W::Real) == 8) {
+                    if (w.err || w.pos != w.end) atomicAdd(P.ctl, 1ull);
+                }
+                active = false;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            int old = 0;
+            if (lane == 0) old = atomicSub(&H.remaining, cnt);
+            old = bcast(old, 0);
+            if (old == cnt) flush_slot(bs, s, P);
+        }
+    }
+}
+
+// records: compacted order -> photon order
+template <class R>
+__global__ void k_scatter_records(const uint32_t *gid, const uint32_t *nscat, const uint32_t *cb,
+                                  const R *th, const R *ph, const R *pa, uint64_t n,
+                                  uint32_t *o_nscat, uint32_t *o_cb, R *o_th, R *o_ph, R *o_pa)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t g = gid[j];
+        if (g >= n) continue;
+        o_nscat[g] = nscat[j];
+        o_cb[g] = cb[j];
+        o_th[g] = th[j];
+        o_ph[g] = ph[j];
+        o_pa[g] = pa[j];
+    }
+}
+
+// fill (graph-safe replacement for hipMemsetAsync)
+__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n;
+         j += (uint64_t)gridDim.x * blockDim.x)
+        p[j] = v;
+}
+
+__global__ void k_philox(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out, uint64_t n)
+{
+    const uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const u32x4 r = philox4x32_10(u32x4{ctr[4 * j], ctr[4 * j + 1], ctr[4 * j + 2], ctr[4 * j + 3]}, k0, k1);
+    out[4 * j] = r.x; out[4 * j + 1] = r.y; out[4 * j + 2] = r.z; out[4 * j + 3] = r.w;
+}
+
+}  // namespace mcs
+
+// =================================================================== host
+using namespace mcs;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct mcs_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    char err[512] = {0};
+    mcs_launch launch{256, 4, 2048, 8};
+    // host copies
+    std::vector<mcs_case> cases;
+    std::vector<mcs_band> bands;
+    std::vector<DevCase> dcases;
+    std::vector<DevBand> dbands;
+    int64_t total_photons = 0;
+    bool cases_set = false;
+    // device buffers
+    DevBuf d_cases, d_bands, d_chunk0, d_tally_f, d_tally_u, d_ctl, d_pband, d_rdev, d_roff;
+    DevBuf d_rec[6], d_out[5];
+    int last_mode = MCS_NATIVE;
+    bool last_records = false;
+    int64_t last_n = 0;
+    bool pending = false;
+    mcs_stats stats{};
+    // graph cache
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    uint64_t gkey = 0;
+};
+
+static int set_err(mcs_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                              \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return set_err(ctx, e_ == hipErrorOutOfMemory ? MCS_ERR_NOMEM : MCS_ERR_HIP, \
+                           "%s: %s", #expr, hipGetErrorString(e_));                    \
+    } while (0)
+
+static int ensure(mcs_ctx *c, DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return MCS_OK;
+    if (b.p) {
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(b.p);
+        b.p = nullptr;
+        b.cap = 0;
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess)
+        return set_err(c, MCS_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    b.cap = bytes;
+    c->gkey = 0;  // buffer addresses changed: recapture any graph
+    return MCS_OK;
+}
+
+static uint32_t prob_bits(double p)
+{
+    double t = p * 4294967296.0;
+    if (!(t > 0.0)) return 0u;
+    if (t >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)llround(t);
+}
+
+extern "C" {
+
+int mcs_create(int device, mcs_ctx **out)
+{
+    if (!out) return MCS_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MCS_ERR_HIP;
+    if (device < 0 || device >= n) return MCS_ERR_ARG;
+    mcs_ctx *c = new (std::nothrow) mcs_ctx();
+    if (!c) return MCS_ERR_NOMEM;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) { delete c; return MCS_ERR_HIP; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return MCS_ERR_HIP; }
+    c->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return MCS_ERR_HIP;
+    }
+    *out = c;
+    return MCS_OK;
+}
+
+int mcs_destroy(mcs_ctx *c)
+{
+    if (!c) return MCS_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    DevBuf *all[] = {&c->d_cases, &c->d_bands, &c->d_chunk0, &c->d_tally_f, &c->d_tally_u,
+                     &c->d_ctl, &c->d_pband, &c->d_rdev, &c->d_roff};
+    for (DevBuf *b : all) if (b->p) (void)hipFree(b->p);
+    for (auto &b : c->d_rec) if (b.p) (void)hipFree(b.p);
+    for (auto &b : c->d_out) if (b.p) (void)hipFree(b.p);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MCS_OK;
+}
+
+const char *mcs_last_error(const mcs_ctx *c) { return c ? c->err : "null context"; }
+
+int mcs_device_cus(const mcs_ctx *c, int32_t *out)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    *out = c->cus;
+    return MCS_OK;
+}
+
+int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_band *bands,
+                  int32_t n_bands)
+{
+    if (!c) return MCS_ERR_ARG;
+    if (!cases || !bands || n_cases <= 0 || n_bands <= 0)
+        return set_err(c, MCS_ERR_ARG, "cases/bands must be non-empty");
+    std::vector<DevCase> dc(n_cases);
+    std::vector<DevBand> db(n_bands);
+    for (int b = 0; b < n_bands; ++b) {
+        const mcs_band &B = bands[b];
+        const double v[] = {B.wvn, B.g, B.ssa_ice, B.ssa_imp, B.ext_cff, B.p_ext_imp, B.snow_depth, B.cdf};
+        for (double x : v)
+            if (!std::isfinite(x)) return set_err(c, MCS_ERR_ARG, "band %d: non-finite value", b);
+        if (!(B.g > -1.0 && B.g < 1.0)) return set_err(c, MCS_ERR_ARG, "band %d: |g| must be < 1", b);
+        if (B.ssa_ice < 0 || B.ssa_ice > 1 || B.ssa_imp < 0 || B.ssa_imp > 1)
+            return set_err(c, MCS_ERR_ARG, "band %d: ssa outside [0, 1]", b);
+        if (!(B.ext_cff > 0)) return set_err(c, MCS_ERR_ARG, "band %d: ext_cff must be > 0", b);
+        if (B.p_ext_imp < 0 || B.p_ext_imp > 1) return set_err(c, MCS_ERR_ARG, "band %d: p_ext_imp outside [0, 1]", b);
+        if (B.wvn <= 0) return set_err(c, MCS_ERR_ARG, "band %d: wvn must be > 0", b);
+        DevBand &D = db[b];
+        D.wvn = B.wvn; D.g = B.g; D.ssa_ice = B.ssa_ice; D.ssa_imp = B.ssa_imp;
+        D.ext_cff = B.ext_cff; D.p_ext_imp = B.p_ext_imp; D.cdf = B.cdf;
+        D.iso = (B.g == 0.0);
+        const double g = D.iso ? 0.5 : B.g;
+        D.hg_a = (float)(1.0 / (2.0 * g)); D.hg_b = (float)(1.0 + g * g); D.hg_c = (float)(1.0 - g * g);
+        D.hg_d = (float)(1.0 - g); D.hg_e = (float)(2.0 * g);
+        D.inv_ext = (float)(1.0 / B.ext_cff);
+        D.t_abs_ice = prob_bits(1.0 - B.ssa_ice);
+        D.t_abs_imp = prob_bits(1.0 - B.ssa_imp);
+        D.t_imp = prob_bits(B.p_ext_imp);
+    }
+    int64_t total = 0;
+    for (int k = 0; k < n_cases; ++k) {
+        const mcs_case &C = cases[k];
+        if (!(C.tau_tot > 0) || !std::isfinite(C.tau_tot)) return set_err(c, MCS_ERR_ARG, "case %d: tau_tot must be finite and > 0", k);
+        if (!std::isfinite(C.mux0) || !std::isfinite(C.muz0) || !(C.muz0 < 0) ||
+            std::fabs(C.mux0 * C.mux0 + C.muz0 * C.muz0 - 1.0) > 1e-9)
+            return set_err(c, MCS_ERR_ARG, "case %d: (mux0, muz0) must be a downward unit vector", k);
+        if (!(C.r_lambertian >= 0 && C.r_lambertian <= 1)) return set_err(c, MCS_ERR_ARG, "case %d: r_lambertian outside [0, 1]", k);
+        if (C.n_bands < 1 || C.n_bands > MCS_MAX_BANDS || C.band_begin < 0 || C.band_begin + C.n_bands > n_bands)
+            return set_err(c, MCS_ERR_ARG, "case %d: band range [%d, %d) invalid", k, C.band_begin, C.band_begin + C.n_bands);
+        if (C.n_photons < 0 || C.n_photons > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "case %d: n_photons outside [0, 2^32)", k);
+        DevCase &D = dc[k];
+        D.tau_tot = C.tau_tot; D.mux0 = C.mux0; D.muz0 = C.muz0; D.r_lamb = C.r_lambertian;
+        D.t_refl = (unsigned long long)llround(C.r_lambertian * 4294967296.0);
+        D.tau_tot_f = (float)C.tau_tot; D.mux0_f = (float)C.mux0; D.muz0_f = (float)C.muz0;
+        D.lamb = C.lambertian_bottom ? 1 : 0; D.band_begin = C.band_begin; D.n_bands = C.n_bands;
+        D.n_photons = C.n_photons; D.gid_base = total;
+        total += C.n_photons;
+    }
+    (void)hipSetDevice(c->device);
+    if (c->pending) (void)hipStreamSynchronize(c->stream), c->pending = false;
+    int r;
+    if ((r = ensure(c, c->d_cases, sizeof(DevCase) * n_cases))) return r;
+    if ((r = ensure(c, c->d_bands, sizeof(DevBand) * n_bands))) return r;
+    if ((r = ensure(c, c->d_tally_f, sizeof(double) * TF * n_cases))) return r;
+    if ((r = ensure(c, c->d_tally_u, sizeof(unsigned long long) * TU * n_cases))) return r;
+    if ((r = ensure(c, c->d_chunk0, sizeof(int32_t) * (n_cases + 1)))) return r;
+    if ((r = ensure(c, c->d_ctl, sizeof(unsigned long long) * 16))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->d_cases.p, dc.data(), sizeof(DevCase) * n_cases, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_bands.p, db.data(), sizeof(DevBand) * n_bands, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->cases.assign(cases, cases + n_cases);
+    c->bands.assign(bands, bands + n_bands);
+    c->dcases.swap(dc);
+    c->dbands.swap(db);
+    c->total_photons = total;
+    c->cases_set = true;
+    c->gkey = 0;
+    return MCS_OK;
+}
+
+int mcs_set_launch(mcs_ctx *c, const mcs_launch *l)
+{
+    if (!c || !l) return MCS_ERR_ARG;
+    if (l->block < 64 || l->block > 1024 || l->block % 64 || l->blocks_per_cu < 1 || l->blocks_per_cu > 64 ||
+        l->chunk < 64 || l->refill_min < 1 || l->refill_min > 64)
+        return set_err(c, MCS_ERR_ARG, "launch shape out of range");
+    c->launch = *l;
+    c->gkey = 0;
+    return MCS_OK;
+}
+
+int mcs_get_launch(const mcs_ctx *c, mcs_launch *out)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    *out = c->launch;
+    return MCS_OK;
+}
+
+static int finish_walk(mcs_ctx *c);
+
+static uint64_t mix64(uint64_t h, uint64_t v)
+{
+    h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+    return h;
+}
+
+// enqueue fill + walk + scatter on the context stream
+static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunks, int64_t n, bool records)
+{
+    const int nc = (int)c->cases.size();
+    const unsigned blocks_fill = 256;
+    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_fill), dim3(256), 0, c->stream,
+                       (unsigned long long *)c->d_tally_f.p, (uint64_t)TF * nc, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, c->stream,
+                       (unsigned long long *)c->d_tally_u.p, (uint64_t)TU * nc, 0ull);
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream,
+                       (unsigned long long *)c->d_ctl.p, (uint64_t)16, 0ull);
+    const int block = (mode == MCS_REPLAY) ? std::min(c->launch.block, ReplayWalker::MAX_BLOCK)
+                                           : c->launch.block;
+    int grid = c->launch.blocks_per_cu * c->cus;
+    if ((uint32_t)grid > n_chunks) grid = (int)std::max<uint32_t>(n_chunks, 1u);
+    if (mode == MCS_NATIVE)
+        hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), 0, c->stream, P);
+    else
+        hipLaunchKernelGGL(k_walk<ReplayWalker>, dim3(grid), dim3(block), 0, c->stream, P);
+    if (records && n > 0) {
+        const unsigned g2 = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+        if (mode == MCS_NATIVE)
+            hipLaunchKernelGGL(k_scatter_records<float>, dim3(g2), dim3(256), 0, c->stream,
+                               (uint32_t *)c->d_rec[0].p, (uint32_t *)c->d_rec[1].p, (uint32_t *)c->d_rec[2].p,
+                               (float *)c->d_rec[3].p, (float *)c->d_rec[4].p, (float *)c->d_rec[5].p, (uint64_t)n,
+                               (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (float *)c->d_out[2].p,
+                               (float *)c->d_out[3].p, (float *)c->d_out[4].p);
+        else
+            hipLaunchKernelGGL(k_scatter_records<double>, dim3(g2), dim3(256), 0, c->stream,
+                               (uint32_t *)c->d_rec[0].p, (uint32_t *)c->d_rec[1].p, (uint32_t *)c->d_rec[2].p,
+                               (double *)c->d_rec[3].p, (double *)c->d_rec[4].p, (double *)c->d_rec[5].p, (uint64_t)n,
+                               (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (double *)c->d_out[2].p,
+                               (double *)c->d_out[3].p, (double *)c->d_out[4].p);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, MCS_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
+    return MCS_OK;
+}
+
+int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
+{
+    if (!c || !a) return MCS_ERR_ARG;
+    if (!c->cases_set) return set_err(c, MCS_ERR_STATE, "mcs_set_cases must come first");
+    if (a->mode != MCS_NATIVE && a->mode != MCS_REPLAY) return set_err(c, MCS_ERR_ARG, "unknown mode %d", a->mode);
+    if (a->max_steps < 0 || a->max_steps > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "max_steps outside [0, 2^32)");
+    (void)hipSetDevice(c->device);
+    if (c->pending) {
+        int r = finish_walk(c);
+        if (r) return r;
+    }
+    const int64_t n = c->total_photons;
+    const bool records = a->want_records != 0;
+    if (records && n > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "records need fewer than 2^32 photons");
+    if (a->mode == MCS_REPLAY) {
+        if (!a->replay_dev || !a->replay_off || !a->photon_band)
+            return set_err(c, MCS_ERR_ARG, "replay needs replay_dev, replay_off and photon_band");
+    }
+    const int nc = (int)c->cases.size();
+    // photon bands: validate against each case's band count
+    if (a->photon_band) {
+        for (int k = 0; k < nc; ++k) {
+            const DevCase &D = c->dcases[k];
+            for (int64_t j = 0; j < D.n_photons; ++j)
+                if (a->photon_band[D.gid_base + j] >= D.n_bands)
+                    return set_err(c, MCS_ERR_ARG, "photon %lld: band %d >= n_bands %d of case %d",
+                                   (long long)(D.gid_base + j), a->photon_band[D.gid_base + j], D.n_bands, k);
+        }
+    }
+    int64_t rlen = 0;
+    if (a->mode == MCS_REPLAY) {
+        if (a->replay_off[0] != 0) return set_err(c, MCS_ERR_ARG, "replay_off[0] must be 0");
+        for (int64_t j = 0; j < n; ++j)
+            if (a->replay_off[j + 1] < a->replay_off[j] + 3)
+                return set_err(c, MCS_ERR_ARG, "replay slice %lld shorter than the initial triple", (long long)j);
+        rlen = a->replay_off[n];
+    }
+    // chunking
+    const int chunk = c->launch.chunk;
+    std::vector<int32_t> chunk0(nc + 1);
+    int64_t nch = 0;
+    for (int k = 0; k < nc; ++k) {
+        chunk0[k] = (int32_t)nch;
+        nch += (c->dcases[k].n_photons + chunk - 1) / chunk;
+        if (nch > 0x7FFFFFFF) return set_err(c, MCS_ERR_ARG, "too many chunks; raise the chunk size");
+    }
+    chunk0[nc] = (int32_t)nch;
+    int r;
+    HIPCHK(c, hipMemcpyAsync(c->d_chunk0.p, chunk0.data(), sizeof(int32_t) * (nc + 1), hipMemcpyHostToDevice, c->stream));
+    if (a->photon_band) {
+        if ((r = ensure(c, c->d_pband, (size_t)n))) return r;
+        HIPCHK(c, hipMemcpyAsync(c->d_pband.p, a->photon_band, (size_t)n, hipMemcpyHostToDevice, c->stream));
+    }
+    if (a->mode == MCS_REPLAY) {
+        if ((r = ensure(c, c->d_rdev, sizeof(double) * (size_t)rlen))) return r;
+        if ((r = ensure(c, c->d_roff, sizeof(int64_t) * (size_t)(n + 1)))) return r;
+        HIPCHK(c, hipMemcpyAsync(c->d_rdev.p, a->replay_dev, sizeof(double) * (size_t)rlen, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_roff.p, a->replay_off, sizeof(int64_t) * (size_t)(n + 1), hipMemcpyHostToDevice, c->stream));
+    }
+    const size_t rsz = (a->mode == MCS_NATIVE) ? 4 : 8;
+    if (records) {
+        for (int k = 0; k < 3; ++k) if ((r = ensure(c, c->d_rec[k], 4 * (size_t)n))) return r;
+        for (int k = 3; k < 6; ++k) if ((r = ensure(c, c->d_rec[k], rsz * (size_t)n))) return r;
+        for (int k = 0; k < 2; ++k) if ((r = ensure(c, c->d_out[k], 4 * (size_t)n))) return r;
+        for (int k = 2; k < 5; ++k) if ((r = ensure(c, c->d_out[k], rsz * (size_t)n))) return r;
+    }
+    KParams P;
+    memset(&P, 0, sizeof P);
+    P.cases = (const DevCase *)c->d_cases.p;
+    P.bands = (const DevBand *)c->d_bands.p;
+    P.case_chunk0 = (const int32_t *)c->d_chunk0.p;
+    P.n_cases = nc;
+    P.chunk = chunk;
+    P.n_chunks = (uint32_t)nch;
+    P.refill_min = (uint32_t)c->launch.refill_min;
+    P.key0 = (uint32_t)a->seed;
+    P.key1 = (uint32_t)(a->seed >> 32);
+    P.max_steps = a->max_steps ? (uint32_t)a->max_steps : (1u << 24);
+    P.photon_band = a->photon_band ? (const uint8_t *)c->d_pband.p : nullptr;
+    P.rdev = (const double *)c->d_rdev.p;
+    P.roff = (const int64_t *)c->d_roff.p;
+    P.tally_f = (double *)c->d_tally_f.p;
+    P.tally_u = (unsigned long long *)c->d_tally_u.p;
+    P.ctl = (unsigned long long *)c->d_ctl.p;
+    P.chunk_counter = (unsigned int *)((unsigned long long *)c->d_ctl.p + 8);
+    if (records) {
+        P.rec_gid = (uint32_t *)c->d_rec[0].p;
+        P.rec_nscat = (uint32_t *)c->d_rec[1].p;
+        P.rec_cb = (uint32_t *)c->d_rec[2].p;
+        P.rec_theta = c->d_rec[3].p;
+        P.rec_phi = c->d_rec[4].p;
+        P.rec_path = c->d_rec[5].p;
+    }
+    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+    if (a->use_graph) {
+        uint64_t key = 0x1234;
+        const uint64_t *w = (const uint64_t *)&P;
+        for (size_t k = 0; k < sizeof(P) / 8; ++k) key = mix64(key, w[k]);
+        key = mix64(key, (uint64_t)a->mode);
+        key = mix64(key, (uint64_t)records);
+        key = mix64(key, (uint64_t)n);
+        key = mix64(key, (uint64_t)c->launch.block * 131 + c->launch.blocks_per_cu);
+        if (!c->gexec || c->gkey != key) {
+            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            int er = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records);
+            hipGraph_t g = nullptr;
+            hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            if (er) { if (g) (void)hipGraphDestroy(g); return er; }
+            if (ee != hipSuccess) return set_err(c, MCS_ERR_HIP, "graph capture: %s", hipGetErrorString(ee));
+            c->graph = g;
+            HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+            c->gkey = key;
+        }
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    } else {
+        if ((r = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records))) return r;
+    }
+    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    c->last_mode = a->mode;
+    c->last_records = records;
+    c->last_n = n;
+    c->pending = true;
+    if (a->async) return MCS_OK;
+    return finish_walk(c);
+}
+
+static int finish_walk(mcs_ctx *c)
+{
+    c->pending = false;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    const int nc = (int)c->cases.size();
+    std::vector<unsigned long long> u((size_t)TU * nc), ctl(16);
+    HIPCHK(c, hipMemcpy(u.data(), c->d_tally_u.p, sizeof(unsigned long long) * TU * nc, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ctl.data(), c->d_ctl.p, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost));
+    mcs_stats s{};
+    for (int k = 0; k < nc; ++k) {
+        s.photons += u[(size_t)k * TU + U_N];
+        s.steps += u[(size_t)k * TU + U_STEPS];
+        s.capped += u[(size_t)k * TU + U_CAP];
+        s.phi_unbound += u[(size_t)k * TU + U_AXIS];
+    }
+    s.kernel_ms = ms;
+    c->stats = s;
+    if (s.photons != (uint64_t)c->last_n)
+        return set_err(c, MCS_ERR_HIP, "walk finished %llu of %lld photons",
+                       (unsigned long long)s.photons, (long long)c->last_n);
+    if (c->last_mode == MCS_REPLAY && ctl[0])
+        return set_err(c, MCS_ERR_REPLAY, "%llu photons did not consume exactly their replay slice",
+                       (unsigned long long)ctl[0]);
+    if (s.capped) return set_err(c, MCS_ERR_STEP_CAP, "%llu photons reached max_steps", (unsigned long long)s.capped);
+    if (c->last_mode == MCS_REPLAY && s.phi_unbound)
+        return set_err(c, MCS_ERR_PHI_UNBOUND,
+                       "%llu photons end on an axis: the reference leaves phi_n unbound there",
+                       (unsigned long long)s.phi_unbound);
+    return MCS_OK;
+}
+
+int mcs_sync(mcs_ctx *c)
+{
+    if (!c) return MCS_ERR_ARG;
+    if (!c->pending) return MCS_OK;
+    (void)hipSetDevice(c->device);
+    return finish_walk(c);
+}
+
+int mcs_read_records(mcs_ctx *c, const mcs_records *o, int64_t n)
+{
+    if (!c || !o) return MCS_ERR_ARG;
+    if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (!c->last_records) return set_err(c, MCS_ERR_STATE, "last walk kept no records");
+    if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
+    (void)hipSetDevice(c->device);
+    std::vector<uint32_t> nscat(n), cb(n);
+    HIPCHK(c, hipMemcpy(nscat.data(), c->d_out[0].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(cb.data(), c->d_out[1].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+    for (int64_t j = 0; j < n; ++j) {
+        if (o->n_scat) o->n_scat[j] = nscat[j];
+        if (o->condition) o->condition[j] = (int8_t)(cb[j] & 0xff);
+        if (o->band) o->band[j] = (uint8_t)(cb[j] >> 8);
+    }
+    double *dst[3] = {o->theta_n, o->phi_n, o->path_length};
+    if (c->last_mode == MCS_REPLAY) {
+        for (int k = 0; k < 3; ++k)
+            if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], c->d_out[2 + k].p, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        for (int k = 0; k < 3; ++k) {
+            if (!dst[k]) continue;
+            HIPCHK(c, hipMemcpy(tmp.data(), c->d_out[2 + k].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j) dst[k][j] = tmp[j];
+        }
+    }
+    return MCS_OK;
+}
+
+int mcs_read_tallies(mcs_ctx *c, double *out, int32_t n_cases)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (n_cases != (int32_t)c->cases.size()) return set_err(c, MCS_ERR_ARG, "n_cases mismatch");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpy(out, c->d_tally_f.p, sizeof(double) * TF * n_cases, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+int mcs_read_counts(mcs_ctx *c, uint64_t *out, int32_t n_cases)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (n_cases != (int32_t)c->cases.size()) return set_err(c, MCS_ERR_ARG, "n_cases mismatch");
+    (void)hipSetDevice(c->device);
+    HIPCHK(c, hipMemcpy(out, c->d_tally_u.p, sizeof(uint64_t) * TU * n_cases, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+int mcs_last_stats(const mcs_ctx *c, mcs_stats *out)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    *out = c->stats;
+    return MCS_OK;
+}
+
+int mcs_autotune(mcs_ctx *c, const mcs_launch *cand, int32_t n, int64_t photons_cap, int32_t repeats,
+                 uint64_t seed, double *times_ms, mcs_launch *best)
+{
+    if (!c || !cand || n <= 0 || repeats <= 0 || !best) return MCS_ERR_ARG;
+    if (!c->cases_set) return set_err(c, MCS_ERR_STATE, "mcs_set_cases must come first");
+    // a capped copy of the current cases
+    std::vector<mcs_case> saved = c->cases;
+    std::vector<mcs_band> bands = c->bands;
+    std::vector<mcs_case> capped = saved;
+    if (photons_cap > 0)
+        for (auto &k : capped) k.n_photons = std::min<int64_t>(k.n_photons, photons_cap);
+    const mcs_launch keep = c->launch;
+    int r = mcs_set_cases(c, capped.data(), (int32_t)capped.size(), bands.data(), (int32_t)bands.size());
+    if (r) return r;
+    double best_t = 1e300;
+    int best_i = -1;
+    for (int k = 0; k < n; ++k) {
+        if ((r = mcs_set_launch(c, &cand[k]))) break;
+        mcs_walk_args a;
+        memset(&a, 0, sizeof a);
+        a.mode = MCS_NATIVE;
+        a.seed = seed;
+        if ((r = mcs_walk(c, &a))) break;  // warm-up
+        std::vector<double> t;
+        for (int q = 0; q < repeats; ++q) {
+            if ((r = mcs_walk(c, &a))) break;
+            t.push_back(c->stats.kernel_ms);
+        }
+        if (r) break;
+        std::sort(t.begin(), t.end());
+        const double med = t[t.size() / 2];
+        if (times_ms) times_ms[k] = med;
+        if (med < best_t) { best_t = med; best_i = k; }
+    }
+    int r2 = mcs_set_cases(c, saved.data(), (int32_t)saved.size(), bands.data(), (int32_t)bands.size());
+    if (r) { c->launch = keep; return r; }
+    if (r2) return r2;
+    *best = cand[best_i];
+    c->launch = cand[best_i];
+    return MCS_OK;
+}
+
+// Diagnostic: Philox4x32-10 of n counters (4 x u32 each) with key (k0, k1),
+// computed on the device (checks the device generator against a host one).
+int mcs_philox(mcs_ctx *c, const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out, int64_t n)
+{
+    if (!c || !ctr || !out || n <= 0) return MCS_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    void *dc = nullptr, *dout = nullptr;
+    HIPCHK(c, hipMalloc(&dc, 16 * (size_t)n));
+    hipError_t e = hipMalloc(&dout, 16 * (size_t)n);
+    if (e != hipSuccess) { (void)hipFree(dc); return set_err(c, MCS_ERR_NOMEM, "hipMalloc"); }
+    e = hipMemcpy(dc, ctr, 16 * (size_t)n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_philox, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream,
+                           (const uint32_t *)dc, k0, k1, (uint32_t *)dout, (uint64_t)n);
+        e = hipStreamSynchronize(c->stream);
+    }
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 16 * (size_t)n, hipMemcpyDeviceToHost);
+    (void)hipFree(dc);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return set_err(c, MCS_ERR_HIP, "mcs_philox: %s", hipGetErrorString(e));
+    return MCS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,206 @@
+#!/usr/bin/env python
+"""Write golden per-photon records by running the reference's own
+``MonteCarlo.run`` (monte_carloMPI/monte_carlo3D.py:1492-1648) in THIS
+container.  Nothing here runs on a GPU box; the fixtures it writes
+(``golden_records.npz``) are data: inputs (case parameters, SSP tables, seed)
+and the records the reference printed.
+
+Why an import is needed: no closed form pins the exact per-photon records
+of ``monte_carlo3D`` for a given deviate stream (branch choices on
+``muz_0 == +-1``, correction order, phi_n quadrants, the draw order around
+``normal()`` / ``initial_pdfs``), and the reference ships no tests or
+fixtures.
+
+How the reference is made to run here (nothing of it is copied):
+* ``mpi4py`` is not installed: a one-rank ``COMM_WORLD`` (scatter returns
+  part 0, gather returns ``[answer]``) is written to a temp dir;
+* the SSP NetCDF tarball is absent: small synthetic ``ice_wrn_%04d.nc`` and
+  impurity files are written with scipy.io.netcdf_file;
+* the global numpy stream is seeded with ``np.random.seed(seed)`` before
+  each ``run()``; the reference never seeds it itself.
+
+Usage:  python tests/golden/make_golden.py   (needs /root/reference)
+"""
+import json
+import os
+import re
+import sys
+import tempfile
+
+import numpy as np
+
+REF = '/root/reference'
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+MPI_STUB = '''
+class _Comm(object):
+    size = 1
+    rank = 0
+    def scatter(self, data, root=0):
+        return data[0]
+    def gather(self, data, root=0):
+        return [data]
+class MPI(object):
+    COMM_WORLD = _Comm()
+'''
+
+CONFIG_INI = '''[model parameters]
+tau_tot=1000000.0
+imp_cnc=0e-9
+rho_snw = 300
+rho_ice = 917
+[plot options]
+flg_crt=1
+flg_3D=999
+[data]
+output_dir=out
+optics_dir=optics
+fi_imp=imp.nc
+'''
+
+# spectral grid in metres: 0.205 .. 4.995 um every 0.01 um, so a photon
+# wavelength rounded to 0.01 um sits half way between two grid points
+GRID_M = (0.205 + 0.01 * np.arange(480)) * 1e-6
+
+
+def flat_table(ssa, ext, g):
+    n = GRID_M.size
+    return dict(wvl=GRID_M, ss_alb=np.full(n, ssa), ext_cff_mss=np.full(n, ext),
+                asm_prm=np.full(n, g))
+
+
+def spectral_table():
+    lam = GRID_M * 1e6
+    return dict(wvl=GRID_M,
+                ss_alb=1.0 - 1e-3 * (1.0 + 40.0 * (lam - 1.2) ** 2),
+                ext_cff_mss=16.0 + 0.5 * np.sin(3.0 * lam),
+                asm_prm=0.86 + 0.03 * np.cos(2.0 * lam))
+
+
+def imp_table(ssa, ext):
+    n = GRID_M.size
+    return dict(wvl=GRID_M, ss_alb=np.full(n, ssa), ext_cff_mss=np.full(n, ext))
+
+
+EXT_100 = 3 * 2.0 / (4 * 917.0 * 100e-6)      # Q_ext = 2 spheres, r_eff = 100 um
+
+CASES = [
+    # name, seed, run kwargs, model kwargs, ice table, impurity table
+    dict(name='cfg1_semi_inf', seed=1001, n=120, wvl0=1.3, fwhm=1e-12, rds=100,
+         theta0=60.0, lb=True, R=0.5, tau=4907.306, imp_cnc=0.0,
+         ice=flat_table(0.999, EXT_100, 0.89), imp=imp_table(0.3, 12000.0)),
+    dict(name='vdh_free', seed=1002, n=1500, wvl0=0.5, fwhm=1e-12, rds=100,
+         theta0=0.0, lb=False, R=0.0, tau=2.0, imp_cnc=0.0,
+         ice=flat_table(0.9, EXT_100, 0.75), imp=imp_table(0.3, 12000.0)),
+    dict(name='thin_lamb', seed=1003, n=800, wvl0=1.0, fwhm=1e-12, rds=100,
+         theta0=60.0, lb=True, R=0.5, tau=2.0, imp_cnc=0.0,
+         ice=flat_table(0.99, EXT_100, 0.8), imp=imp_table(0.3, 12000.0)),
+    dict(name='spectral_imp', seed=1004, n=400, wvl0=1.3, fwhm=0.085, rds=250,
+         theta0=30.0, lb=True, R=0.8, tau=10.0, imp_cnc=1e-4,
+         ice=spectral_table(), imp=imp_table(0.3, 12000.0)),
+    dict(name='nadir_bounce', seed=1005, n=250, wvl0=0.9, fwhm=1e-12, rds=100,
+         theta0=0.0, lb=True, R=1.0, tau=1.0, imp_cnc=0.0,
+         ice=flat_table(0.999, EXT_100, 0.85), imp=imp_table(0.3, 12000.0)),
+    dict(name='isotropic', seed=1006, n=400, wvl0=1.1, fwhm=1e-12, rds=100,
+         theta0=45.0, lb=False, R=0.0, tau=5.0, imp_cnc=0.0,
+         ice=flat_table(0.95, EXT_100, 0.0), imp=imp_table(0.3, 12000.0)),
+    dict(name='backward_g', seed=1007, n=400, wvl0=0.7, fwhm=1e-12, rds=100,
+         theta0=75.0, lb=True, R=0.3, tau=3.0, imp_cnc=0.0,
+         ice=flat_table(0.97, EXT_100, -0.3), imp=imp_table(0.3, 12000.0)),
+    dict(name='one_photon', seed=1008, n=1, wvl0=1.3, fwhm=0.085, rds=100,
+         theta0=20.0, lb=True, R=0.5, tau=30.0, imp_cnc=0.0,
+         ice=spectral_table(), imp=imp_table(0.3, 12000.0)),
+]
+
+
+def write_nc(path, table):
+    from scipy.io import netcdf_file
+    f = netcdf_file(path, 'w')
+    f.createDimension('wvl', table['wvl'].size)
+    for key, arr in table.items():
+        v = f.createVariable(key, 'd', ('wvl',))
+        v[:] = arr
+    f.close()
+
+
+_NUM = re.compile(r'np\.float64\(([^)]*)\)')
+_ARR = re.compile(r'array\(\[([^\]]*)\]\)')
+
+
+def parse_output(path):
+    """Rows of the reference CSV.  Under numpy 2 ``%r`` of a float64 prints
+    ``np.float64(x)`` (full precision); with one photon the reference's
+    size-1 SSP path (mc3D:519-564) makes some fields 1-element arrays, printed
+    as ``array([x])`` with numpy's 8-decimal array repr.  Returns the rows and
+    the set of column indices that were printed at that reduced precision."""
+    rows = []
+    lowprec = set()
+    with open(path) as fh:
+        header = fh.readline()
+        for line in fh:
+            fields = line.split(' ')
+            for j, f in enumerate(fields):
+                if _ARR.search(f):
+                    lowprec.add(j)
+            line = _ARR.sub(r'\1', _NUM.sub(r'\1', line))
+            rows.append(line.split())
+    return header, rows, sorted(lowprec)
+
+
+def main():
+    tmp = tempfile.mkdtemp(prefix='mc_golden_')
+    os.makedirs(os.path.join(tmp, 'mpi4py'))
+    with open(os.path.join(tmp, 'mpi4py', '__init__.py'), 'w') as fh:
+        fh.write(MPI_STUB)
+    with open(os.path.join(tmp, 'config.ini'), 'w') as fh:
+        fh.write(CONFIG_INI)
+    os.chdir(tmp)
+    sys.argv = [sys.argv[0]]
+    sys.path[:0] = [tmp, REF]
+    import matplotlib
+    matplotlib.use('Agg')
+    from monte_carloMPI import monte_carlo3D
+
+    store = {}
+    meta = []
+    for c in CASES:
+        optics = os.path.join(tmp, 'optics_' + c['name'])
+        os.makedirs(os.path.join(optics, 'mie', 'snicar'))
+        write_nc(os.path.join(optics, 'mie', 'snicar', 'ice_wrn_%04d.nc' % c['rds']), c['ice'])
+        write_nc(os.path.join(optics, 'mie', 'snicar', 'imp.nc'), c['imp'])
+        outdir = os.path.join(tmp, 'out_' + c['name'])
+        np.random.seed(c['seed'])
+        mc = monte_carlo3D.MonteCarlo(tau_tot=c['tau'], imp_cnc=c['imp_cnc'],
+                                      output_dir=outdir, optics_dir=optics,
+                                      fi_imp='imp.nc')
+        mc.run(c['n'], c['wvl0'], c['fwhm'], c['rds'], theta_0=c['theta0'],
+               Lambertian_bottom=c['lb'], Lambertian_reflectance=c['R'])
+        files = os.listdir(os.path.join(outdir, 'sphere'))
+        assert len(files) == 1, files
+        header, rows, lowprec = parse_output(os.path.join(outdir, 'sphere', files[0]))
+        arr = np.array(rows, dtype=object)
+        p = c['name'] + '/'
+        store[p + 'condition'] = arr[:, 0].astype(np.int64)
+        for j, key in enumerate(['wvn', 'theta_n', 'phi_n'], start=1):
+            store[p + key] = np.array([float(x) for x in arr[:, j]])
+        store[p + 'n_scat'] = arr[:, 4].astype(np.int64)
+        store[p + 'path_length'] = np.array([float(x) for x in arr[:, 5]])
+        store[p + 'snow_depth'] = np.array([float(x) for x in arr[:, 6]])
+        for tname in ('ice', 'imp'):
+            for key, val in c[tname].items():
+                store[p + tname + '_' + key] = np.asarray(val)
+        m = {k: v for k, v in c.items() if k not in ('ice', 'imp')}
+        m['file_name'] = files[0]
+        m['header'] = header
+        m['lowprec_columns'] = lowprec
+        meta.append(m)
+        print('%-14s n=%5d  conditions=%s  file=%s' % (
+            c['name'], c['n'], np.bincount(store[p + 'condition'], minlength=6).tolist(), files[0]))
+
+    np.savez_compressed(os.path.join(HERE, 'golden_records.npz'), **store)
+    with open(os.path.join(HERE, 'golden_cases.json'), 'w') as fh:
+        json.dump(dict(rho_snw=300.0, rho_ice=917.0, cases=meta), fh, indent=1)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,52 @@
+"""C ABI: the library loads without a GPU and exports every function that
+include/*.h declares; the ctypes structs match the header's layout."""
+import ctypes
+import glob
+import os
+import re
+
+import pytest
+
+from monte_carlompi_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, 'include', '*.h')):
+        text = re.sub(r'/\*.*?\*/', '', open(h).read(), flags=re.S)
+        names |= set(re.findall(r'^\s*(?:const\s+)?\w+\s*\*?\s*(mcs_\w+)\s*\(', text, flags=re.M))
+    return names
+
+
+def test_header_declares_the_boundary():
+    names = declared_functions()
+    for n in ('mcs_create', 'mcs_set_cases', 'mcs_walk', 'mcs_read_records', 'mcs_read_tallies'):
+        assert n in names
+
+
+def test_library_exports_every_declared_function():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(declared_functions()) <= set(_lib.EXPORTS) | set()
+
+
+def test_struct_sizes_match_header():
+    assert ctypes.sizeof(_lib.McsCase) == 56
+    assert ctypes.sizeof(_lib.McsBand) == 64
+    assert ctypes.sizeof(_lib.McsWalkArgs) == 56
+    assert ctypes.sizeof(_lib.McsLaunch) == 16
+    assert ctypes.sizeof(_lib.McsRecords) == 48
+    assert ctypes.sizeof(_lib.McsStats) == 40
+
+
+def test_create_without_gpu_fails_cleanly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip('a GPU is present')
+    ctx = ctypes.c_void_p()
+    rc = _lib.lib().mcs_create(0, ctypes.byref(ctx))
+    assert rc != 0 and not ctx.value
+    assert _lib.lib().mcs_destroy(None) == -1
