@@ -1,0 +1,119 @@
+#!/usr/bin/env python
+"""Benchmark of the photon random walk on one MI355X.
+
+Workload (one "step"): one pass of the hot path over one batch -- 1,000,000
+photon packets of the single snowpack case (1.30 um, r_eff 100 um ice
+spheres, Q_ext 2, omega 0.999, g 0.89, 1.0 m of snow at rho 300 kg/m^3,
+theta0 60 deg, Lambertian bottom R 0.5), FP32 walk with Philox4x32-10
+streams, per-photon records (photon order) and FP64 BRF tallies kept in HBM.
+Case tables are resident before the timed region; D2H of records is not
+timed.  Prints one JSON line.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon MC'
+N_PHOTONS = 1_000_000
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=20)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--photons', type=int, default=N_PHOTONS)
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
+    ap.add_argument('--cpu-photons', type=int, default=10_000)
+    ap.add_argument('--cpu-procs', type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
+        print('bench.py measures one GPU: the photon walk is built and measured on a single '
+              'MI355X here (multi-GPU sharding is a design note, not a launched path); '
+              'run with --gpus 1', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from monte_carlompi_amd import roofline
+    from monte_carlompi_amd.gpu import GpuWalker, fractions_from_tallies
+    from monte_carlompi_amd.workloads import SNOW_1M, snow_1m_case
+
+    w = GpuWalker(0)
+    w.set_cases([snow_1m_case(args.photons)])
+    w.set_launch(256, 4, 2048, 8)
+    for k in range(args.warmup):
+        w.walk(mode='native', seed=10_000 + k, records=True)
+    torch.cuda.synchronize()
+    w.sync()
+    kern_ms, steps = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        w.walk(mode='native', seed=k, records=True)   # returns after the walk is checked
+        st = w.stats()
+        kern_ms.append(st['kernel_ms'])
+        steps.append(st['steps'])
+    w.sync()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    fr = fractions_from_tallies(w.tallies())
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    photons_per_s = args.photons / (ms_per_step * 1e-3)
+    avg_kern = sum(kern_ms) / len(kern_ms)
+    avg_steps = sum(steps) / len(steps)
+    rf = roofline.walk_roofline(avg_steps, avg_kern)
+
+    traffic = None
+    pmc = os.path.join(ROOT, 'profiles', 'k_walk_native_snow1m_traffic.json')
+    if os.path.exists(pmc):
+        with open(pmc) as fh:
+            traffic = json.load(fh).get('hbm_bytes_per_launch')
+
+    out = {
+        'metric': METRIC, 'value': round(photons_per_s, 1), 'unit': 'photons/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': round(ms_per_step, 4), 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'fp32', 'data': 'synthetic',
+        'config': {'workload': 'snow_1m single case, 1e6 photons, native Philox FP32 walk '
+                               '(per-photon records + FP64 18x36 BRF tallies in HBM)',
+                   'photons': args.photons, 'wvl_um': SNOW_1M['wvl'], 'r_eff_um': SNOW_1M['r_eff'],
+                   'omega': SNOW_1M['ssa'], 'g': SNOW_1M['g'], 'depth_m': SNOW_1M['depth_m'],
+                   'theta0_deg': SNOW_1M['theta0'], 'parallelism': 'photon-per-lane, 1 GPU'},
+        'photon_steps_per_s': round(avg_steps / (ms_per_step * 1e-3), 1),
+        'brf_cases_per_s': round(1e3 / ms_per_step, 3),
+        'kernel_ms_avg': round(avg_kern, 4),
+        'albedo': round(float(fr['albedo'][0]), 6),
+        'roofline': {
+            'bound': 'valu', 'achieved': round(rf['fp32_tflops'], 4),
+            'peak': roofline.PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
+            'frac': round(rf['fp32_frac'], 5), 'traffic': traffic,
+            'valu_lane_ops_frac': round(rf['valu_frac'], 4),
+            'basis': '48 FP32 flops and 170 VALU lane-ops per photon-step x photon-steps '
+                     'per launch / walk-kernel HIP-event time',
+        },
+    }
+    w.close()
+    if not args.no_cpu:
+        from oracle import cpu_baseline
+        cb = cpu_baseline.run(n_photon=args.cpu_photons, processes=args.cpu_procs or None, seed=1)
+        out['cpu_baseline'] = {
+            'value': round(cb['photons_per_s'], 2), 'unit': 'photons/s', 'cores': cb['processes'],
+            'kind': 'port',
+            'sample': '%d photons of the same case (the reference CPU configuration), %d ranks '
+                      '(array_split scatter / ordered gather) running the oracle restatement of '
+                      'monte_carlo3D with its per-step numpy structure; walk phase %.2f s, '
+                      '%.3g photon-steps/s; CPU: %s' % (cb['photons'], cb['processes'], cb['walk_s'],
+                                                         cb['steps_per_s'], cpu_baseline.cpu_model()),
+        }
+        out['gpu_over_cpu'] = round(photons_per_s / cb['photons_per_s'], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -134,8 +134,10 @@ typedef struct {
     uint64_t capped;           /* photons stopped at max_steps                */
     uint64_t phi_unbound;      /* photons whose final direction lies on an
                                   axis (phi_n from atan2; replay: error)      */
-    double kernel_ms;          /* device time of the last walk (HIP events on
-                                  the context stream)                         */
+    double kernel_ms;          /* device time of the last walk kernel (HIP
+                                  events on the context stream around the
+                                  walk kernel; around the whole graph when
+                                  use_graph is set)                           */
 } mcs_stats;
 
 /* replaces: process start + MonteCarlo() construction on every rank (drv:104) */

@@ -146,6 +146,7 @@ struct NativeWalker {
     unsigned long long t_refl;
     bool iso, lamb, has_imp;
     int cond;
+    PhiloxPre pre;
 
     __device__ void start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
     {
@@ -174,41 +175,43 @@ struct NativeWalker {
         i = 0; p = photon; c = (uint32_t)case_id;
         bounce = false;
         cond = 0;
+        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
     }
 
     // one step; returns true when the photon has left or been absorbed
     __device__ __forceinline__ bool step(const KParams &P)
     {
         ++i;
-        const u32x4 r = philox4x32_10(u32x4{p, c, i, 0u}, P.key0, P.key1);
+        const u32x4 r = philox_step(pre, i, P.key0, P.key1);
         const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(r.z));
-        float mxn = mux, myn = muy, mzn = muz;
+        float mzn = muz;
         if (i > 1) {
             float cost, sint;
-            bool vertical_up = false;
             if (bounce) {
-                // cosine-weighted hemisphere: cos(theta) = sqrt(u), the pdf
+                // cosine-weighted hemisphere: cos(theta) = sqrt(u) has the pdf
                 // 2 sin cos of the rejection loop in mc3D:1242-1250
                 const float u = u01f(r.x);
-                cost = __builtin_sqrtf(u);
-                sint = __builtin_sqrtf(fmaxf(1.f - u, 0.f));
-                vertical_up = true;
-                bounce = false;
-            } else if (iso) {
-                cost = fmaf(-2.f, u01f(r.x), 1.f);
-                sint = __builtin_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+                cost = __builtin_amdgcn_sqrtf(u);
+                sint = __builtin_amdgcn_sqrtf(fmaxf(1.f - u, 0.f));
             } else {
-                const float t = cc / fmaf(e, u01f(r.x), d);
-                cost = fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-                sint = __builtin_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+                const float u = u01f(r.x);
+                if (iso) {
+                    cost = fmaf(-2.f, u, 1.f);
+                } else {
+                    const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u, d));
+                    cost = fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+                }
+                sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
             }
             // v_sin/v_cos take the angle in revolutions: 2 pi u -> u
             const float rev = u01f(r.y);
             const float sinp = __builtin_amdgcn_sinf(rev);
             const float cosp = __builtin_amdgcn_cosf(rev);
             const float s2 = fmaf(-muz, muz, 1.f);
-            if (vertical_up) {
+            float mxn, myn;
+            if (bounce) {
                 mxn = sint * cosp; myn = sint * sinp; mzn = cost;
+                bounce = false;
             } else if (s2 < 1e-7f) {
                 const float sg = muz > 0.f ? 1.f : -1.f;
                 mxn = sint * cosp; myn = sg * sint * sinp; mzn = sg * cost;
@@ -223,11 +226,6 @@ struct NativeWalker {
         }
         z = fmaf(dtau, mzn, z);
         tau_acc += dtau;
-        bool imp = false;
-        if (has_imp) {
-            const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
-            imp = r2.x < t_imp;
-        }
         if (z > 0.f) {
             tau_acc -= fminf(z / mzn, dtau);
             cond = 1;
@@ -246,8 +244,16 @@ struct NativeWalker {
             cond = (i == 1) ? 3 : 2;
             return true;
         }
-        if (r.w < (imp ? t_abs_imp : t_abs_ice)) {
-            cond = imp ? 5 : 4;
+        // species (mc3D:1374-1383) then absorption (mc3D:1461-1466); the
+        // threshold is picked with a mask so no select-of-address is formed
+        uint32_t imp_mask = 0u;
+        if (has_imp) {
+            const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
+            imp_mask = (r2.x < t_imp) ? 0xFFFFFFFFu : 0u;
+        }
+        const uint32_t t_abs = t_abs_ice ^ ((t_abs_ice ^ t_abs_imp) & imp_mask);
+        if (r.w < t_abs) {
+            cond = imp_mask ? 5 : 4;
             return true;
         }
         return false;
@@ -495,6 +501,143 @@ __device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint6
     ((R *)P.rec_path)[pos] = pa;
 }
 
+// Lanes are EMPTY (no photon), LIVE or DONE (finished, record pending).
+enum { EMPTY = 0, LIVE = 1, DONE = 2 };
+
+// Record + tally the DONE lanes of the wave; flush chunks that complete.
+template <class W>
+__device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, int &state,
+                                       int my_slot, uint64_t my_gid)
+{
+    const int lane = lane_id();
+    unsigned long long fin = __ballot(state == DONE);
+    while (fin) {
+        const int l0 = __ffsll((long long)fin) - 1;
+        const int s = bcast(my_slot, l0);
+        const unsigned long long same = __ballot(state == DONE && my_slot == s) & fin;
+        fin &= ~same;
+        const int cnt = __popcll(same);
+        const bool mine = (same >> lane) & 1ull;
+        SlotHdr &H = bs.hdr[s];
+        SlotTally &T = bs.tal[s];
+        int base = 0;
+        if (lane == 0) base = atomicAdd(&H.rec_next, cnt);
+        base = bcast(base, 0);
+        if (mine) {
+            typename W::Real th, ph, pa;
+            bool axis;
+            w.result(th, ph, pa, axis);
+            if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
+            const DevBand &B = P.bands[P.cases[H.case_id].band_begin + w.band];
+            const double wvn = B.wvn;
+            unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
+            atomicAdd(&T.u[U_N], 1ull);
+            atomicAdd(&T.u[U_STEPS], (unsigned long long)w.i);
+            if (w.cond >= 1) {
+                unsafeAtomicAdd(&T.f[T_QCOND + w.cond - 1], wvn);
+                atomicAdd(&T.u[w.cond], 1ull);
+            } else {
+                unsafeAtomicAdd(&T.f[T_QCAP], wvn);
+                atomicAdd(&T.u[U_CAP], 1ull);
+            }
+            if (axis) atomicAdd(&T.u[U_AXIS], 1ull);
+            if (w.cond == 1) {
+                const double thd = (double)th, phd = (double)ph;
+                int jt = (int)(thd * (MCS_THETA_BINS / (PI_D / 2)));
+                int kp = (int)(phd * (MCS_PHI_BINS / (2 * PI_D)));
+                jt = jt < 0 ? 0 : (jt > MCS_THETA_BINS - 1 ? MCS_THETA_BINS - 1 : jt);
+                kp = kp < 0 ? 0 : (kp > MCS_PHI_BINS - 1 ? MCS_PHI_BINS - 1 : kp);
+                if (phd != phd) kp = 0;
+                unsafeAtomicAdd(&T.f[jt * MCS_PHI_BINS + kp], wvn);
+                unsafeAtomicAdd(&T.f[T_QPATH], wvn * (double)pa);
+            }
+            if constexpr (sizeof(typename W::Real) == 8) {
+                if (w.err || w.pos != w.end) atomicAdd(P.ctl, 1ull);
+            }
+            state = EMPTY;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        int old = 0;
+        if (lane == 0) old = atomicSub(&H.remaining, cnt);
+        old = bcast(old, 0);
+        if (old == cnt) flush_slot(bs, s, P);
+    }
+}
+
+// Give photons of the block's current chunk to EMPTY lanes, advancing the
+// block to the next chunk of the global queue when the current one is used up.
+template <class W>
+__device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, int &state,
+                                      int &my_slot, uint64_t &my_gid)
+{
+    const int lane = lane_id();
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        const unsigned long long idle = __ballot(state == EMPTY);
+        const int n = __popcll(idle);
+        if (n == 0) break;
+        const unsigned long long cur = lds_load(&bs.cur);
+        uint32_t seq = (uint32_t)(cur >> 32);
+        uint32_t off = (uint32_t)cur;
+        unsigned long long meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+        uint32_t count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
+        int got = 0;
+        if (off < count) {
+            unsigned long long old = 0;
+            if (lane == 0) old = atomicAdd(&bs.cur, (unsigned long long)n);
+            old = bcast64(old, 0);
+            seq = (uint32_t)(old >> 32);
+            off = (uint32_t)old;
+            meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+            count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
+            got = off < count ? (int)min((uint32_t)n, count - off) : 0;
+            const int rk = lane_rank(idle);
+            if (state == EMPTY && rk < got) {
+                const int s = seq % SLOTS;
+                const SlotHdr &H = bs.hdr[s];
+                const uint32_t photon = H.p0 + off + rk;
+                my_gid = H.gid0 + off + rk;
+                my_slot = s;
+                w.start(P, H.case_id, photon, my_gid);
+                state = LIVE;
+            }
+        }
+        if (got == n) break;
+        // current chunk used up: one wave advances the block to the next
+        int lk = 1;
+        if (lane == 0) lk = atomicCAS(&bs.lock, 0, 1);
+        lk = bcast(lk, 0);
+        if (lk != 0) break;
+        bool advanced = false;
+        const unsigned long long now = lds_load(&bs.cur);
+        const uint32_t seq_now = (uint32_t)(now >> 32);
+        if (seq_now == seq && !lds_load(&bs.exhausted)) {
+            const int ns = (seq_now + 1) % SLOTS;
+            if (lds_load(&bs.hdr[ns].state) == 0) {
+                if (acquire_chunk(bs, seq_now + 1, P)) {
+                    if (lane == 0)
+                        __atomic_store_n(&bs.cur, (unsigned long long)(seq_now + 1) << 32, __ATOMIC_RELAXED);
+                    advanced = true;
+                }
+            }
+        } else if (seq_now != seq) {
+            advanced = true;  // someone else advanced: retry the handout
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        if (lane == 0) __atomic_store_n(&bs.lock, 0, __ATOMIC_RELAXED);
+        if (!advanced) break;
+    }
+}
+
+// Wave issue priority from the oldest live photon: long walks set the tail
+// of a launch, so their waves win issue slots over young ones.
+__device__ __forceinline__ void set_priority(uint32_t max_i)
+{
+    if (max_i > 6000u) __builtin_amdgcn_s_setprio(3);
+    else if (max_i > 3000u) __builtin_amdgcn_s_setprio(2);
+    else if (max_i > 1500u) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
 template <class W>
 __global__ void __launch_bounds__(W::MAX_BLOCK) k_walk(const KParams P)
 {
@@ -515,141 +658,46 @@ __global__ void __launch_bounds__(W::MAX_BLOCK) k_walk(const KParams P)
     __syncthreads();
 
     W w;
-    bool active = false;
+    int state = EMPTY;
     int my_slot = 0;
     uint64_t my_gid = 0;
     int spins = 0;
 
     for (;;) {
-        // ---------------- refill idle lanes from the block's chunk
-        unsigned long long idle = __ballot(!active);
-        const int n_idle = __popcll(idle);
+        // ---------------- cold: retire finished lanes, refill empty ones
+        if (__ballot(state == DONE)) retire(bs, P, w, state, my_slot, my_gid);
         const bool exhausted = lds_load(&bs.exhausted) != 0;
-        if (n_idle && !exhausted && (n_idle >= (int)P.refill_min || n_idle == 64)) {
-            for (int attempt = 0; attempt < 2; ++attempt) {
-                idle = __ballot(!active);
-                const int n = __popcll(idle);
-                if (n == 0) break;
-                const unsigned long long cur = lds_load(&bs.cur);
-                uint32_t seq = (uint32_t)(cur >> 32);
-                uint32_t off = (uint32_t)cur;
-                unsigned long long meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
-                uint32_t count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
-                int got = 0;
-                if (off < count) {
-                    unsigned long long old = 0;
-                    if (lane == 0) old = atomicAdd(&bs.cur, (unsigned long long)n);
-                    old = bcast64(old, 0);
-                    seq = (uint32_t)(old >> 32);
-                    off = (uint32_t)old;
-                    meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
-                    count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
-                    got = off < count ? (int)min((uint32_t)n, count - off) : 0;
-                    const int rk = lane_rank(idle);
-                    if (!active && rk < got) {
-                        const int s = seq % SLOTS;
-                        const SlotHdr &H = bs.hdr[s];
-                        const int cid = H.case_id;
-                        const uint32_t photon = H.p0 + off + rk;
-                        my_gid = H.gid0 + off + rk;
-                        my_slot = s;
-                        w.start(P, cid, photon, my_gid);
-                        active = true;
-                    }
-                }
-                if (got == n) break;
-                // current chunk used up: one wave advances the block to the next
-                int lk = 1;
-                if (lane == 0) lk = atomicCAS(&bs.lock, 0, 1);
-                lk = bcast(lk, 0);
-                if (lk != 0) break;
-                bool advanced = false;
-                const unsigned long long now = lds_load(&bs.cur);
-                const uint32_t seq_now = (uint32_t)(now >> 32);
-                if (seq_now == seq && !lds_load(&bs.exhausted)) {
-                    const int ns = (seq_now + 1) % SLOTS;
-                    if (lds_load(&bs.hdr[ns].state) == 0) {
-                        if (acquire_chunk(bs, seq_now + 1, P)) {
-                            if (lane == 0)
-                                __atomic_store_n(&bs.cur, (unsigned long long)(seq_now + 1) << 32,
-                                                 __ATOMIC_RELAXED);
-                            advanced = true;
-                        }
-                    }
-                } else if (seq_now != seq) {
-                    advanced = true;  // someone else advanced: retry the handout
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-                if (lane == 0) __atomic_store_n(&bs.lock, 0, __ATOMIC_RELAXED);
-                if (!advanced) break;
-            }
-        }
-        if (__ballot(active) == 0ull) {
+        if (!exhausted && __ballot(state == EMPTY)) refill(bs, P, w, state, my_slot, my_gid);
+        const unsigned long long live = __ballot(state == LIVE);
+        if (live == 0ull) {
             if (lds_load(&bs.exhausted)) break;
             __builtin_amdgcn_s_sleep(2);
-            if (++spins > (1 << 30)) break;  // cannot happen: chunks always finish
+            if (++spins > (1 << 28)) break;  // cannot happen: chunks always finish
             continue;
         }
-
-        // ---------------- one step of every active photon
-        bool done = false;
-        if (active) {
-            done = w.step(P);
-            if (!done && w.i >= P.max_steps) { w.cond = 0; done = true; }
-        }
-
-        // ---------------- finished photons: record, tally, maybe flush
-        unsigned long long fin = __ballot(done);
-        while (fin) {
-            const int l0 = __ffsll((long long)fin) - 1;
-            const int s = bcast(my_slot, l0);
-            const unsigned long long same = __ballot(done && my_slot == s) & fin;
-            fin &= ~same;
-            const int cnt = __popcll(same);
-            const bool mine = (same >> lane) & 1ull;
-            SlotHdr &H = bs.hdr[s];
-            SlotTally &T = bs.tal[s];
-            int base = 0;
-            if (lane == 0) base = atomicAdd(&H.rec_next, cnt);
-            base = bcast(base, 0);
-            if (mine) {
-                typename W::Real th, ph, pa;
-                bool axis;
-                w.result(th, ph, pa, axis);
-                if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
-                const DevBand &B = P.bands[P.cases[H.case_id].band_begin + w.band];
-                const double wvn = B.wvn;
-                unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
-                atomicAdd(&T.u[U_N], 1ull);
-                atomicAdd(&T.u[U_STEPS], (unsigned long long)w.i);
-                if (w.cond >= 1) {
-                    unsafeAtomicAdd(&T.f[T_QCOND + w.cond - 1], wvn);
-                    atomicAdd(&T.u[w.cond], 1ull);
-                } else {
-                    unsafeAtomicAdd(&T.f[T_QCAP], wvn);
-                    atomicAdd(&T.u[U_CAP], 1ull);
+        // leave the hot loop once `thresh` lanes are not live
+        const int thresh = lds_load(&bs.exhausted) ? 64 : min(64, (64 - __popcll(live)) + (int)P.refill_min);
+        // ---------------- hot: step every live photon
+        for (uint32_t iter = 1;; ++iter) {
+            if (state == LIVE) {
+                const bool d = w.step(P);
+                if (d || w.i >= P.max_steps) {
+                    if (!d) w.cond = 0;
+                    state = DONE;
                 }
-                if (axis) atomicAdd(&T.u[U_AXIS], 1ull);
-                if (w.cond == 1) {
-                    const double thd = (double)th, phd = (double)ph;
-                    int jt = (int)(thd * (MCS_THETA_BINS / (PI_D / 2)));
-                    int kp = (int)(phd * (MCS_PHI_BINS / (2 * PI_D)));
-                    jt = jt < 0 ? 0 : (jt > MCS_THETA_BINS - 1 ? MCS_THETA_BINS - 1 : jt);
-                    kp = kp < 0 ? 0 : (kp > MCS_PHI_BINS - 1 ? MCS_PHI_BINS - 1 : kp);
-                    if (phd != phd) kp = 0;
-                    unsafeAtomicAdd(&T.f[jt * MCS_PHI_BINS + kp], wvn);
-                    unsafeAtomicAdd(&T.f[T_QPATH], wvn * (double)pa);
-                }
-                if constexpr (sizeof(typename W::Real) == 8) {
-                    if (w.err || w.pos != w.end) atomicAdd(P.ctl, 1ull);
-                }
-                active = false;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            int old = 0;
-            if (lane == 0) old = atomicSub(&H.remaining, cnt);
-            old = bcast(old, 0);
-            if (old == cnt) flush_slot(bs, s, P);
+            const unsigned long long nl = __ballot(state == LIVE);
+            if (nl == 0ull || 64 - __popcll(nl) >= thresh) break;
+            if ((iter & 255u) == 0u) {
+                uint32_t mi = (state == LIVE) ? w.i : 0u;
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 1));
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 2));
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 4));
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 8));
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 16));
+                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 32));
+                set_priority(mi);
+            }
         }
     }
 }
@@ -923,7 +971,8 @@ static uint64_t mix64(uint64_t h, uint64_t v)
 }
 
 // enqueue fill + walk + scatter on the context stream
-static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunks, int64_t n, bool records)
+static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunks, int64_t n, bool records,
+                        bool events)
 {
     const int nc = (int)c->cases.size();
     const unsigned blocks_fill = 256;
@@ -937,10 +986,15 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                                            : c->launch.block;
     int grid = c->launch.blocks_per_cu * c->cus;
     if ((uint32_t)grid > n_chunks) grid = (int)std::max<uint32_t>(n_chunks, 1u);
+    // events bracket the walk kernel alone: kernel_ms is its device time
+    if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
+        return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (mode == MCS_NATIVE)
         hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), 0, c->stream, P);
     else
         hipLaunchKernelGGL(k_walk<ReplayWalker>, dim3(grid), dim3(block), 0, c->stream, P);
+    if (events && hipEventRecord(c->ev1, c->stream) != hipSuccess)
+        return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (records && n > 0) {
         const unsigned g2 = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
         if (mode == MCS_NATIVE)
@@ -1054,7 +1108,6 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
         P.rec_phi = c->d_rec[4].p;
         P.rec_path = c->d_rec[5].p;
     }
-    HIPCHK(c, hipEventRecord(c->ev0, c->stream));
     if (a->use_graph) {
         uint64_t key = 0x1234;
         const uint64_t *w = (const uint64_t *)&P;
@@ -1067,7 +1120,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
             if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
             if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            int er = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records);
+            int er = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records, false);
             hipGraph_t g = nullptr;
             hipError_t ee = hipStreamEndCapture(c->stream, &g);
             if (er) { if (g) (void)hipGraphDestroy(g); return er; }
@@ -1076,11 +1129,13 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
             HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
             c->gkey = key;
         }
+        // a graph launch is timed as a whole (fills + walk + record scatter)
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
         HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     } else {
-        if ((r = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records))) return r;
+        if ((r = enqueue_walk(c, a->mode, P, (uint32_t)nch, n, records, true))) return r;
     }
-    HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     c->last_mode = a->mode;
     c->last_records = records;
     c->last_n = n;
