@@ -68,7 +68,7 @@ typedef struct {
     int64_t n_photons;         /* photon packets of this case (>= 0)          */
 } mcs_case;
 
-#define MCS_MAX_BANDS 64
+#define MCS_MAX_BANDS 256
 
 /* One wavelength bin of a case. */
 typedef struct {

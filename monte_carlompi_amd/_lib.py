@@ -20,7 +20,7 @@ PHI_BINS = 36
 BRF_BINS = THETA_BINS * PHI_BINS
 TALLY_F64 = 656
 TALLY_U64 = 16
-MAX_BANDS = 64
+MAX_BANDS = 256
 
 # names declared in include/mcsnow.h (tests check every one is exported)
 EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs_set_cases',

@@ -1,0 +1,111 @@
+"""Host side of the drop-in (no GPU): record file format and naming, config
+reading, Parallel order, optics interpolation, BRF/albedo estimators."""
+import os
+
+import numpy as np
+import pytest
+
+from monte_carlompi_amd import optics, post_processing
+from monte_carlompi_amd.monte_carlo3D import MonteCarlo
+from monte_carlompi_amd.parallelize import Parallel
+from oracle import walk
+from tests.golden_cases import load
+
+META, CASES = load()
+
+
+def test_record_file_format_roundtrip(tmp_path):
+    g = [c for c in CASES if c['name'] == 'thin_lamb'][0]
+    rec = dict(g['records'])
+    path = tmp_path / 'out.txt'
+    MonteCarlo.write_records(str(path), rec)
+    lines = path.read_text().splitlines()
+    assert lines[0] == 'condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m]'
+    assert len(lines) == 1 + g['n']
+    f = lines[1].split()
+    assert int(f[0]) == rec['condition'][0] and int(f[4]) == rec['n_scat'][0]
+    assert float(f[2]) == rec['theta_n'][0] and repr(float(rec['theta_n'][0])) == f[2]
+    df = post_processing.read_output(str(path))
+    np.testing.assert_array_equal(df['theta_n'].to_numpy(), rec['theta_n'])
+    np.testing.assert_array_equal(df['path_length[m],'].to_numpy(), rec['path_length'])
+    assert list(df.columns)[5] == 'path_length[m],'   # the reference header's stray comma
+
+
+def test_output_naming_matches_reference(tmp_path):
+    for c in CASES:
+        mc = MonteCarlo(output_dir=str(tmp_path / c['name']))
+        mc.theta_0 = (np.pi * c['theta0']) / 180.
+        mc.snow_effective_radius = c['rds']
+        p1 = mc.setup_output(c['n'], c['wvl0'], c['fwhm'])
+        assert os.path.basename(p1) == c['file_name']
+        open(p1, 'w').close()
+        p2 = mc.setup_output(c['n'], c['wvl0'], c['fwhm'])
+        assert os.path.basename(p2) == c['file_name'][:-4] + '_1.txt'
+
+
+def test_config_ini_and_flags(tmp_path, monkeypatch):
+    (tmp_path / 'config.ini').write_text('[model parameters]\ntau_tot=12.5\nimp_cnc=0\nrho_snw=250\n'
+                                         'rho_ice=917\n[data]\noutput_dir=o\noptics_dir=x\nfi_imp=f.nc\n'
+                                         '[gpu]\nmode=replay\nssa_ice=0.95\n')
+    monkeypatch.chdir(tmp_path)
+    monkeypatch.setattr('sys.argv', ['prog', '--rho_snw', '310'])
+    mc = MonteCarlo(imp_cnc=1e-6)
+    assert mc.tau_tot == 12.5 and mc.rho_snw == 310.0 and mc.imp_cnc == 1e-6
+    assert mc.mode == 'replay' and mc._param_ssa_ice == 0.95 and mc.fi_imp == 'f.nc'
+
+
+def test_parallel_split_order():
+    data = np.arange(10)
+    parts = [Parallel(data, size=3, rank=r).working_set for r in range(3)]
+    np.testing.assert_array_equal(np.concatenate(parts), data)
+    assert [p.size for p in parts] == [4, 3, 3]
+    assert Parallel(data).answer_and_reduce([1, 2], MonteCarlo.flatten_list) == [1, 2]
+
+
+@pytest.mark.parametrize('case', [c for c in CASES if c['name'] in ('spectral_imp', 'one_photon')],
+                         ids=lambda c: c['name'])
+def test_spectral_lookup_matches_reference_interpolation(case):
+    t = optics.SpectralTable(case['ice']['wvl'], {k: case['ice'][k] for k in ('ss_alb', 'ext_cff_mss', 'asm_prm')})
+    for w in np.round(np.linspace(1.0, 1.6, 61), 2):
+        for k in ('ss_alb', 'ext_cff_mss', 'asm_prm'):
+            want = float(walk.interp_two_nearest(case['ice']['wvl'], case['ice'][k], w))
+            assert t.lookup(k, w) == want
+    # outside the grid: nearest-value fallback, as the reference's ValueError path
+    assert t.lookup('ss_alb', 0.1) == float(walk.interp_two_nearest(case['ice']['wvl'], case['ice']['ss_alb'], 0.1))
+
+
+def test_netcdf_reader(tmp_path):
+    from scipy.io import netcdf_file
+    case = [c for c in CASES if c['name'] == 'spectral_imp'][0]
+    p = tmp_path / 'ice_wrn_0250.nc'
+    f = netcdf_file(str(p), 'w')
+    f.createDimension('wvl', case['ice']['wvl'].size)
+    for k, v in case['ice'].items():
+        var = f.createVariable(k, 'd', ('wvl',))
+        var[:] = v
+    f.close()
+    t = optics.SpectralTable.from_netcdf(str(p))
+    np.testing.assert_array_equal(t.columns['asm_prm'], case['ice']['asm_prm'])
+
+
+def test_brf_and_albedo_estimators():
+    assert post_processing.calculate_bins(1., 175.) == 137
+    g = [c for c in CASES if c['name'] == 'vdh_free'][0]['records']
+    mid, b = post_processing.brf(g, 90)
+    refl = g['condition'] == 1
+    h, e = np.histogram(g['theta_n'][refl], bins=90, range=(0, np.pi / 2), weights=g['wvn'][refl])
+    m = e[:-1] + np.diff(e) / 2
+    np.testing.assert_allclose(b, h / (g['wvn'].sum() * np.sin(m) * np.cos(m) / np.sum(np.sin(m) * np.cos(m))))
+    # the BRF integrates back to the albedo: sum_k BRF_k w_k = albedo
+    w = np.sin(mid) * np.cos(mid) / np.sum(np.sin(mid) * np.cos(mid))
+    assert abs(np.sum(b * w) - post_processing.albedo(g)) < 1e-12
+    assert post_processing.albedo(g) == walk.calculate_albedo(
+        list(zip(g['condition'], g['wvn'])))
+
+
+def test_unsupported_reference_modes_raise():
+    mc = MonteCarlo()
+    with pytest.raises(NotImplementedError):
+        mc.run(10, 1.3, 0.085, 100, shape='droxtal')
+    with pytest.raises(NotImplementedError):
+        mc.run(10, 1.3, 0.085, 100, Lambertian_surface=True)
