@@ -196,6 +196,62 @@ int mcs_autotune(mcs_ctx *ctx, const mcs_launch *candidates, int32_t n,
 int mcs_philox(mcs_ctx *ctx, const uint32_t *ctr, uint32_t k0, uint32_t k1,
                uint32_t *out, int64_t n);
 
+/* ---------------------------------------------------------------------
+ * Learned surrogate: an MLP n_in -> n_hidden_layers x hidden (SiLU) -> n_out
+ * trained and served with hand-written BF16 MFMA GEMM kernels (FP32
+ * accumulate) and an FP32 Adam.  No reference counterpart: the reference
+ * always runs the Monte Carlo; these entry points serve cases the plain
+ * walk would otherwise have to simulate, and the walk stays the judge.
+ *
+ * Parameter layout (flat float32, unpadded): for layer l = 0..n_hidden_layers
+ * the weight W_l [rows_l x cols_l] row-major then its bias b_l [rows_l];
+ * rows = hidden (hidden layers) or n_out (last), cols = n_in (first layer)
+ * or hidden.  Gradients use the same layout.
+ * ------------------------------------------------------------------- */
+typedef struct mcs_mlp mcs_mlp;
+
+typedef struct {
+    int32_t n_in;              /* inputs, 1..32                               */
+    int32_t hidden;            /* hidden width, multiple of 64 (<= 1024)      */
+    int32_t n_hidden_layers;   /* >= 1                                        */
+    int32_t n_out;             /* outputs, >= 1                               */
+    int32_t max_batch;         /* rows per device pass, multiple of 64        */
+    int32_t reserved;
+} mcs_mlp_dims;
+
+typedef struct {
+    float lr, beta1, beta2, eps, weight_decay;
+    int32_t step;              /* Adam step count before this update (>= 0)   */
+} mcs_adam;
+
+int mcs_mlp_create(int device, const mcs_mlp_dims *dims, mcs_mlp **out);
+int mcs_mlp_destroy(mcs_mlp *mlp);
+const char *mcs_mlp_last_error(const mcs_mlp *mlp);
+/* number of float32 parameters of the layout above */
+int mcs_mlp_param_count(const mcs_mlp *mlp, int64_t *out);
+int mcs_mlp_set_params(mcs_mlp *mlp, const float *params, int64_t n);
+int mcs_mlp_get_params(mcs_mlp *mlp, float *params, int64_t n);
+
+/* y [n x n_out] = MLP(x [n x n_in]); host buffers, any n >= 0 (processed in
+ * max_batch slices).  forward_ms of mcs_mlp_last_ms = device time of the
+ * GEMM / epilogue kernels (HIP events), excluding host copies. */
+int mcs_mlp_forward(mcs_mlp *mlp, const float *x, int64_t n, float *y);
+int mcs_mlp_last_ms(const mcs_mlp *mlp, double *forward_ms);
+
+/* Loss = sum over rows and outputs of w_o (y - t)^2 / n (w: n_out weights).
+ * Gradients of that loss for one batch (n <= max_batch, multiple of 64),
+ * flat layout above; no update.  */
+int mcs_mlp_grads(mcs_mlp *mlp, const float *x, const float *t, const float *w, int32_t n,
+                  float *grads, float *loss);
+
+/* Keep a training set on the device: x [n x n_in], t [n x n_out]. */
+int mcs_mlp_set_dataset(mcs_mlp *mlp, const float *x, const float *t, int64_t n);
+/* n_steps (1..1000) Adam steps; step s trains on rows idx[s*batch .. (s+1)*batch) of
+ * the resident dataset (batch multiple of 64, <= max_batch); w as above;
+ * losses (optional, n_steps) receives each step's loss. */
+int mcs_mlp_train_steps(mcs_mlp *mlp, const int32_t *idx, int32_t n_steps, int32_t batch,
+                        const float *w, const mcs_adam *adam, float *losses);
+
 #ifdef __cplusplus
 }
 #endif

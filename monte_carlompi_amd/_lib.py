@@ -26,8 +26,9 @@ MAX_BANDS = 256
 EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs_set_cases',
            'mcs_set_launch', 'mcs_get_launch', 'mcs_walk', 'mcs_sync', 'mcs_read_records',
            'mcs_read_tallies', 'mcs_read_counts', 'mcs_last_stats', 'mcs_autotune', 'mcs_philox',
-           'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_set_params', 'mcs_mlp_get_params',
-           'mcs_mlp_forward', 'mcs_mlp_train_step', 'mcs_mlp_grads', 'mcs_mlp_last_error']
+           'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_last_error', 'mcs_mlp_param_count',
+           'mcs_mlp_set_params', 'mcs_mlp_get_params', 'mcs_mlp_forward', 'mcs_mlp_last_ms',
+           'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps']
 
 
 class McsCase(ctypes.Structure):
@@ -122,14 +123,14 @@ def lib():
         'mcs_mlp_last_error': ([P], ctypes.c_char_p),
         'mcs_mlp_set_params': ([P, P, ctypes.c_int64], I),
         'mcs_mlp_get_params': ([P, P, ctypes.c_int64], I),
-        'mcs_mlp_forward': ([P, P, ctypes.c_int32, P], I),
-        'mcs_mlp_train_step': ([P, P, P, P, ctypes.c_int32, ctypes.POINTER(McsAdam),
-                                ctypes.POINTER(ctypes.c_float)], I),
+        'mcs_mlp_param_count': ([P, ctypes.POINTER(ctypes.c_int64)], I),
+        'mcs_mlp_forward': ([P, P, ctypes.c_int64, P], I),
+        'mcs_mlp_last_ms': ([P, ctypes.POINTER(ctypes.c_double)], I),
         'mcs_mlp_grads': ([P, P, P, P, ctypes.c_int32, P, ctypes.POINTER(ctypes.c_float)], I),
+        'mcs_mlp_set_dataset': ([P, P, P, ctypes.c_int64], I),
+        'mcs_mlp_train_steps': ([P, P, ctypes.c_int32, ctypes.c_int32, P, ctypes.POINTER(McsAdam), P], I),
     }
     for name, (args, res) in sig.items():
-        if not hasattr(L, name):
-            continue
         f = getattr(L, name)
         f.argtypes = args
         f.restype = res

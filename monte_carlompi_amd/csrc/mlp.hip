@@ -1,0 +1,647 @@
+// mlp.hip -- BF16 MFMA multilayer perceptron (train + serve) for gfx950.
+//
+// GEMMs: 64 x 64 output tiles per 256-thread block (4 waves, each a 32 x 32
+// sub-tile), K staged through LDS 32 at a time, v_mfma_f32_32x32x16_bf16
+// with FP32 accumulation.  Operands are BF16 in HBM; either operand may be
+// stored transposed, so forward (X W^T), input-gradient (dZ W) and
+// weight-gradient (dZ^T A) products are the same kernel.  Epilogues are
+// fused: bias + SiLU (forward, keeping the FP32 pre-activation for the
+// backward pass), SiLU' (backward), or a plain FP32 store.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "mcsnow.h"
+
+namespace mlp {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32, LDSP = BK + 8;  // LDS row pitch (bf16)
+constexpr int K0 = 32;                                    // padded input width
+
+enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3 };
+
+__device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
+
+struct GemmArgs {
+    const bf16 *A;  // logical A [M x K]; stored [M][lda] or (A_T) [K][lda]
+    const bf16 *B;  // logical B [N x K] (C = A B^T); stored [N][ldb] or (B_T) [K][ldb]
+    int M, N, K, lda, ldb;
+    const float *bias;  // [N] or null
+    float *C;           // EPI_F32: C [M][ldc]; EPI_SILU: Z [M][ldc]
+    bf16 *Cb;           // EPI_SILU / EPI_SILU_NOZ: silu(z) [M][ldcb]; EPI_DSILU: dZ [M][ldcb]
+    const float *Zin;   // EPI_DSILU: pre-activation [M][ldc]
+    int ldc, ldcb;
+};
+
+template <bool A_T, bool B_T, int EPI>
+__global__ void __launch_bounds__(256) k_gemm(const GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16 As[BM][LDSP];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[BN][LDSP];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    f32x16 acc;
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        // ---- A tile -> As[m][k]
+        if (!A_T) {
+            const int row = t >> 2, kc = (t & 3) * 8;
+            const int m = m0 + row;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (m < g.M) v = *reinterpret_cast<const uint4 *>(g.A + (size_t)m * g.lda + k0 + kc);
+            *reinterpret_cast<uint4 *>(&As[row][kc]) = v;
+        } else {
+            const int k = t >> 3, mc = (t & 7) * 8;
+            const int m = m0 + mc;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (m + 8 <= g.M) {
+                v = *reinterpret_cast<const uint4 *>(g.A + (size_t)(k0 + k) * g.lda + m);
+            } else if (m < g.M) {
+                bf16 tmp[8];
+                for (int j = 0; j < 8; ++j) tmp[j] = (m + j < g.M) ? g.A[(size_t)(k0 + k) * g.lda + m + j] : (bf16)0.f;
+                memcpy(&v, tmp, 16);
+            }
+            bf16 e[8];
+            memcpy(e, &v, 16);
+            for (int j = 0; j < 8; ++j) As[mc + j][k] = e[j];
+        }
+        // ---- B tile -> Bs[n][k]
+        if (!B_T) {
+            const int row = t >> 2, kc = (t & 3) * 8;
+            const int n = n0 + row;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (n < g.N) v = *reinterpret_cast<const uint4 *>(g.B + (size_t)n * g.ldb + k0 + kc);
+            *reinterpret_cast<uint4 *>(&Bs[row][kc]) = v;
+        } else {
+            const int k = t >> 3, nc = (t & 7) * 8;
+            const int n = n0 + nc;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (n + 8 <= g.N) {
+                v = *reinterpret_cast<const uint4 *>(g.B + (size_t)(k0 + k) * g.ldb + n);
+            } else if (n < g.N) {
+                bf16 tmp[8];
+                for (int j = 0; j < 8; ++j) tmp[j] = (n + j < g.N) ? g.B[(size_t)(k0 + k) * g.ldb + n + j] : (bf16)0.f;
+                memcpy(&v, tmp, 16);
+            }
+            bf16 e[8];
+            memcpy(e, &v, 16);
+            for (int j = 0; j < 8; ++j) Bs[nc + j][k] = e[j];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            // 32x32x16: lane holds row (lane % 32), k = kk + (lane / 32) * 8 .. +8
+            const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&As[wm * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&Bs[wn * 32 + (lane & 31)][kk + (lane >> 5) * 8]);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // ---- epilogue: item r of lane -> row (r/4)*8 + (lane/32)*4 + r%4, col lane%32
+    const int n = n0 + wn * 32 + (lane & 31);
+    if (n >= g.N) return;
+    const float bias = g.bias ? g.bias[n] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int m = m0 + wm * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+        if (m >= g.M) continue;
+        const float v = acc[r] + bias;
+        if (EPI == EPI_F32) {
+            g.C[(size_t)m * g.ldc + n] = v;
+        } else if (EPI == EPI_SILU || EPI == EPI_SILU_NOZ) {
+            if (EPI == EPI_SILU) g.C[(size_t)m * g.ldc + n] = v;
+            g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * sigm(v));
+        } else {  // EPI_DSILU: v is dA; dZ = dA * silu'(z)
+            const float z = g.Zin[(size_t)m * g.ldc + n];
+            const float s = sigm(z);
+            g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * s * (1.f + z * (1.f - s)));
+        }
+    }
+}
+
+// X rows (fp32, n_in wide) -> bf16 [rows][K0], zero padded
+__global__ void k_cast_x(const float *x, int n_in, int rows, bf16 *X)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= rows * K0) return;
+    const int r = i / K0, c = i % K0;
+    X[i] = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
+}
+
+// minibatch gather from the resident dataset
+__global__ void k_gather(const float *xd, const float *td, const int32_t *idx, int n_in, int n_out, int np,
+                         int rows, int64_t n_data, bf16 *X, float *T)
+{
+    const int r = blockIdx.x;
+    if (r >= rows) return;
+    int64_t s = idx[r];
+    if (s < 0 || s >= n_data) s = 0;
+    for (int c = threadIdx.x; c < K0; c += blockDim.x)
+        X[(size_t)r * K0 + c] = (bf16)(c < n_in ? xd[s * n_in + c] : 0.f);
+    for (int c = threadIdx.x; c < np; c += blockDim.x)
+        T[(size_t)r * np + c] = c < n_out ? td[s * n_out + c] : 0.f;
+}
+
+// dY = 2 w (y - t) / rows (bf16) and loss = sum w (y - t)^2 / rows
+__global__ void k_loss(const float *Y, const float *T, const float *w, int rows, int np, bf16 *dY, float *loss)
+{
+    __shared__ float red[256];
+    float acc = 0.f;
+    const float inv = 1.f / rows;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)rows * np;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % np);
+        const float d = Y[i] - T[i];
+        acc += w[c] * d * d;
+        dY[i] = (bf16)(2.f * w[c] * d * inv);
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
+}
+
+// db[c] = sum over rows of dZ[r][c]
+__global__ void k_colsum(const bf16 *dZ, int rows, int cols, int ld, float *db)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.f;
+    for (int r = 0; r < rows; ++r) s += (float)dZ[(size_t)r * ld + c];
+    db[c] = s;
+}
+
+__global__ void k_fill_f32(float *p, size_t n, float v)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Adam (decoupled weight decay) over the padded parameter buffer, refreshing
+// the bf16 copies of weights; mask zeroes updates of padding.
+__global__ void k_adam(float *p, const float *gr, float *m, float *v, const uint8_t *mask, size_t n,
+                       float lr_t, float b1, float b2, float eps, float wd_lr, bf16 *pb, const int64_t *pb_index)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (!mask[i]) continue;
+        const float g = gr[i];
+        const float mi = b1 * m[i] + (1.f - b1) * g;
+        const float vi = b2 * v[i] + (1.f - b2) * g * g;
+        m[i] = mi;
+        v[i] = vi;
+        const float np_ = p[i] - lr_t * mi / (sqrtf(vi) + eps) - wd_lr * p[i];
+        p[i] = np_;
+        const int64_t j = pb_index[i];
+        if (j >= 0) pb[j] = (bf16)np_;
+    }
+}
+
+__global__ void k_copy_out(const float *Y, int rows, int np, int n_out, float *y)
+{
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= (size_t)rows * n_out) return;
+    const int r = (int)(i / n_out), c = (int)(i % n_out);
+    y[i] = Y[(size_t)r * np + c];
+}
+
+}  // namespace mlp
+
+// =================================================================== host
+using namespace mlp;
+
+struct Layer {
+    int rows, cols;         // logical (unpadded) shape
+    int prow, pcol;         // padded device shape
+    size_t w_off, b_off;    // into the padded fp32 buffer
+    size_t wb_off;          // into the bf16 weight buffer
+};
+
+struct mcs_mlp {
+    int device = 0;
+    mcs_mlp_dims d{};
+    int np = 0;  // padded outputs
+    hipStream_t stream = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    char err[512] = {0};
+    std::vector<Layer> layers;
+    size_t n_pad = 0, n_flat = 0, n_wb = 0;
+    float *p = nullptr, *gr = nullptr, *m = nullptr, *v = nullptr;
+    uint8_t *mask = nullptr;
+    int64_t *pb_index = nullptr;
+    bf16 *wb = nullptr;
+    // activations (max_batch rows)
+    bf16 *X = nullptr, *dY = nullptr, *dZa = nullptr, *dZb = nullptr;
+    std::vector<float *> Z;
+    std::vector<bf16 *> A;
+    float *Y = nullptr, *T = nullptr, *w = nullptr, *yout = nullptr, *xin = nullptr, *loss = nullptr;
+    int32_t *idx = nullptr;
+    float *xd = nullptr, *td = nullptr;
+    int64_t n_data = 0;
+    double last_ms = 0.0;
+};
+
+static int merr(mcs_mlp *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(c->err, sizeof c->err, fmt, ap);
+        va_end(ap);
+    }
+    return code;
+}
+
+#define MCHK(ctx, expr)                                                                \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return merr(ctx, e_ == hipErrorOutOfMemory ? MCS_ERR_NOMEM : MCS_ERR_HIP,  \
+                        "%s: %s", #expr, hipGetErrorString(e_));                       \
+    } while (0)
+
+template <class T>
+static hipError_t dalloc(T **p, size_t count)
+{
+    return hipMalloc((void **)p, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+template <bool AT, bool BT, int EPI>
+static void gemm(mcs_mlp *c, const GemmArgs &g)
+{
+    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
+    hipLaunchKernelGGL((k_gemm<AT, BT, EPI>), grid, dim3(256), 0, c->stream, g);
+}
+
+static GemmArgs ga(const bf16 *A, int lda, const bf16 *B, int ldb, int M, int N, int K)
+{
+    GemmArgs g;
+    memset(&g, 0, sizeof g);
+    g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.M = M; g.N = N; g.K = K;
+    return g;
+}
+
+// forward of `rows` rows already in c->X; keep Z for backward when train
+static void forward_pass(mcs_mlp *c, int rows, bool train)
+{
+    const int L = c->d.n_hidden_layers;
+    const bf16 *in = c->X;
+    int ld_in = K0;
+    for (int l = 0; l < L; ++l) {
+        const Layer &ly = c->layers[l];
+        GemmArgs g = ga(in, ld_in, c->wb + ly.wb_off, ly.pcol, rows, ly.prow, ly.pcol);
+        g.bias = c->p + ly.b_off;
+        g.C = c->Z[l]; g.ldc = ly.prow;
+        g.Cb = c->A[l]; g.ldcb = ly.prow;
+        if (train) gemm<false, false, EPI_SILU>(c, g);
+        else gemm<false, false, EPI_SILU_NOZ>(c, g);
+        in = c->A[l];
+        ld_in = ly.prow;
+    }
+    const Layer &lo = c->layers[L];
+    GemmArgs g = ga(in, ld_in, c->wb + lo.wb_off, lo.pcol, rows, lo.prow, lo.pcol);
+    g.bias = c->p + lo.b_off;
+    g.C = c->Y; g.ldc = c->np;
+    gemm<false, false, EPI_F32>(c, g);
+}
+
+// loss + backward for `rows` rows (X, T in place); gradients into c->gr
+static void backward_pass(mcs_mlp *c, int rows)
+{
+    const int L = c->d.n_hidden_layers;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, c->loss, (size_t)1, 0.f);
+    hipLaunchKernelGGL(k_loss, dim3(256), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY, c->loss);
+    // output layer
+    const Layer &lo = c->layers[L];
+    const bf16 *Aprev = c->A[L - 1];
+    {
+        GemmArgs g = ga(c->dY, c->np, Aprev, lo.pcol, lo.prow, lo.pcol, rows);  // dW = dY^T A
+        g.C = c->gr + lo.w_off; g.ldc = lo.pcol;
+        gemm<true, true, EPI_F32>(c, g);
+        hipLaunchKernelGGL(k_colsum, dim3((lo.prow + 255) / 256), dim3(256), 0, c->stream, c->dY, rows, lo.prow,
+                           c->np, c->gr + lo.b_off);
+        GemmArgs h = ga(c->dY, c->np, c->wb + lo.wb_off, lo.pcol, rows, lo.pcol, lo.prow);  // dA = dY W
+        h.Zin = c->Z[L - 1]; h.ldc = lo.pcol;
+        h.Cb = c->dZa; h.ldcb = lo.pcol;
+        gemm<false, true, EPI_DSILU>(c, h);
+    }
+    bf16 *dz = c->dZa, *dzn = c->dZb;
+    for (int l = L - 1; l >= 0; --l) {
+        const Layer &ly = c->layers[l];
+        const bf16 *inp = (l == 0) ? c->X : c->A[l - 1];
+        const int ldi = (l == 0) ? K0 : c->layers[l - 1].prow;
+        GemmArgs g = ga(dz, ly.prow, inp, ldi, ly.prow, ly.pcol, rows);  // dW_l = dZ^T in
+        g.C = c->gr + ly.w_off; g.ldc = ly.pcol;
+        gemm<true, true, EPI_F32>(c, g);
+        hipLaunchKernelGGL(k_colsum, dim3((ly.prow + 255) / 256), dim3(256), 0, c->stream, dz, rows, ly.prow,
+                           ly.prow, c->gr + ly.b_off);
+        if (l > 0) {
+            GemmArgs h = ga(dz, ly.prow, c->wb + ly.wb_off, ly.pcol, rows, ly.pcol, ly.prow);
+            h.Zin = c->Z[l - 1]; h.ldc = ly.pcol;
+            h.Cb = dzn; h.ldcb = ly.pcol;
+            gemm<false, true, EPI_DSILU>(c, h);
+            std::swap(dz, dzn);
+        }
+    }
+}
+
+static int check_dims(const mcs_mlp_dims *d)
+{
+    return d && d->n_in >= 1 && d->n_in <= K0 && d->hidden >= 64 && d->hidden % 64 == 0 && d->hidden <= 1024 &&
+           d->n_hidden_layers >= 1 && d->n_hidden_layers <= 16 && d->n_out >= 1 && d->n_out <= 65536 &&
+           d->max_batch >= 64 && d->max_batch % 64 == 0 && d->max_batch <= (1 << 20);
+}
+
+extern "C" {
+
+int mcs_mlp_create(int device, const mcs_mlp_dims *dims, mcs_mlp **out)
+{
+    if (!out) return MCS_ERR_ARG;
+    *out = nullptr;
+    if (!check_dims(dims)) return MCS_ERR_ARG;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return MCS_ERR_HIP;
+    if (device < 0 || device >= n) return MCS_ERR_ARG;
+    mcs_mlp *c = new (std::nothrow) mcs_mlp();
+    if (!c) return MCS_ERR_NOMEM;
+    c->device = device;
+    c->d = *dims;
+    c->np = (dims->n_out + 63) / 64 * 64;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+        mcs_mlp_destroy(c);
+        return MCS_ERR_HIP;
+    }
+    const int L = dims->n_hidden_layers, H = dims->hidden;
+    size_t off = 0, woff = 0, flat = 0;
+    for (int l = 0; l <= L; ++l) {
+        Layer ly;
+        ly.rows = (l == L) ? dims->n_out : H;
+        ly.cols = (l == 0) ? dims->n_in : H;
+        ly.prow = (l == L) ? c->np : H;
+        ly.pcol = (l == 0) ? K0 : H;
+        ly.w_off = off; off += (size_t)ly.prow * ly.pcol;
+        ly.b_off = off; off += ly.prow;
+        ly.wb_off = woff; woff += (size_t)ly.prow * ly.pcol;
+        flat += (size_t)ly.rows * ly.cols + ly.rows;
+        c->layers.push_back(ly);
+    }
+    c->n_pad = off; c->n_wb = woff; c->n_flat = flat;
+    const size_t MB = dims->max_batch;
+    std::vector<uint8_t> mask(off, 0);
+    std::vector<int64_t> pbi(off, -1);
+    for (const Layer &ly : c->layers) {
+        for (int r = 0; r < ly.rows; ++r) {
+            for (int k = 0; k < ly.cols; ++k) {
+                mask[ly.w_off + (size_t)r * ly.pcol + k] = 1;
+            }
+            mask[ly.b_off + r] = 1;
+        }
+        for (int r = 0; r < ly.prow; ++r)
+            for (int k = 0; k < ly.pcol; ++k) pbi[ly.w_off + (size_t)r * ly.pcol + k] = (int64_t)(ly.wb_off + (size_t)r * ly.pcol + k);
+    }
+    bool ok = dalloc(&c->p, off) == hipSuccess && dalloc(&c->gr, off) == hipSuccess &&
+              dalloc(&c->m, off) == hipSuccess && dalloc(&c->v, off) == hipSuccess &&
+              dalloc(&c->mask, off) == hipSuccess && dalloc(&c->pb_index, off) == hipSuccess &&
+              dalloc(&c->wb, woff) == hipSuccess && dalloc(&c->X, MB * K0) == hipSuccess &&
+              dalloc(&c->dY, MB * c->np) == hipSuccess && dalloc(&c->dZa, MB * H) == hipSuccess &&
+              dalloc(&c->dZb, MB * H) == hipSuccess && dalloc(&c->Y, MB * c->np) == hipSuccess &&
+              dalloc(&c->T, MB * c->np) == hipSuccess && dalloc(&c->w, c->np) == hipSuccess &&
+              dalloc(&c->yout, MB * dims->n_out) == hipSuccess && dalloc(&c->xin, MB * dims->n_in) == hipSuccess &&
+              dalloc(&c->loss, 1024) == hipSuccess && dalloc(&c->idx, MB) == hipSuccess;
+    for (int l = 0; ok && l < L; ++l) {
+        float *z = nullptr;
+        bf16 *a = nullptr;
+        ok = dalloc(&z, MB * H) == hipSuccess && dalloc(&a, MB * H) == hipSuccess;
+        c->Z.push_back(z);
+        c->A.push_back(a);
+    }
+    if (!ok) {
+        mcs_mlp_destroy(c);
+        return MCS_ERR_NOMEM;
+    }
+    if (hipMemcpy(c->mask, mask.data(), off, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c->pb_index, pbi.data(), off * sizeof(int64_t), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemset(c->p, 0, off * 4) != hipSuccess || hipMemset(c->m, 0, off * 4) != hipSuccess ||
+        hipMemset(c->v, 0, off * 4) != hipSuccess || hipMemset(c->gr, 0, off * 4) != hipSuccess ||
+        hipMemset(c->wb, 0, woff * 2) != hipSuccess) {
+        mcs_mlp_destroy(c);
+        return MCS_ERR_HIP;
+    }
+    *out = c;
+    return MCS_OK;
+}
+
+int mcs_mlp_destroy(mcs_mlp *c)
+{
+    if (!c) return MCS_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    void *bufs[] = {c->p, c->gr, c->m, c->v, c->mask, c->pb_index, c->wb, c->X, c->dY, c->dZa, c->dZb,
+                    c->Y, c->T, c->w, c->yout, c->xin, c->loss, c->idx, c->xd, c->td};
+    for (void *b : bufs) if (b) (void)hipFree(b);
+    for (float *z : c->Z) if (z) (void)hipFree(z);
+    for (bf16 *a : c->A) if (a) (void)hipFree(a);
+    if (c->e0) (void)hipEventDestroy(c->e0);
+    if (c->e1) (void)hipEventDestroy(c->e1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return MCS_OK;
+}
+
+const char *mcs_mlp_last_error(const mcs_mlp *c) { return c ? c->err : "null context"; }
+
+int mcs_mlp_param_count(const mcs_mlp *c, int64_t *out)
+{
+    if (!c || !out) return MCS_ERR_ARG;
+    *out = (int64_t)c->n_flat;
+    return MCS_OK;
+}
+
+int mcs_mlp_set_params(mcs_mlp *c, const float *params, int64_t n)
+{
+    if (!c || !params) return MCS_ERR_ARG;
+    if (n != (int64_t)c->n_flat) return merr(c, MCS_ERR_ARG, "expected %zu parameters, got %lld", c->n_flat, (long long)n);
+    for (int64_t i = 0; i < n; ++i)
+        if (!std::isfinite(params[i])) return merr(c, MCS_ERR_ARG, "parameter %lld is not finite", (long long)i);
+    (void)hipSetDevice(c->device);
+    std::vector<float> pad(c->n_pad, 0.f);
+    std::vector<bf16> wb(c->n_wb, (bf16)0.f);
+    size_t k = 0;
+    for (const Layer &ly : c->layers) {
+        for (int r = 0; r < ly.rows; ++r)
+            for (int q = 0; q < ly.cols; ++q) {
+                pad[ly.w_off + (size_t)r * ly.pcol + q] = params[k];
+                wb[ly.wb_off + (size_t)r * ly.pcol + q] = (bf16)params[k];
+                ++k;
+            }
+        for (int r = 0; r < ly.rows; ++r) pad[ly.b_off + r] = params[k++];
+    }
+    MCHK(c, hipMemcpy(c->p, pad.data(), pad.size() * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->wb, wb.data(), wb.size() * 2, hipMemcpyHostToDevice));
+    MCHK(c, hipMemset(c->m, 0, c->n_pad * 4));
+    MCHK(c, hipMemset(c->v, 0, c->n_pad * 4));
+    return MCS_OK;
+}
+
+static void unpad(const mcs_mlp *c, const std::vector<float> &pad, float *out)
+{
+    size_t k = 0;
+    for (const Layer &ly : c->layers) {
+        for (int r = 0; r < ly.rows; ++r)
+            for (int q = 0; q < ly.cols; ++q) out[k++] = pad[ly.w_off + (size_t)r * ly.pcol + q];
+        for (int r = 0; r < ly.rows; ++r) out[k++] = pad[ly.b_off + r];
+    }
+}
+
+int mcs_mlp_get_params(mcs_mlp *c, float *params, int64_t n)
+{
+    if (!c || !params) return MCS_ERR_ARG;
+    if (n != (int64_t)c->n_flat) return merr(c, MCS_ERR_ARG, "expected %zu parameters, got %lld", c->n_flat, (long long)n);
+    (void)hipSetDevice(c->device);
+    std::vector<float> pad(c->n_pad);
+    MCHK(c, hipMemcpy(pad.data(), c->p, pad.size() * 4, hipMemcpyDeviceToHost));
+    unpad(c, pad, params);
+    return MCS_OK;
+}
+
+int mcs_mlp_forward(mcs_mlp *c, const float *x, int64_t n, float *y)
+{
+    if (!c || n < 0 || (n > 0 && (!x || !y))) return MCS_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    const int MB = c->d.max_batch, ni = c->d.n_in, no = c->d.n_out;
+    double total_ms = 0.0;
+    for (int64_t s = 0; s < n; s += MB) {
+        const int rows = (int)std::min<int64_t>(MB, n - s);
+        MCHK(c, hipMemcpyAsync(c->xin, x + s * ni, sizeof(float) * rows * ni, hipMemcpyHostToDevice, c->stream));
+        MCHK(c, hipEventRecord(c->e0, c->stream));
+        hipLaunchKernelGGL(k_cast_x, dim3((rows * K0 + 255) / 256), dim3(256), 0, c->stream, c->xin, ni, rows, c->X);
+        forward_pass(c, rows, false);
+        hipLaunchKernelGGL(k_copy_out, dim3((unsigned)(((size_t)rows * no + 255) / 256)), dim3(256), 0, c->stream,
+                           c->Y, rows, c->np, no, c->yout);
+        MCHK(c, hipEventRecord(c->e1, c->stream));
+        MCHK(c, hipGetLastError());
+        MCHK(c, hipMemcpyAsync(y + s * no, c->yout, sizeof(float) * rows * no, hipMemcpyDeviceToHost, c->stream));
+        MCHK(c, hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        MCHK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
+        total_ms += ms;
+    }
+    c->last_ms = total_ms;
+    return MCS_OK;
+}
+
+int mcs_mlp_last_ms(const mcs_mlp *c, double *ms)
+{
+    if (!c || !ms) return MCS_ERR_ARG;
+    *ms = c->last_ms;
+    return MCS_OK;
+}
+
+static int upload_weights(mcs_mlp *c, const float *w)
+{
+    std::vector<float> wp(c->np, 0.f);
+    for (int k = 0; k < c->d.n_out; ++k) {
+        if (!std::isfinite(w[k]) || w[k] < 0) return merr(c, MCS_ERR_ARG, "output weight %d invalid", k);
+        wp[k] = w[k];
+    }
+    MCHK(c, hipMemcpyAsync(c->w, wp.data(), wp.size() * 4, hipMemcpyHostToDevice, c->stream));
+    MCHK(c, hipStreamSynchronize(c->stream));
+    return MCS_OK;
+}
+
+int mcs_mlp_grads(mcs_mlp *c, const float *x, const float *t, const float *w, int32_t n, float *grads, float *loss)
+{
+    if (!c || !x || !t || !w || !grads) return MCS_ERR_ARG;
+    if (n <= 0 || n % 64 || n > c->d.max_batch) return merr(c, MCS_ERR_ARG, "n must be a positive multiple of 64 <= max_batch");
+    (void)hipSetDevice(c->device);
+    int r = upload_weights(c, w);
+    if (r) return r;
+    const int ni = c->d.n_in, no = c->d.n_out;
+    std::vector<float> tp((size_t)n * c->np, 0.f);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < no; ++k) tp[(size_t)i * c->np + k] = t[(size_t)i * no + k];
+    MCHK(c, hipMemcpy(c->xin, x, sizeof(float) * n * ni, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->T, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_cast_x, dim3((n * K0 + 255) / 256), dim3(256), 0, c->stream, c->xin, ni, n, c->X);
+    forward_pass(c, n, true);
+    backward_pass(c, n);
+    MCHK(c, hipGetLastError());
+    MCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<float> pad(c->n_pad);
+    MCHK(c, hipMemcpy(pad.data(), c->gr, pad.size() * 4, hipMemcpyDeviceToHost));
+    unpad(c, pad, grads);
+    if (loss) MCHK(c, hipMemcpy(loss, c->loss, 4, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+int mcs_mlp_set_dataset(mcs_mlp *c, const float *x, const float *t, int64_t n)
+{
+    if (!c || !x || !t || n <= 0) return MCS_ERR_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->xd) { (void)hipFree(c->xd); c->xd = nullptr; }
+    if (c->td) { (void)hipFree(c->td); c->td = nullptr; }
+    c->n_data = 0;
+    MCHK(c, dalloc(&c->xd, (size_t)n * c->d.n_in));
+    MCHK(c, dalloc(&c->td, (size_t)n * c->d.n_out));
+    MCHK(c, hipMemcpy(c->xd, x, sizeof(float) * n * c->d.n_in, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->td, t, sizeof(float) * n * c->d.n_out, hipMemcpyHostToDevice));
+    c->n_data = n;
+    return MCS_OK;
+}
+
+int mcs_mlp_train_steps(mcs_mlp *c, const int32_t *idx, int32_t n_steps, int32_t batch, const float *w,
+                        const mcs_adam *a, float *losses)
+{
+    if (!c || !idx || !w || !a || n_steps <= 0) return MCS_ERR_ARG;
+    if (batch <= 0 || batch % 64 || batch > c->d.max_batch) return merr(c, MCS_ERR_ARG, "batch must be a positive multiple of 64 <= max_batch");
+    if (!c->xd) return merr(c, MCS_ERR_STATE, "mcs_mlp_set_dataset must come first");
+    if (n_steps > 1000) return merr(c, MCS_ERR_ARG, "at most 1000 steps per call");
+    if (!(a->lr >= 0) || !(a->beta1 >= 0 && a->beta1 < 1) || !(a->beta2 >= 0 && a->beta2 < 1) || !(a->eps > 0) || a->step < 0)
+        return merr(c, MCS_ERR_ARG, "invalid Adam settings");
+    for (int64_t i = 0; i < (int64_t)n_steps * batch; ++i)
+        if (idx[i] < 0 || idx[i] >= c->n_data) return merr(c, MCS_ERR_ARG, "idx[%lld] out of range", (long long)i);
+    (void)hipSetDevice(c->device);
+    int r = upload_weights(c, w);
+    if (r) return r;
+    int32_t *didx = nullptr;
+    MCHK(c, dalloc(&didx, (size_t)n_steps * batch));
+    hipError_t e = hipMemcpy(didx, idx, sizeof(int32_t) * (size_t)n_steps * batch, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { (void)hipFree(didx); return merr(c, MCS_ERR_HIP, "idx upload: %s", hipGetErrorString(e)); }
+    hipLaunchKernelGGL(k_fill_f32, dim3(4), dim3(256), 0, c->stream, c->loss, (size_t)1024, 0.f);
+    for (int s = 0; s < n_steps; ++s) {
+        hipLaunchKernelGGL(k_gather, dim3(batch), dim3(64), 0, c->stream, c->xd, c->td, didx + (size_t)s * batch,
+                           c->d.n_in, c->d.n_out, c->np, batch, c->n_data, c->X, c->T);
+        forward_pass(c, batch, true);
+        backward_pass(c, batch);
+        // k_loss accumulates into loss[0]; keep step s's value in loss[1 + s]
+        (void)hipMemcpyAsync(c->loss + 1 + s, c->loss, 4, hipMemcpyDeviceToDevice, c->stream);
+        const int t = a->step + s + 1;
+        const float lr_t = a->lr * std::sqrt(1.f - std::pow(a->beta2, (float)t)) / (1.f - std::pow(a->beta1, (float)t));
+        hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, c->stream, c->p, c->gr, c->m, c->v, c->mask, c->n_pad,
+                           lr_t, a->beta1, a->beta2, a->eps, a->lr * a->weight_decay, c->wb, c->pb_index);
+    }
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(didx);
+    if (e != hipSuccess) return merr(c, MCS_ERR_HIP, "train: %s", hipGetErrorString(e));
+    if (losses) MCHK(c, hipMemcpy(losses, c->loss + 1, sizeof(float) * n_steps, hipMemcpyDeviceToHost));
+    return MCS_OK;
+}
+
+}  // extern "C"
