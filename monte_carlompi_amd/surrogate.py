@@ -1,0 +1,380 @@
+"""Learned BRF surrogate with the plain Monte Carlo as fallback and judge.
+
+Inputs per case: (r_eff [um], omega, g, theta0 [deg], tau) -- the snowpack
+optical properties the walk consumes.  The network sees eight standardised
+functions of those five numbers (log r_eff, log10(1 - omega), g, cos theta0,
+log10 tau, and three classic combinations of them, see ``features``).
+Outputs (learned in log space so the loss tracks relative error): the
+18 x 36 BRF grid (648 bins, phi-symmetrised: the layer is mirror symmetric
+about the plane of incidence, so bins k and 35 - k estimate the same value),
+albedo, diffuse and direct transmittance -- 651 numbers.
+
+Training data come only from the GPU walk (batched sweeps, Philox streams);
+the network is an MLP 8 -> 4 x 256 (SiLU) -> 651 trained by the BF16-MFMA
+kernels with FP32 Adam.  Serving evaluates the network for every query,
+re-runs a seeded fraction of the queries with the plain walk at high photon
+counts, and falls back to the plain walk for every query in an input region
+where the audited errors exceed the bars (albedo > 1 %, or BRF bins beyond
+3 % once the judge's own Monte Carlo noise is removed).
+
+    python -m monte_carlompi_amd.surrogate --help
+"""
+from __future__ import annotations
+
+import json
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import gpu as gpumod
+from .gpu import BRF_BINS, PHI_BINS, THETA_BINS, Band, Case, GpuWalker
+from .workloads import RHO_SNW, sphere_ext_cff_mss
+
+N_OUT = BRF_BINS + 3
+RANGES = dict(r_eff=(30.0, 1500.0), eps=(1e-5, 5e-2), g=(0.85, 0.90), theta0=(0.0, 70.0),
+              tau=(1.0, 1e4))
+
+
+def sample_training_inputs(n, seed, thin_frac=0.25, thin_tau=10.0):
+    """Training cases: the query distribution of sample_inputs plus an extra
+    `thin_frac` share with tau log-uniform in [1, thin_tau] (thin layers
+    carry the sharpest dependence on tau and theta0)."""
+    base = sample_inputs(n, seed)
+    k = int(round(thin_frac * n))
+    if k:
+        extra = sample_inputs(k, seed + 1)
+        rng = np.random.Generator(np.random.PCG64(seed + 2))
+        extra[:, 4] = np.exp(rng.uniform(0.0, np.log(thin_tau), k))
+        base = np.concatenate([base, extra])
+    return base
+
+
+def sample_inputs(n, seed):
+    """n cases: log-uniform r_eff, omega = 1 - U(1e-5, 5e-2), g ~ U(0.85, 0.90),
+    theta0 ~ U(0, 70) deg, log-uniform tau in [1, 1e4].  Columns:
+    r_eff, omega, g, theta0, tau."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lo, hi = RANGES['r_eff']
+    r = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+    om = 1.0 - rng.uniform(*RANGES['eps'], n)
+    g = rng.uniform(*RANGES['g'], n)
+    th = rng.uniform(*RANGES['theta0'], n)
+    tlo, thi = RANGES['tau']
+    tau = np.exp(rng.uniform(np.log(tlo), np.log(thi), n))
+    return np.stack([r, om, g, th, tau], axis=1)
+
+
+N_FEAT = 9
+
+
+def features(inp):
+    """Standardised network inputs, all functions of (r_eff, omega, g,
+    theta0, tau): log r_eff, log10(1 - omega), g, cos theta0, log10 tau, plus
+    three classic radiative-transfer combinations of them -- the direct-beam
+    transmittance exp(-tau / mu0), the similarity parameter
+    sqrt((1 - omega) / (1 - omega g)), the scaled depth log10(tau (1 - omega g))
+    and the slant depth log10(tau / mu0)."""
+    r, om, g, th, tau = (inp[:, k] for k in range(5))
+    mu0 = np.cos(np.radians(th))
+    f = np.empty((inp.shape[0], N_FEAT), np.float32)
+    f[:, 0] = (np.log(r) - 5.2) / 1.13
+    f[:, 1] = (np.log10(np.maximum(1.0 - om, 1e-12)) + 2.0) / 0.75
+    f[:, 2] = (g - 0.875) / 0.0144
+    f[:, 3] = (mu0 - 0.7) / 0.25
+    f[:, 4] = (np.log10(tau) - 2.0) / 1.15
+    f[:, 5] = (np.exp(-tau / mu0) - 0.05) / 0.15
+    f[:, 6] = (np.sqrt((1.0 - om) / (1.0 - om * g)) - 0.35) / 0.17
+    f[:, 7] = (np.log10(tau * (1.0 - om * g)) - 1.1) / 1.15
+    f[:, 8] = (np.log10(tau / mu0) - 2.15) / 1.15
+    return f
+
+
+# targets are learned in log space so that the loss follows relative error
+OFFSETS = np.concatenate([np.full(BRF_BINS, 1e-2), [1e-4, 1e-3, 1e-6]])
+
+
+def to_log(targets, var=None):
+    """log(target + offset), plus var / (2 (target + offset)^2) when the
+    targets' Monte Carlo variance is given: the delta-method correction that
+    makes the log of a noisy estimate unbiased for the log of its mean to
+    second order (E[log x] ~ log E[x] - var / (2 E[x]^2))."""
+    x = np.maximum(targets, 0.0) + OFFSETS
+    y = np.log(x)
+    if var is not None:
+        y = y + var / (2.0 * x * x)
+    return y
+
+
+def from_log(y):
+    return np.exp(y) - OFFSETS
+
+
+def mc_cases(inp, photons, wvl=1.0):
+    """Plain-walk cases for inputs (free bottom, one wavelength band)."""
+    out = []
+    for r, om, g, th, tau in inp:
+        ext_cff = sphere_ext_cff_mss(r) * RHO_SNW
+        out.append(Case(tau_tot=float(tau), theta0_deg=float(th), n_photons=int(photons),
+                        bands=[Band(wvl=wvl, g=float(g), ssa_ice=float(om), ext_cff=ext_cff)]))
+    return out
+
+
+def symmetrise_brf(brf):
+    """Average the 18 x 36 grid with its mirror phi -> 2 pi - phi."""
+    b = brf.reshape(-1, THETA_BINS, PHI_BINS)
+    return 0.5 * (b + b[:, :, ::-1])
+
+
+def targets_from_tallies(t):
+    brf = symmetrise_brf(gpumod.brf_from_tallies(t)).reshape(-1, BRF_BINS)
+    fr = gpumod.fractions_from_tallies(t)
+    return np.concatenate([brf, fr['albedo'][:, None], fr['t_diffuse'][:, None],
+                           fr['t_direct'][:, None]], axis=1)
+
+
+def target_variance(t, targets, photons):
+    """Monte Carlo variance of each target of targets_from_tallies for cases
+    of `photons` photons of one wavelength: a BRF bin estimated from c
+    reflected photons (pooled over the two mirrored bins) has variance
+    BRF^2 / c; a fraction p has variance p (1 - p) / photons."""
+    H = t[:, :BRF_BINS].reshape(-1, THETA_BINS, PHI_BINS)
+    per_photon = t[:, gpumod.T_QDOWN] / photons               # wvn of the case
+    c = (H + H[:, :, ::-1]) / per_photon[:, None, None]        # photons in the pooled bins
+    c = c.reshape(-1, BRF_BINS)
+    var = np.zeros_like(targets)
+    b = targets[:, :BRF_BINS]
+    var[:, :BRF_BINS] = np.where(c > 0, b * b / np.maximum(c, 1.0), 0.0)
+    p = targets[:, BRF_BINS:]
+    var[:, BRF_BINS:] = np.clip(p, 0, 1) * (1 - np.clip(p, 0, 1)) / photons
+    return var
+
+
+def run_mc(walker, inp, photons, seed, chunk_cases=None, use_graph=True):
+    """Targets, tallies and device seconds of the plain walk for all inputs
+    (launches of at most ~2e9 photons)."""
+    tg, tl = [], []
+    dev_s = 0.0
+    if chunk_cases is None:
+        chunk_cases = int(max(1, min(20000, 2e9 // max(photons, 1))))
+    for s in range(0, inp.shape[0], chunk_cases):
+        part = inp[s:s + chunk_cases]
+        walker.set_cases(mc_cases(part, photons))
+        walker.walk(mode='native', seed=seed + s, records=False, use_graph=use_graph)
+        dev_s += walker.stats()['kernel_ms'] * 1e-3
+        t = walker.tallies()
+        tl.append(t)
+        tg.append(targets_from_tallies(t))
+    return np.concatenate(tg), np.concatenate(tl), dev_s
+
+
+@dataclass
+class Surrogate:
+    hidden: int = 256
+    layers: int = 4
+    device: int = 0
+    max_batch: int = 65536
+    y_mean: np.ndarray = None
+    y_std: np.ndarray = None
+    history: list = field(default_factory=list)
+
+    def __post_init__(self):
+        from .mlp import Mlp
+        self.net = Mlp(N_FEAT, self.hidden, self.layers, N_OUT, max_batch=self.max_batch, device=self.device)
+
+    def out_weights(self):
+        w = np.full(N_OUT, 1.0 / BRF_BINS)
+        w[BRF_BINS:] = (1.0, 0.5, 0.5)
+        return (w / w.sum()).astype(np.float32)
+
+    def fit(self, inp, targets, epochs=200, batch=4096, lr=2e-3, lr_end=2e-5, seed=0,
+            weight_decay=0.0, log_every=0, var=None):
+        """Train on walk targets; `var` (target_variance) bias-corrects the
+        log transform of noisy targets."""
+        x = features(inp)
+        lt = to_log(targets, var)
+        self.y_mean = lt.mean(axis=0)
+        self.y_std = lt.std(axis=0) + 1e-6
+        y = ((lt - self.y_mean) / self.y_std).astype(np.float32)
+        self.net.init_params(seed=seed)
+        self.net.set_dataset(x, y)
+        n = x.shape[0]
+        rng = np.random.default_rng(seed)
+        w = self.out_weights()
+        steps_per_epoch = n // batch
+        total = epochs * steps_per_epoch
+        done = 0
+        t0 = time.perf_counter()
+        for ep in range(epochs):
+            perm = rng.permutation(n)[:steps_per_epoch * batch].astype(np.int32)
+            # cosine schedule, updated per chunk of steps
+            for s in range(0, steps_per_epoch, 256):
+                k = min(256, steps_per_epoch - s)
+                frac = done / max(total, 1)
+                lr_now = lr_end + 0.5 * (lr - lr_end) * (1 + np.cos(np.pi * frac))
+                losses = self.net.train_steps(perm[s * batch:(s + k) * batch], batch, w, lr=lr_now,
+                                              weight_decay=weight_decay)
+                done += k
+            self.history.append(float(losses[-1]))
+            if log_every and (ep % log_every == 0 or ep == epochs - 1):
+                print('epoch %4d loss %.5f lr %.2e (%.1f s)' % (ep, losses[-1], lr_now,
+                                                                time.perf_counter() - t0), flush=True)
+        return self.history
+
+    def predict(self, inp):
+        y = self.net.forward(features(inp))
+        return from_log(y * self.y_std + self.y_mean)
+
+    def predict_ms(self):
+        return self.net.last_ms()
+
+    def save(self, path):
+        np.savez(path, params=self.net.get_params(), y_mean=self.y_mean, y_std=self.y_std,
+                 hidden=self.hidden, layers=self.layers)
+
+    def load(self, path):
+        d = np.load(path, allow_pickle=False)
+        self.y_mean, self.y_std = d['y_mean'], d['y_std']
+        self.net.set_params(d['params'])
+
+
+# ---------------------------------------------------------------- judging
+def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=400):
+    """Per-case albedo relative error and BRF error of the prediction against
+    a Monte Carlo run.  BRF: over bins holding >= min_bin_photons reflected
+    photons (after symmetrisation), the rms relative difference with the
+    judge's own binomial noise removed in quadrature, and the raw median."""
+    alb_err = np.abs(pred[:, BRF_BINS] - mc[:, BRF_BINS]) / np.maximum(mc[:, BRF_BINS], 1e-12)
+    H = symmetrise_brf(mc_tallies[:, :BRF_BINS]) / mc_tallies[:, gpumod.T_QDOWN][:, None, None]
+    nbin = (H * photons * 2).reshape(-1, BRF_BINS)          # two mirrored bins pooled
+    rel = (pred[:, :BRF_BINS] - mc[:, :BRF_BINS]) / np.where(mc[:, :BRF_BINS] > 0, mc[:, :BRF_BINS], 1)
+    noise2 = 1.0 / np.maximum(nbin, 1)
+    ok = nbin >= min_bin_photons
+    rms_corr = np.empty(pred.shape[0])
+    med = np.empty(pred.shape[0])
+    for i in range(pred.shape[0]):
+        m = ok[i]
+        if not m.any():
+            rms_corr[i] = 0.0
+            med[i] = 0.0
+            continue
+        rms_corr[i] = np.sqrt(max(0.0, np.mean(rel[i, m] ** 2) - np.mean(noise2[i, m])))
+        med[i] = np.median(np.abs(rel[i, m]))
+    return dict(albedo=alb_err, brf_rms_excess=rms_corr, brf_median=med)
+
+
+def region_of(inp, n_tau=8, n_eps=6, n_mu=3):
+    """Input-space region: a grid over log10 tau x log10(1 - omega) x cos theta0."""
+    lt = np.log10(inp[:, 4]) / 4.0
+    le = (np.log10(1 - inp[:, 1]) + 5.0) / (5.0 + np.log10(5e-2))
+    lm = (np.cos(np.radians(inp[:, 3])) - np.cos(np.radians(70.0))) / (1.0 - np.cos(np.radians(70.0)))
+    a = np.clip((lt * n_tau).astype(int), 0, n_tau - 1)
+    b = np.clip((le * n_eps).astype(int), 0, n_eps - 1)
+    c = np.clip((lm * n_mu).astype(int), 0, n_mu - 1)
+    return (a * n_eps + b) * n_mu + c
+
+
+def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_photons=1_000_000,
+          seed=0, alb_bar=0.01, brf_bar=0.03, region_fail_frac=0.05, fallback_photons=1_000_000,
+          run_fallback=True):
+    """Surrogate answers for every query, a seeded audit by the plain walk,
+    region-wise fallback.  Returns answers and a report."""
+    t0 = time.perf_counter()
+    pred = sur.predict(queries)
+    t_pred = time.perf_counter() - t0
+    pred_dev_s = sur.predict_ms() * 1e-3
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = queries.shape[0]
+    n_audit = max(1, int(round(audit_frac * n)))
+    aud = np.sort(rng.choice(n, n_audit, replace=False))
+    mc, mct, aud_dev_s = run_mc(walker, queries[aud], audit_photons, seed=seed + 7)
+    err = case_errors(pred[aud], mc, mct, audit_photons)
+    fail = (err['albedo'] > alb_bar) | (err['brf_rms_excess'] > brf_bar)
+    reg_q = region_of(queries)
+    reg_a = reg_q[aud]
+    bad_regions = []
+    for r in np.unique(reg_a):
+        m = reg_a == r
+        if np.mean(fail[m]) > region_fail_frac:
+            bad_regions.append(int(r))
+    fb = np.isin(reg_q, bad_regions)
+    fb[aud] = False
+    answers = pred.copy()
+    answers[aud] = mc                               # audited queries keep the judge's answer
+    fb_dev_s = 0.0
+    if run_fallback and fb.any():
+        fmc, _, fb_dev_s = run_mc(walker, queries[fb], fallback_photons, seed=seed + 99)
+        answers[fb] = fmc
+    dec = np.clip(np.log10(queries[aud, 4]).astype(int), 0, 3)
+    diag = dict(fail_albedo=int(np.sum(err['albedo'] > alb_bar)),
+                fail_brf=int(np.sum(err['brf_rms_excess'] > brf_bar)),
+                fail_frac_by_tau_decade=[float(np.mean(fail[dec == d])) if np.any(dec == d) else None
+                                         for d in range(4)])
+    rep = dict(n_queries=int(n), n_audit=int(n_audit), audit_fail_frac=float(np.mean(fail)),
+               diagnostics=diag,
+               fallback_regions=bad_regions, fallback_frac=float(np.mean(fb)),
+               predict_s_host=t_pred, predict_s_device=pred_dev_s, audit_s_device=aud_dev_s,
+               fallback_s_device=fb_dev_s,
+               albedo_err=dict(median=float(np.median(err['albedo'])),
+                               p95=float(np.quantile(err['albedo'], 0.95)), max=float(err['albedo'].max())),
+               brf_rms_excess=dict(median=float(np.median(err['brf_rms_excess'])),
+                                   p95=float(np.quantile(err['brf_rms_excess'], 0.95))),
+               brf_median_abs=dict(median=float(np.median(err['brf_median']))))
+    rep['cases_per_s_surrogate'] = n / max(pred_dev_s, 1e-12)
+    rep['cases_per_s_served'] = n / max(pred_dev_s + aud_dev_s + fb_dev_s, 1e-12)
+    return answers, rep
+
+
+def plain_mc_rate(walker, inp, photons=1_000_000, seed=5):
+    """Cases/s of the plain walk at `photons` per case (batched, device time)."""
+    _, _, dev_s = run_mc(walker, inp, photons, seed)
+    return inp.shape[0] / dev_s
+
+
+def main():
+    import argparse
+    ap = argparse.ArgumentParser(description='train / serve the BRF surrogate on the GPU walk')
+    ap.add_argument('--train-cases', type=int, default=200_000)
+    ap.add_argument('--train-photons', type=int, default=100_000)
+    ap.add_argument('--epochs', type=int, default=200)
+    ap.add_argument('--thin-frac', type=float, default=0.25)
+    ap.add_argument('--queries', type=int, default=1_000_000)
+    ap.add_argument('--audit-frac', type=float, default=0.01)
+    ap.add_argument('--audit-photons', type=int, default=1_000_000)
+    ap.add_argument('--rate-cases', type=int, default=512)
+    ap.add_argument('--seed', type=int, default=0)
+    ap.add_argument('--save', type=str, default='')
+    ap.add_argument('--report', type=str, default='')
+    a = ap.parse_args()
+    w = GpuWalker(0)
+    w.set_launch(256, 4, 4096, 8)
+    t0 = time.perf_counter()
+    inp = sample_training_inputs(a.train_cases, a.seed, thin_frac=a.thin_frac)
+    tg, tl, gen_s = run_mc(w, inp, a.train_photons, seed=a.seed * 1000 + 1)
+    var = target_variance(tl, tg, a.train_photons)
+    del tl
+    print('data: %d cases x %d photons, walk %.1f s device (%.1f s wall)'
+          % (inp.shape[0], a.train_photons, gen_s, time.perf_counter() - t0), flush=True)
+    sur = Surrogate()
+    t1 = time.perf_counter()
+    sur.fit(inp, tg, epochs=a.epochs, log_every=max(1, a.epochs // 10), seed=a.seed, var=var)
+    train_s = time.perf_counter() - t1
+    if a.save:
+        sur.save(a.save)
+    q = sample_inputs(a.queries, a.seed + 12345)
+    _, rep = serve(sur, w, q, audit_frac=a.audit_frac, audit_photons=a.audit_photons, seed=a.seed + 3)
+    mc_rate = plain_mc_rate(w, sample_inputs(a.rate_cases, a.seed + 777), a.audit_photons)
+    rep.update(train_cases=int(inp.shape[0]), train_photons=a.train_photons, data_walk_s_device=gen_s,
+               train_s=train_s, epochs=a.epochs, final_loss=sur.history[-1],
+               plain_mc_cases_per_s=mc_rate,
+               gain_surrogate=rep['cases_per_s_surrogate'] / mc_rate,
+               gain_served=rep['cases_per_s_served'] / mc_rate)
+    print(json.dumps(rep, indent=1), flush=True)
+    if a.report:
+        with open(a.report, 'w') as fh:
+            json.dump(rep, fh, indent=1)
+    w.close()
+
+
+if __name__ == '__main__':
+    main()
