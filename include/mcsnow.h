@@ -64,7 +64,10 @@ typedef struct {
     int32_t lambertian_bottom; /* 1: Lambertian bottom (mc3D:1420-1459)       */
     int32_t band_begin;        /* first band row of this case                 */
     int32_t n_bands;           /* number of band rows (1..MCS_MAX_BANDS)      */
-    int32_t reserved;
+    int32_t lambertian_surface; /* 1: the layer is a Lambertian reflector of
+                                   reflectance r_lambertian: first event at
+                                   the surface, re-emission cosine-weighted
+                                   (mc3D:1228-1238, 1385-1387)                */
     int64_t n_photons;         /* photon packets of this case (>= 0)          */
 } mcs_case;
 

@@ -13,7 +13,7 @@ import numpy as np
 from monte_carlompi_amd import monte_carlo3D
 
 DEBUG = False
-LAMBERTIAN_SURFACE = False      # Lambertian surface mode (not built)
+LAMBERTIAN_SURFACE = False      # True: simulate a Lambertian surface instead of snow
 LAMBERTIAN_BOTTOM = True        # Lambertian boundary under the snow ...
 LAMBERTIAN_REFLECTANCE = 0.5    # ... with this reflectance
 

@@ -35,7 +35,7 @@ class McsCase(ctypes.Structure):
     _fields_ = [('tau_tot', ctypes.c_double), ('mux0', ctypes.c_double), ('muz0', ctypes.c_double),
                 ('r_lambertian', ctypes.c_double), ('lambertian_bottom', ctypes.c_int32),
                 ('band_begin', ctypes.c_int32), ('n_bands', ctypes.c_int32),
-                ('reserved', ctypes.c_int32), ('n_photons', ctypes.c_int64)]
+                ('lambertian_surface', ctypes.c_int32), ('n_photons', ctypes.c_int64)]
 
 
 class McsBand(ctypes.Structure):

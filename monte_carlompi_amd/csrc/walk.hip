@@ -52,7 +52,8 @@ struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
     unsigned long long t_refl;  // reflect iff 32 random bits < t_refl (R * 2^32)
     float tau_tot_f, mux0_f, muz0_f;
-    int32_t lamb, band_begin, n_bands;
+    uint32_t t_abs_surf;        // Lambertian surface: absorbed iff bits < (1 - R) 2^32
+    int32_t lamb, band_begin, n_bands, surf;
     int64_t n_photons, gid_base;
 };
 
@@ -142,9 +143,9 @@ struct NativeWalker {
     bool bounce;
     // per-photon parameters
     float a, b, cc, d, e, tau_tot, inv_ext;
-    uint32_t t_abs_ice, t_abs_imp, t_imp;
+    uint32_t t_abs_ice, t_abs_imp, t_imp, t_abs_surf;
     unsigned long long t_refl;
-    bool iso, lamb, has_imp;
+    bool iso, lamb, has_imp, surf;
     int cond;
     PhiloxPre pre;
 
@@ -171,6 +172,8 @@ struct NativeWalker {
         tau_tot = C.tau_tot_f;
         lamb = C.lamb != 0;
         t_refl = C.t_refl;
+        surf = C.surf != 0;
+        t_abs_surf = C.t_abs_surf;
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0; p = photon; c = (uint32_t)case_id;
         bounce = false;
@@ -183,11 +186,13 @@ struct NativeWalker {
     {
         ++i;
         const u32x4 r = philox_step(pre, i, P.key0, P.key1);
-        const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(r.z));
+        float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(r.z));
+        if (surf && i == 1) dtau = 0.f;  // Lambertian surface: first event at z = 0
         float mzn = muz;
         if (i > 1) {
             float cost, sint;
-            if (bounce) {
+            const bool lambert = bounce || surf;
+            if (lambert) {
                 // cosine-weighted hemisphere: cos(theta) = sqrt(u) has the pdf
                 // 2 sin cos of the rejection loop in mc3D:1242-1250
                 const float u = u01f(r.x);
@@ -209,7 +214,7 @@ struct NativeWalker {
             const float cosp = __builtin_amdgcn_cosf(rev);
             const float s2 = fmaf(-muz, muz, 1.f);
             float mxn, myn;
-            if (bounce) {
+            if (lambert) {
                 mxn = sint * cosp; myn = sint * sinp; mzn = cost;
                 bounce = false;
             } else if (s2 < 1e-7f) {
@@ -251,7 +256,8 @@ struct NativeWalker {
             const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
             imp_mask = (r2.x < t_imp) ? 0xFFFFFFFFu : 0u;
         }
-        const uint32_t t_abs = t_abs_ice ^ ((t_abs_ice ^ t_abs_imp) & imp_mask);
+        uint32_t t_abs = t_abs_ice ^ ((t_abs_ice ^ t_abs_imp) & imp_mask);
+        if (surf) t_abs = t_abs_surf;
         if (r.w < t_abs) {
             cond = imp_mask ? 5 : 4;
             return true;
@@ -287,7 +293,7 @@ struct ReplayWalker {
     int64_t pos, end;
     uint32_t i;
     int band;
-    bool bounce, lamb, err;
+    bool bounce, lamb, err, surf;
     int cond;
 
     __device__ double draw(const KParams &P)
@@ -308,6 +314,7 @@ struct ReplayWalker {
         g = B.g; ssa_ice = B.ssa_ice; ssa_imp = B.ssa_imp; ext_cff = B.ext_cff;
         p_ext_imp = B.p_ext_imp;
         tau_tot = C.tau_tot; R = C.r_lamb; lamb = C.lamb != 0;
+        surf = C.surf != 0;
         pos = P.roff[gid];
         end = P.roff[gid + 1];
         err = false;
@@ -334,12 +341,12 @@ struct ReplayWalker {
             u_ssa = draw(P);
             u_ext = draw(P);
         } else {
-            dtau = dtau0;
+            dtau = surf ? 0.0 : dtau0;   // mc3D:1228-1229
         }
         double mxn = mux, myn = muy, mzn = muz;
         if (i > 1) {
             double cost, sint;
-            if (bounce) {
+            if (bounce || surf) {
                 mux = 0.0; muy = 0.0; muz = 1.0;
                 for (;;) {
                     const double th = 0.0 + (PI_D / 2) * draw(P);
@@ -382,7 +389,7 @@ struct ReplayWalker {
         }
         path += dtau / ext_cff;
         const bool ice = u_ext > p_ext_imp;
-        const double ssa_event = ice ? ssa_ice : ssa_imp;
+        const double ssa_event = surf ? R : (ice ? ssa_ice : ssa_imp);   // mc3D:1374-1387
         if (z_i > 0.0) {
             path += -((z_i * dtau) / ((z_i - z_prev) * ext_cff));
             cond = 1;
@@ -919,6 +926,8 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.t_refl = (unsigned long long)llround(C.r_lambertian * 4294967296.0);
         D.tau_tot_f = (float)C.tau_tot; D.mux0_f = (float)C.mux0; D.muz0_f = (float)C.muz0;
         D.lamb = C.lambertian_bottom ? 1 : 0; D.band_begin = C.band_begin; D.n_bands = C.n_bands;
+        D.surf = C.lambertian_surface ? 1 : 0;
+        D.t_abs_surf = prob_bits(1.0 - C.r_lambertian);
         D.n_photons = C.n_photons; D.gid_base = total;
         total += C.n_photons;
     }

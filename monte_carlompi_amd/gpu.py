@@ -46,6 +46,7 @@ class Case:
     bands: list = field(default_factory=list)
     lambertian_bottom: bool = False
     r_lambertian: float = 0.0
+    lambertian_surface: bool = False
 
     def direction(self):
         th = (np.pi * self.theta0_deg) / 180.           # mc3D:1508
@@ -63,7 +64,8 @@ def pack_cases(cases):
             raise ValueError('case %d has no bands' % k)
         mux0, muz0 = c.direction()
         carr[k] = McsCase(float(c.tau_tot), mux0, muz0, float(c.r_lambertian),
-                          int(bool(c.lambertian_bottom)), b, len(c.bands), 0, int(c.n_photons))
+                          int(bool(c.lambertian_bottom)), b, len(c.bands),
+                          int(bool(c.lambertian_surface)), int(c.n_photons))
         w = np.array([x.weight for x in c.bands], dtype=np.float64)
         cdf = np.cumsum(w) / w.sum()
         cdf[-1] = 1.0

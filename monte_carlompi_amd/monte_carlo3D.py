@@ -14,9 +14,10 @@ Modes (keyword ``mode`` of ``run``, default from ``--mode`` / config):
   reference's draw order (``replay=(values, offsets)``), which reproduces the
   reference's records photon for photon.
 
-Scope: spheres / Henyey-Greenstein only.  The aspherical phase-matrix path,
-Stokes vectors, the Lambertian *surface* mode, debug trajectory plots and
-phase-function plots of the reference raise NotImplementedError.
+Scope: spheres / Henyey-Greenstein (with the Lambertian bottom and the
+Lambertian surface modes).  The aspherical phase-matrix path, Stokes vectors,
+debug trajectory plots and phase-function plots of the reference raise
+NotImplementedError.
 """
 from __future__ import annotations
 
@@ -172,8 +173,6 @@ class MonteCarlo(object):
         if shape != 'sphere' and not self.HG:
             raise NotImplementedError('aspherical phase matrices / Stokes vectors are not built '
                                       '(use shape="sphere" or HG=True)')
-        if Lambertian_surface:
-            raise NotImplementedError('Lambertian surface mode is not built')
         if debug or self.phase_functions:
             raise NotImplementedError('debug trajectory and phase-function plots are not built')
         self.Lambertian_bottom = Lambertian_bottom
@@ -220,7 +219,8 @@ class MonteCarlo(object):
                       ssa_imp=float(ssa_imp[first[j]]), p_ext_imp=float(p_ext_imp[first[j]]),
                       weight=float(np.sum(band == j))) for j in range(uniq.size)]
         case = Case(tau_tot=tau_tot, theta0_deg=theta_0, n_photons=n_photon, bands=bands,
-                    lambertian_bottom=Lambertian_bottom, r_lambertian=Lambertian_reflectance)
+                    lambertian_bottom=Lambertian_bottom, r_lambertian=Lambertian_reflectance,
+                    lambertian_surface=Lambertian_surface)
         w = self.walker()
         w.set_cases([case])
         mode = 'replay' if replay is not None else (self.mode or 'native')

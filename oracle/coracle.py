@@ -23,7 +23,7 @@ class MTState(ctypes.Structure):
 class CaseC(ctypes.Structure):
     _fields_ = [('tau_tot', ctypes.c_double), ('rho_snw', ctypes.c_double),
                 ('mux0', ctypes.c_double), ('muz0', ctypes.c_double),
-                ('lambertian_bottom', ctypes.c_int32), ('pad', ctypes.c_int32),
+                ('lambertian_bottom', ctypes.c_int32), ('lambertian_surface', ctypes.c_int32),
                 ('r_lambertian', ctypes.c_double), ('max_steps', ctypes.c_int64)]
 
 
@@ -75,13 +75,14 @@ ERRORS = {-1: 'bad arguments', -2: 'record capacity exceeded', -3: 'replay slice
 
 def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
          mt: MTState | None = None, replay=None, record=False, max_steps=0,
-         allow_unbound_phi=False):
+         allow_unbound_phi=False, lambertian_surface=False):
     """Walk len(par['g']) photons.  ``replay`` = (values, offsets) CSR slices.
     Returns dict of record arrays (+ 'steps', and 'stream' = (values, offsets)
     when ``record``)."""
     theta_0 = (np.pi * theta_0_deg) / 180.
     c = CaseC(float(tau_tot), float(rho_snw), float(np.sin(theta_0)), float(-np.cos(theta_0)),
-              int(bool(lambertian_bottom)), 0, float(r_lambertian), int(max_steps))
+              int(bool(lambertian_bottom)), int(bool(lambertian_surface)), float(r_lambertian),
+              int(max_steps))
     n = int(par['g'].size)
     arrs = [np.ascontiguousarray(par[k], dtype=np.float64)
             for k in ('g', 'ssa_ice', 'ssa_imp', 'ext_cff_mss', 'p_ext_imp')]
@@ -129,7 +130,8 @@ def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
 
 
 def run_rank(seed, n_photon, wvl0, half_width, ssp_fn, imp_cnc, tau_tot, rho_snw,
-             theta_0_deg, lambertian_bottom, r_lambertian, record=False, max_steps=0):
+             theta_0_deg, lambertian_bottom, r_lambertian, record=False, max_steps=0,
+             lambertian_surface=False):
     """Single-rank run() (mc3D:1492-1619) with ``RandomState(seed)``: the normal
     wavelength draw in numpy, then initial triples + walks in C continuing the
     same MT19937 stream.  Returns (records dict incl. wvn/snow_depth, params)."""
@@ -140,7 +142,7 @@ def run_rank(seed, n_photon, wvl0, half_width, ssp_fn, imp_cnc, tau_tot, rho_snw
     par = pywalk.photon_params(wvls, ice, imp, imp_cnc, tau_tot, rho_snw)
     mt = mt_from_numpy(rs)
     out = walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
-               mt=mt, record=record, max_steps=max_steps)
+               mt=mt, record=record, max_steps=max_steps, lambertian_surface=lambertian_surface)
     out['wvn'] = 1. / wvls
     out['snow_depth'] = par['snow_depth']
     return out, par

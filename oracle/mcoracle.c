@@ -38,7 +38,7 @@ typedef struct {
     double rho_snw;            /* snow density [kg/m3] */
     double mux0, muz0;         /* sin(theta0), -cos(theta0) (mc3D:1119-1121) */
     int32_t lambertian_bottom; /* 1: Lambertian bottom boundary (mc3D:1420-1428) */
-    int32_t pad;
+    int32_t lambertian_surface; /* 1: Lambertian reflector (mc3D:1228-1238, 1385-1387) */
     double r_lambertian;       /* its reflectance */
     int64_t max_steps;         /* 0: unlimited (the reference has no cap) */
 } mco_case;
@@ -197,7 +197,7 @@ int mco_walk(const mco_case *c, int64_t n,
         double mux0 = c->mux0, muy0 = 0.0, muz0 = c->muz0;
         double muz2 = muz0 * muz0;
         double z = 0.0, path_length = 0.0;
-        double dtau = -log(u_tau0), u_ssa = u_ssa0, u_ext = u_ext0;
+        double dtau = c->lambertian_surface ? 0.0 : -log(u_tau0), u_ssa = u_ssa0, u_ext = u_ext0;
         double u_hg = 0.0, u_phi = 0.0;
         int bottom_reflection = 0, condition = 0;
         int64_t i = 0;
@@ -219,7 +219,7 @@ int mco_walk(const mco_case *c, int64_t n,
                 mux_n = mux0; muy_n = muy0; muz_n = muz0;
             } else {
                 double costheta, sintheta;
-                if (bottom_reflection) {
+                if (bottom_reflection || c->lambertian_surface) {
                     mux0 = 0.0; muy0 = 0.0; muz0 = 1.0;
                     for (;;) {
                         double th = 0.0 + half_pi * draw(&s);
@@ -263,7 +263,7 @@ int mco_walk(const mco_case *c, int64_t n,
             }
             path_length += dtau / ext_cff;
             int ice = u_ext > p_ext_imp[k];
-            double ssa_event = ice ? ssa_ice[k] : ssa_imp[k];
+            double ssa_event = c->lambertian_surface ? c->r_lambertian : (ice ? ssa_ice[k] : ssa_imp[k]);
             if (z_i > 0.0) {
                 condition = 1;
                 path_length += -((z_i * dtau) / ((z_i - z_prev) * ext_cff));

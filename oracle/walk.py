@@ -145,8 +145,10 @@ class PhotonUnboundPhi(UnboundLocalError):
 
 def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
                 p_ext_imp_k, snow_depth_k, tau_tot, rho_snw, theta_0,
-                lambertian_bottom, r_lambertian, trajectory=True):
-    """Walk one photon packet (mc3D:1111-1490, sphere/HG, no Lambertian surface).
+                lambertian_bottom, r_lambertian, trajectory=True, lambertian_surface=False):
+    """Walk one photon packet (mc3D:1111-1490, sphere/HG; with
+    ``lambertian_surface`` the layer is a Lambertian reflector, mc3D:1228-1238,
+    1385-1387).
 
     ``first`` is the photon's initial triple (tau0, ssa0, ext0) from
     ``initial_triples``.  Returns the 7 CSV fields
@@ -172,10 +174,12 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
         if i >= 2:                                          # i_rand > i_max: mc3D:1215-1224
             p_r, tau_r, phi_r, ssa_r, ext_r = _draw_step(src, g_k)
         dtau_current = tau_r                                # mc3D:1227
+        if lambertian_surface and i == 1:                   # mc3D:1228-1229
+            dtau_current = 0
 
         if i == 1:                                          # mc3D:1232-1237
             costheta = 1; sintheta = 0; cosphi = 0; sinphi = 0
-        elif bottom_reflection:                             # mc3D:1238-1250
+        elif lambertian_surface or bottom_reflection:       # mc3D:1238-1250
             mux_0 = 0.; muy_0 = 0.; muz_0 = 1.
             while True:
                 theta_rand = src.uniform_half_pi()
@@ -236,6 +240,8 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
         else:
             ext_state = 2
             ssa_event = ssa_imp_k
+        if lambertian_surface:                              # mc3D:1385-1387
+            ssa_event = r_lambertian
 
         if z_i > 0:                                         # mc3D:1390-1397
             condition = 1
@@ -353,7 +359,7 @@ def photon_params(wvls, ice, imp, imp_cnc, tau_tot, rho_snw):
 
 
 def walk_rank(src, par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom,
-              r_lambertian, trajectory=True, photon_hook=None):
+              r_lambertian, trajectory=True, photon_hook=None, lambertian_surface=False):
     """initial_pdfs + the photon loop of run() (mc3D:1594-1616) for one rank's
     working set.  ``photon_hook(k, start_index)`` is called before each walk
     (used to cut per-photon replay slices out of a recorded stream)."""
@@ -368,7 +374,7 @@ def walk_rank(src, par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom,
                                par['g'][k], par['ssa_ice'][k], par['ssa_imp'][k],
                                par['ext_cff_mss'][k], par['p_ext_imp'][k],
                                par['snow_depth'][k], tau_tot, rho_snw, theta_0,
-                               lambertian_bottom, r_lambertian, trajectory))
+                               lambertian_bottom, r_lambertian, trajectory, lambertian_surface))
     return out
 
 
@@ -386,7 +392,7 @@ def records_to_arrays(records):
 
 def replay_run(seed, n_photon, wvl0, half_width, ssp_fn, imp_cnc, tau_tot,
                rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
-               trajectory=False):
+               trajectory=False, lambertian_surface=False):
     """Single-rank run() with ``RandomState(seed)``: returns (records,
     per-photon wavelengths, params, CSR replay stream (values, offsets)).
 
@@ -401,7 +407,8 @@ def replay_run(seed, n_photon, wvl0, half_width, ssp_fn, imp_cnc, tau_tot,
     starts = []
     records = walk_rank(src, par, tau_tot, rho_snw, theta_0_deg,
                         lambertian_bottom, r_lambertian, trajectory,
-                        photon_hook=lambda k: starts.append(len(src.log)))
+                        photon_hook=lambda k: starts.append(len(src.log)),
+                        lambertian_surface=lambertian_surface)
     log = np.asarray(src.log, dtype=np.float64)
     n = n_photon
     init = log[:3 * n].reshape(n, 3)

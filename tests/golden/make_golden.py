@@ -110,6 +110,9 @@ CASES = [
     dict(name='one_photon', seed=1008, n=1, wvl0=1.3, fwhm=0.085, rds=100,
          theta0=20.0, lb=True, R=0.5, tau=30.0, imp_cnc=0.0,
          ice=spectral_table(), imp=imp_table(0.3, 12000.0)),
+    dict(name='lamb_surface', seed=1009, n=500, wvl0=1.0, fwhm=1e-12, rds=100,
+         theta0=35.0, lb=False, R=0.6, tau=5.0, imp_cnc=0.0, ls=True,
+         ice=flat_table(0.99, EXT_100, 0.85), imp=imp_table(0.3, 12000.0)),
 ]
 
 
@@ -174,7 +177,8 @@ def main():
                                       output_dir=outdir, optics_dir=optics,
                                       fi_imp='imp.nc')
         mc.run(c['n'], c['wvl0'], c['fwhm'], c['rds'], theta_0=c['theta0'],
-               Lambertian_bottom=c['lb'], Lambertian_reflectance=c['R'])
+               Lambertian_bottom=c['lb'], Lambertian_reflectance=c['R'],
+               Lambertian_surface=c.get('ls', False))
         files = os.listdir(os.path.join(outdir, 'sphere'))
         assert len(files) == 1, files
         header, rows, lowprec = parse_output(os.path.join(outdir, 'sphere', files[0]))
@@ -190,6 +194,7 @@ def main():
             for key, val in c[tname].items():
                 store[p + tname + '_' + key] = np.asarray(val)
         m = {k: v for k, v in c.items() if k not in ('ice', 'imp')}
+        m.setdefault('ls', False)
         m['file_name'] = files[0]
         m['header'] = header
         m['lowprec_columns'] = lowprec

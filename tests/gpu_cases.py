@@ -4,7 +4,7 @@ import numpy as np
 from monte_carlompi_amd.gpu import Band, Case
 
 
-def case_from_params(par, tau_tot, rho_snw, theta0, lb, R):
+def case_from_params(par, tau_tot, rho_snw, theta0, lb, R, ls=False):
     """One GPU case whose bands are the unique photon wavelengths; returns
     (case, per-photon band index)."""
     wvl = par['wvl']
@@ -17,7 +17,7 @@ def case_from_params(par, tau_tot, rho_snw, theta0, lb, R):
                           ssa_imp=float(par['ssa_imp'][k]), p_ext_imp=float(par['p_ext_imp'][k]),
                           weight=float(np.sum(inv == j))))
     case = Case(tau_tot=tau_tot, theta0_deg=theta0, n_photons=int(wvl.size), bands=bands,
-                lambertian_bottom=bool(lb), r_lambertian=float(R))
+                lambertian_bottom=bool(lb), r_lambertian=float(R), lambertian_surface=bool(ls))
     return case, inv.astype(np.uint8)
 
 

@@ -25,7 +25,8 @@ def write_nc(path, table):
     f.close()
 
 
-@pytest.mark.parametrize('case', [c for c in CASES if c['name'] in ('spectral_imp', 'thin_lamb', 'cfg1_semi_inf')],
+@pytest.mark.parametrize('case', [c for c in CASES if c['name'] in ('spectral_imp', 'thin_lamb', 'cfg1_semi_inf',
+                                                                   'lamb_surface')],
                          ids=lambda c: c['name'])
 def test_run_replay_reproduces_reference_file(tmp_path, monkeypatch, case):
     print('driver replay', case['name'], flush=True)
@@ -40,7 +41,7 @@ def test_run_replay_reproduces_reference_file(tmp_path, monkeypatch, case):
     np.random.seed(case['seed'])
     path = mc.run(case['n'], case['wvl0'], case['fwhm'], case['rds'], theta_0=case['theta0'],
                   Lambertian_bottom=case['lb'], Lambertian_reflectance=case['R'],
-                  replay=(values, offsets))
+                  Lambertian_surface=case.get('ls', False), replay=(values, offsets))
     mc.close()
     assert os.path.basename(path) == case['file_name']
     df = post_processing.read_output(path)

@@ -163,3 +163,68 @@ def test_records_independent_of_launch_shape_and_graph(walker):
                     np.testing.assert_array_equal(r[f], ref[f], err_msg='%s %s' % (shape, f))
                 np.testing.assert_array_equal(u, uref)
     walker.set_launch(256, 4, 2048, 8)
+
+
+def test_impurity_two_species_against_oracle(walker):
+    """Impurity share of extinction P_ext_imp with its own albedo: exit,
+    ice-absorption (4) and impurity-absorption (5) fractions within 3 sigma of
+    the C oracle (mc3D:1374-1383, 1461-1466)."""
+    from monte_carlompi_amd.gpu import Band, Case
+    n = 1_000_000
+    imp_cnc, ext_imp = 2e-4, 12000.0
+    ext_mix = EXT100 * (1 - imp_cnc) + ext_imp * imp_cnc
+    p_imp = imp_cnc * ext_imp / (imp_cnc * ext_imp + (1 - imp_cnc) * EXT100)
+    tau = 1.0 * ext_mix * 300.0 * 0.01     # 1 cm of dirty snow
+    band = Band(wvl=1.0, g=0.86, ssa_ice=0.9995, ext_cff=ext_mix * 300.0, ssa_imp=0.3, p_ext_imp=p_imp)
+    walker.set_cases([Case(tau_tot=tau, theta0_deg=30.0, n_photons=n, bands=[band],
+                           lambertian_bottom=True, r_lambertian=0.4)])
+    walker.walk(mode='native', seed=8, records=False)
+    c = walker.counts()[0].astype(float)
+    par = lambda k: dict(wvl=np.full(k, 1.0), g=np.full(k, 0.86), ssa_ice=np.full(k, 0.9995),
+                         ssa_imp=np.full(k, 0.3), ext_cff_mss=np.full(k, ext_mix),
+                         p_ext_imp=np.full(k, p_imp), snow_depth=np.full(k, 0.01))
+    cpu = coracle.run_ranks(300, 8, 25_000, par, tau, 300.0, 30.0, True, 0.4)
+    m = cpu['condition'].size
+    assert c[5] > 0.05 * n
+    for cond in (1, 2, 3, 4, 5):
+        z = two_sample_z(c[cond], n, np.sum(cpu['condition'] == cond), m)
+        assert abs(z) < 3.0, (cond, c[cond] / n, np.mean(cpu['condition'] == cond), z)
+
+
+def test_lambertian_surface_mode(walker):
+    """A Lambertian reflector of reflectance R: albedo R, absorption 1 - R at
+    the surface, exit cosines with pdf 2 mu (E[mu] = 2/3), n_scat = 1."""
+    from monte_carlompi_amd.gpu import Band, Case
+    n, R = 1_000_000, 0.37
+    walker.set_cases([Case(tau_tot=5.0, theta0_deg=20.0, n_photons=n, lambertian_surface=True,
+                           r_lambertian=R, bands=[Band(wvl=1.0, g=0.85, ssa_ice=0.99, ext_cff=5000.0)])])
+    walker.walk(mode='native', seed=12, records=True)
+    r = walker.records()
+    refl = r['condition'] == 1
+    assert set(np.unique(r['condition'])) <= {1, 4}
+    assert abs(refl.mean() - R) < 3 * np.sqrt(R * (1 - R) / n)
+    mu = np.cos(r['theta_n'][refl])
+    assert abs(mu.mean() - 2 / 3) < 3 * np.sqrt((0.5 - 4 / 9) / refl.sum())
+    assert np.all(r['n_scat'][refl] == 1) and np.all(r['n_scat'][~refl] == 0)
+    assert np.all(np.abs(r['path_length'][refl]) < 1e-6)
+
+
+def test_reference_known_answer_through_drop_in(tmp_path, monkeypatch):
+    """The reference's test() (mc3D:1849-1866): tau 2, omega 0.9, g 0.75,
+    normal incidence, set through MonteCarlo(test=True) overrides; with the
+    free bottom its docstring values R = 0.09739, T = 0.66096 hold."""
+    from monte_carlompi_amd.monte_carlo3D import MonteCarlo
+    monkeypatch.setattr('sys.argv', ['prog'])
+    mc = MonteCarlo(tau_tot=2.0, imp_cnc=0, output_dir=str(tmp_path), optics_dir=str(tmp_path / 'none'),
+                    ssa_ice=0.5, g=0.5, seed=21)
+    mc.ssa_ice = 0.9
+    mc.g = 0.75
+    n = 1_000_000
+    mc.run(n, 0.5, 1e-12, 100, test=True, Lambertian_bottom=False, write=False)
+    rec = mc.records
+    R = np.mean(rec['condition'] == 1)
+    T = np.mean((rec['condition'] == 2) | (rec['condition'] == 3))
+    assert abs(R - 0.09739) < 3 * np.sqrt(0.09739 * 0.90261 / n) + 2e-5
+    assert abs(T - 0.66096) < 3 * np.sqrt(0.66096 * 0.33904 / n) + 2e-5
+    assert abs(mc.calculate_albedo(rec) - R) < 1e-12
+    mc.close()

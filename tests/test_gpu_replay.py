@@ -36,7 +36,7 @@ def test_replay_matches_reference_golden(walker, case):
     print('replay golden', case['name'], flush=True)
     records, par, values, offsets = run_case(case)
     gcase, pband = case_from_params(par, case['tau'], META['rho_snw'], case['theta0'],
-                                    case['lb'], case['R'])
+                                    case['lb'], case['R'], case.get('ls', False))
     walker.set_cases([gcase])
     walker.walk(mode='replay', replay=(values, offsets), photon_band=pband)
     got = walker.records()

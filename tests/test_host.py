@@ -108,4 +108,4 @@ def test_unsupported_reference_modes_raise():
     with pytest.raises(NotImplementedError):
         mc.run(10, 1.3, 0.085, 100, shape='droxtal')
     with pytest.raises(NotImplementedError):
-        mc.run(10, 1.3, 0.085, 100, Lambertian_surface=True)
+        mc.run(10, 1.3, 0.085, 100, debug=True)

@@ -28,7 +28,7 @@ def test_c_oracle_matches_reference_records(case):
     out, par = coracle.run_rank(case['seed'], case['n'], case['wvl0'], case['fwhm'],
                                 oracle_ssp_fn(case), case['imp_cnc'], case['tau'],
                                 META['rho_snw'], case['theta0'], case['lb'], case['R'],
-                                record=True)
+                                record=True, lambertian_surface=case.get('ls', False))
     ref = case['records']
     np.testing.assert_array_equal(out['condition'], ref['condition'])
     np.testing.assert_array_equal(out['n_scat'], ref['n_scat'])

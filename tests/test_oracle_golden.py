@@ -22,7 +22,7 @@ def run_case(case, trajectory=False):
     return walk.replay_run(case['seed'], case['n'], case['wvl0'], case['fwhm'],
                            oracle_ssp_fn(case), case['imp_cnc'], case['tau'],
                            META['rho_snw'], case['theta0'], case['lb'], case['R'],
-                           trajectory=trajectory)
+                           trajectory=trajectory, lambertian_surface=case.get('ls', False))
 
 
 @pytest.mark.parametrize('case', CASES, ids=[c['name'] for c in CASES])
