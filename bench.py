@@ -21,6 +21,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon MC'
+SNOW_1M_CANDIDATES = [(256, 4, 2048, 8), (256, 5, 2048, 8), (256, 4, 1024, 4), (512, 2, 2048, 8),
+                      (256, 5, 1024, 16), (128, 8, 1024, 8)]
 N_PHOTONS = 1_000_000
 
 
@@ -33,6 +35,7 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-photons', type=int, default=10_000)
     ap.add_argument('--cpu-procs', type=int, default=0)
+    ap.add_argument('--no-sweep', action='store_true', help='skip the batched 2048-case sweep block')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU: the photon walk is built and measured on a single '
@@ -47,7 +50,8 @@ def main():
 
     w = GpuWalker(0)
     w.set_cases([snow_1m_case(args.photons)])
-    w.set_launch(256, 4, 2048, 8)
+    # the project's autotuner picks the launch shape for this workload (untimed)
+    shape, tune_ms = w.autotune(SNOW_1M_CANDIDATES, photons_cap=0, repeats=3, seed=99)
     for k in range(args.warmup):
         w.walk(mode='native', seed=10_000 + k, records=True)
     torch.cuda.synchronize()
@@ -88,6 +92,7 @@ def main():
         'photon_steps_per_s': round(avg_steps / (ms_per_step * 1e-3), 1),
         'brf_cases_per_s': round(1e3 / ms_per_step, 3),
         'kernel_ms_avg': round(avg_kern, 4),
+        'launch': list(shape),
         'albedo': round(float(fr['albedo'][0]), 6),
         'roofline': {
             'bound': 'valu', 'achieved': round(rf['fp32_tflops'], 4),
@@ -98,6 +103,8 @@ def main():
                      'per launch / walk-kernel HIP-event time',
         },
     }
+    if not args.no_sweep:
+        out['sweep'] = sweep_block(w)
     w.close()
     if not args.no_cpu:
         from oracle import cpu_baseline
@@ -113,6 +120,32 @@ def main():
         }
         out['gpu_over_cpu'] = round(photons_per_s / cb['photons_per_s'], 1)
     print(json.dumps(out), flush=True)
+
+
+def sweep_block(w):
+    """Batched sweep: 64 r_eff x 16 wavelength distributions x 8 theta0 x 2
+    depths = 16,384 cases, photons per case log-uniform in [1e4, 1e6]
+    (seeded), one persistent launch, autotuned shape, replayed through a
+    hipGraph."""
+    from monte_carlompi_amd import roofline, sweep
+    from monte_carlompi_amd.workloads import sweep_cases
+    cases, meta = sweep_cases(seed=0)
+    res = sweep.run(w, cases, seed=1, use_graph=True, tune=True, sample_cases=1024)
+    times = []
+    for k in range(3):                          # graph replays of the captured launch
+        w.walk(mode='native', seed=2 + k, records=False, use_graph=True)
+        times.append(w.stats()['kernel_ms'])
+    st = w.stats()
+    ms = sorted(times)[1]
+    rf = roofline.walk_roofline(st['steps'], ms)
+    return {'workload': '16384 cases (64 r_eff x 16 Gaussian wavelength distributions x 8 theta0 x 2 depths), '
+                        'photons/case log-uniform 1e4..1e6, free bottom',
+            'cases': len(cases), 'photons': int(st['photons']), 'photon_steps': int(st['steps']),
+            'ms_per_sweep': round(ms, 3), 'cases_per_s': round(len(cases) / (ms * 1e-3), 1),
+            'photons_per_s': round(st['photons'] / (ms * 1e-3), 1),
+            'photon_steps_per_s': round(st['steps'] / (ms * 1e-3), 1),
+            'launch': list(res['launch']), 'autotune_ms': {str(k): round(v, 3) for k, v in res['tune_times'].items()},
+            'valu_lane_ops_frac': round(rf['valu_frac'], 4), 'fp32_frac': round(rf['fp32_frac'], 5)}
 
 
 if __name__ == '__main__':

@@ -52,7 +52,6 @@ struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
     unsigned long long t_refl;  // reflect iff 32 random bits < t_refl (R * 2^32)
     float tau_tot_f, mux0_f, muz0_f;
-    uint32_t t_abs_surf;        // Lambertian surface: absorbed iff bits < (1 - R) 2^32
     int32_t lamb, band_begin, n_bands, surf;
     int64_t n_photons, gid_base;
 };
@@ -61,10 +60,19 @@ struct DevBand {
     double wvn, g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, cdf;
     float hg_a, hg_b, hg_c, hg_d, hg_e;  // 1/(2g), 1+g^2, 1-g^2, 1-g, 2g
     float inv_ext;                       // 1/ext_cff [m]
-    uint32_t t_abs_ice, t_abs_imp;       // absorbed iff bits < (1-ssa) 2^32
-    uint32_t t_imp;                      // impurity iff bits < P_ext_imp 2^32
+    uint32_t t_abs5;                     // bits < P_imp (1 - ssa_imp) 2^32: absorbed by impurity
+    uint32_t t_abs_all;                  // bits < [P_imp (1 - ssa_imp) + (1 - P_imp)(1 - ssa_ice)] 2^32: absorbed
     int32_t iso;                         // g == 0
 };
+
+// probability -> 32-bit threshold (bits < t has probability p)
+__host__ __device__ inline uint32_t prob_bits_dev(double p)
+{
+    const double t = p * 4294967296.0;
+    if (!(t > 0.0)) return 0u;
+    if (t >= 4294967295.0) return 0xFFFFFFFFu;
+    return (uint32_t)(t + 0.5);
+}
 
 struct KParams {
     const DevCase *cases;
@@ -134,18 +142,26 @@ __device__ __forceinline__ T lds_load(const T *p)
 }
 
 // ----------------------------------------------------------- native walker
+// FP32 walk with one Philox4x32-10 call per step (4 uniforms: HG / Lambertian
+// cosine, azimuth, free path, absorption).  The common path is branch-free
+// (selects instead of branches) and the next step's Philox call is issued in
+// the same basic block as the current step's arithmetic: the two chains are
+// independent, so their latencies overlap -- what bounds the last, longest
+// walks of a launch.
 struct NativeWalker {
     using Real = float;
     static constexpr int MAX_BLOCK = 1024;
+    static constexpr int WAVES_PER_EU = 5;   // <= 96 VGPRs: 5 waves per SIMD without spills
     float z, mux, muy, muz, tau_acc;
     uint32_t i, p, c;
     int band;
     bool bounce;
+    u32x4 r;                     // uniforms of the next step
     // per-photon parameters
     float a, b, cc, d, e, tau_tot, inv_ext;
-    uint32_t t_abs_ice, t_abs_imp, t_imp, t_abs_surf;
+    uint32_t t_abs5, t_abs_all;  // absorbed as condition 5 iff bits < t_abs5, else 4 iff < t_abs_all
     unsigned long long t_refl;
-    bool iso, lamb, has_imp, surf;
+    bool iso, lamb, surf;
     int cond;
     PhiloxPre pre;
 
@@ -156,8 +172,8 @@ struct NativeWalker {
         if (P.photon_band) {
             bnd = P.photon_band[gid];
         } else if (C.n_bands > 1) {
-            const u32x4 r = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
-            const double u = ((double)r.x + 0.5) * 2.3283064365386963e-10;
+            const u32x4 q = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
+            const double u = ((double)q.x + 0.5) * 2.3283064365386963e-10;
             bnd = C.n_bands - 1;
             for (int k = 0; k < C.n_bands - 1; ++k)
                 if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
@@ -167,102 +183,94 @@ struct NativeWalker {
         a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
         iso = B.iso != 0;
         inv_ext = B.inv_ext;
-        t_abs_ice = B.t_abs_ice; t_abs_imp = B.t_abs_imp; t_imp = B.t_imp;
-        has_imp = B.t_imp != 0u;
+        surf = C.surf != 0;
+        if (surf) {
+            // Lambertian surface: absorbed with probability 1 - R, split by species
+            const double q = 1.0 - C.r_lamb;
+            t_abs5 = prob_bits_dev(B.p_ext_imp * q);
+            t_abs_all = prob_bits_dev(q);
+        } else {
+            t_abs5 = B.t_abs5;
+            t_abs_all = B.t_abs_all;
+        }
         tau_tot = C.tau_tot_f;
         lamb = C.lamb != 0;
         t_refl = C.t_refl;
-        surf = C.surf != 0;
-        t_abs_surf = C.t_abs_surf;
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0; p = photon; c = (uint32_t)case_id;
         bounce = false;
         cond = 0;
         pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
+        r = philox_step(pre, 1u, P.key0, P.key1);
     }
 
     // one step; returns true when the photon has left or been absorbed
     __device__ __forceinline__ bool step(const KParams &P)
     {
         ++i;
-        const u32x4 r = philox_step(pre, i, P.key0, P.key1);
-        float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(r.z));
-        if (surf && i == 1) dtau = 0.f;  // Lambertian surface: first event at z = 0
-        float mzn = muz;
-        if (i > 1) {
-            float cost, sint;
-            const bool lambert = bounce || surf;
-            if (lambert) {
-                // cosine-weighted hemisphere: cos(theta) = sqrt(u) has the pdf
-                // 2 sin cos of the rejection loop in mc3D:1242-1250
-                const float u = u01f(r.x);
-                cost = __builtin_amdgcn_sqrtf(u);
-                sint = __builtin_amdgcn_sqrtf(fmaxf(1.f - u, 0.f));
-            } else {
-                const float u = u01f(r.x);
-                if (iso) {
-                    cost = fmaf(-2.f, u, 1.f);
-                } else {
-                    const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u, d));
-                    cost = fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-                }
-                sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
-            }
-            // v_sin/v_cos take the angle in revolutions: 2 pi u -> u
-            const float rev = u01f(r.y);
-            const float sinp = __builtin_amdgcn_sinf(rev);
-            const float cosp = __builtin_amdgcn_cosf(rev);
-            const float s2 = fmaf(-muz, muz, 1.f);
-            float mxn, myn;
-            if (lambert) {
-                mxn = sint * cosp; myn = sint * sinp; mzn = cost;
-                bounce = false;
-            } else if (s2 < 1e-7f) {
-                const float sg = muz > 0.f ? 1.f : -1.f;
-                mxn = sint * cosp; myn = sg * sint * sinp; mzn = sg * cost;
-            } else {
-                const float rs = __builtin_amdgcn_rsqf(s2);
-                const float st = sint * rs;
-                mxn = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-                myn = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-                mzn = fmaf(-s2 * st, cosp, muz * cost);
-            }
-            mux = mxn; muy = myn; muz = mzn;
-        }
-        z = fmaf(dtau, mzn, z);
+        const u32x4 cur = r;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
+        const bool first = (i == 1u);
+        const bool lam = bounce || surf;
+        // free path; a Lambertian surface takes its first event at z = 0
+        float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
+        dtau = (surf && first) ? 0.f : dtau;
+        // polar cosine: HG inverse CDF (mc3D:790-800), isotropic for g = 0, or
+        // the cosine-weighted hemisphere cos = sqrt(u) whose pdf 2 sin cos is
+        // that of the rejection loop in mc3D:1242-1250
+        const float u0 = u01f(cur.x);
+        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u0, d));
+        float cost = iso ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+        cost = lam ? __builtin_amdgcn_sqrtf(u0) : cost;
+        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
+        const float rev = u01f(cur.y);
+        const float sinp = __builtin_amdgcn_sinf(rev);
+        const float cosp = __builtin_amdgcn_cosf(rev);
+        // rotation (mc3D:1255-1281); near-vertical incoming directions and
+        // Lambertian re-emission (incoming (0, 0, 1)) take the axis form
+        const float s2 = fmaf(-muz, muz, 1.f);
+        const bool vert = lam || (s2 < 1e-7f);
+        const float sg = lam ? 1.f : (muz > 0.f ? 1.f : -1.f);
+        const float st = sint * __builtin_amdgcn_rsqf(fmaxf(s2, 1e-30f));
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        const float gz = fmaf(-s2 * st, cosp, muz * cost);
+        const float vx = sint * cosp, vy = sg * sint * sinp, vz = sg * cost;
+        // the first step keeps the incident direction (mc3D:1344-1347)
+        mux = first ? mux : (vert ? vx : gx);
+        muy = first ? muy : (vert ? vy : gy);
+        muz = first ? muz : (vert ? vz : gz);
+        bounce = false;
+        z = fmaf(dtau, muz, z);
         tau_acc += dtau;
-        if (z > 0.f) {
-            tau_acc -= fminf(z / mzn, dtau);
+        const bool top = z > 0.f;
+        const bool bot = z < -tau_tot;
+        if (!(top || bot || cur.w < t_abs_all)) return false;
+        // ---- rare: exits and absorption
+        if (top) {                                   // mc3D:1390-1397
+            tau_acc -= fminf(z / muz, dtau);
             cond = 1;
             return true;
         }
-        if (z < -tau_tot) {
-            tau_acc -= fminf((z + tau_tot) / mzn, dtau);
+        if (bot) {                                   // mc3D:1399-1459
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
             if (lamb) {
-                const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
-                if ((unsigned long long)r2.y < t_refl) {
+                const u32x4 q = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
+                if ((unsigned long long)q.y < t_refl) {
                     bounce = true;
                     return false;
                 }
             }
-            cond = (i == 1) ? 3 : 2;
+            cond = first ? 3 : 2;
             return true;
         }
-        // species (mc3D:1374-1383) then absorption (mc3D:1461-1466); the
-        // threshold is picked with a mask so no select-of-address is formed
-        uint32_t imp_mask = 0u;
-        if (has_imp) {
-            const u32x4 r2 = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
-            imp_mask = (r2.x < t_imp) ? 0xFFFFFFFFu : 0u;
-        }
-        uint32_t t_abs = t_abs_ice ^ ((t_abs_ice ^ t_abs_imp) & imp_mask);
-        if (surf) t_abs = t_abs_surf;
-        if (r.w < t_abs) {
-            cond = imp_mask ? 5 : 4;
-            return true;
-        }
-        return false;
+        // absorbed: one uniform decides species and absorption jointly, with
+        // P(cond 5) = P_imp (1 - omega_imp), P(cond 4) = (1 - P_imp)(1 - omega_ice)
+        // (mc3D:1374-1383, 1461-1466)
+        cond = (cur.w < t_abs5) ? 5 : 4;
+        return true;
     }
 
     __device__ void result(float &theta, float &phi, float &path, bool &axis) const
@@ -287,6 +295,7 @@ struct NativeWalker {
 struct ReplayWalker {
     using Real = double;
     static constexpr int MAX_BLOCK = 256;
+    static constexpr int WAVES_PER_EU = 2;
     double z, mux, muy, muz, muz2, path;
     double u_ssa, u_ext, dtau0;
     double g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, tau_tot, R;
@@ -646,7 +655,8 @@ __device__ __forceinline__ void set_priority(uint32_t max_i)
 }
 
 template <class W>
-__global__ void __launch_bounds__(W::MAX_BLOCK) k_walk(const KParams P)
+__global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(W::WAVES_PER_EU)))
+k_walk(const KParams P)
 {
     __shared__ BlockState bs;
     const int lane = lane_id();
@@ -906,9 +916,9 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.hg_a = (float)(1.0 / (2.0 * g)); D.hg_b = (float)(1.0 + g * g); D.hg_c = (float)(1.0 - g * g);
         D.hg_d = (float)(1.0 - g); D.hg_e = (float)(2.0 * g);
         D.inv_ext = (float)(1.0 / B.ext_cff);
-        D.t_abs_ice = prob_bits(1.0 - B.ssa_ice);
-        D.t_abs_imp = prob_bits(1.0 - B.ssa_imp);
-        D.t_imp = prob_bits(B.p_ext_imp);
+        const double q5 = B.p_ext_imp * (1.0 - B.ssa_imp);
+        D.t_abs5 = prob_bits(q5);
+        D.t_abs_all = prob_bits(q5 + (1.0 - B.p_ext_imp) * (1.0 - B.ssa_ice));
     }
     int64_t total = 0;
     for (int k = 0; k < n_cases; ++k) {
@@ -927,7 +937,6 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.tau_tot_f = (float)C.tau_tot; D.mux0_f = (float)C.mux0; D.muz0_f = (float)C.muz0;
         D.lamb = C.lambertian_bottom ? 1 : 0; D.band_begin = C.band_begin; D.n_bands = C.n_bands;
         D.surf = C.lambertian_surface ? 1 : 0;
-        D.t_abs_surf = prob_bits(1.0 - C.r_lambertian);
         D.n_photons = C.n_photons; D.gid_base = total;
         total += C.n_photons;
     }

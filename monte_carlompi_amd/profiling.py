@@ -4,6 +4,11 @@ summaries of rocprofv3 CSV output (kernel stats, FETCH_SIZE / WRITE_SIZE).
     python -m monte_carlompi_amd.profiling copy [--mib 256]
         three device copies of MIB MiB (run under rocprofv3 --pmc to read what
         FETCH_SIZE / WRITE_SIZE report for a known number of bytes)
+    python -m monte_carlompi_amd.profiling sweep [--repeats 2]
+        the batched sweep (16,384 cases) alone, to profile it
+    python -m monte_carlompi_amd.profiling valu COUNTERS_CSV [--kernel k_walk]
+        VALU busy and lane utilisation from SQ_ACTIVE_INST_VALU,
+        SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE
     python -m monte_carlompi_amd.profiling stats KERNEL_STATS_CSV
     python -m monte_carlompi_amd.profiling traffic --kernel NAME --fetch F.csv
         --write W.csv --cal-fetch CF.csv --cal-write CW.csv [--mib 256] [--json OUT]
@@ -25,6 +30,43 @@ def copy_kernel(mib: int = 256):
         y.copy_(x)
     torch.cuda.synchronize()
     print('copied %d bytes x3' % (n * 4))
+
+
+def run_sweep(repeats=2, launch=(256, 4, 4096, 8)):
+    """The batched sweep (16,384 cases) `repeats` times with a fixed launch shape
+    (a workload to profile on its own)."""
+    from .gpu import GpuWalker
+    from .workloads import sweep_cases
+    cases, _ = sweep_cases(seed=0)
+    with GpuWalker(0) as w:
+        w.set_cases(cases)
+        w.set_launch(*launch)
+        for k in range(repeats):
+            w.walk(mode='native', seed=1 + k, records=False)
+            st = w.stats()
+            print('sweep %d: %.2f ms, %d photons, %d photon-steps' % (k, st['kernel_ms'], st['photons'],
+                                                                      st['steps']), flush=True)
+
+
+def valu_metrics(path, kernel, simds=1024):
+    """VALU busy / lane utilisation of `kernel` from SQ/GRBM counters:
+    busy = SQ_ACTIVE_INST_VALU * 4 / SIMDs / GRBM_GUI_ACTIVE (cycles a SIMD
+    issues VALU work, wave64 = 4 cycles per instruction), lane utilisation =
+    SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU * 64)."""
+    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(path)):
+        if kernel in r['Kernel_Name']:
+            vals[r['Dispatch_Id']][r['Counter_Name']].append(float(r['Counter_Value']))
+    out = []
+    for d, c in vals.items():
+        g = {k: sum(v) for k, v in c.items()}
+        res = dict(dispatch=int(d), **{k: v for k, v in g.items()})
+        if g.get('GRBM_GUI_ACTIVE') and g.get('SQ_ACTIVE_INST_VALU') is not None:
+            res['valu_busy'] = g['SQ_ACTIVE_INST_VALU'] * 4 / simds / g['GRBM_GUI_ACTIVE']
+        if g.get('SQ_ACTIVE_INST_VALU'):
+            res['valu_lane_util'] = g.get('SQ_THREAD_CYCLES_VALU', 0) / (g['SQ_ACTIVE_INST_VALU'] * 64)
+        out.append(res)
+    return out
 
 
 def kernel_stats(path):
@@ -69,6 +111,8 @@ def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
     c = sub.add_parser('copy'); c.add_argument('--mib', type=int, default=256)
+    sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
+    vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
     s = sub.add_parser('stats'); s.add_argument('csv')
     t = sub.add_parser('traffic')
     for k in ('kernel', 'fetch', 'write', 'cal_fetch', 'cal_write'):
@@ -78,6 +122,11 @@ def main():
     a = ap.parse_args()
     if a.cmd == 'copy':
         copy_kernel(a.mib)
+    elif a.cmd == 'sweep':
+        run_sweep(a.repeats)
+    elif a.cmd == 'valu':
+        for r in valu_metrics(a.csv, a.kernel):
+            print(json.dumps(r))
     elif a.cmd == 'stats':
         for r in kernel_stats(a.csv):
             print('%-60s calls %4d  avg %12.1f ns  min %12.1f  max %12.1f  %6.2f%%' % (

@@ -7,7 +7,7 @@ photons of whatever case is next instead of waiting at case boundaries, and
 each chunk's FP64 BRF histogram lives in LDS until the chunk completes.  The
 launch (tally clears + walk + record scatter) is captured once in a hipGraph
 and replayed for repeated sweeps of the same shape.  ``autotune`` times a
-grid of launch shapes on a capped copy of the cases and keeps the fastest;
+grid of launch shapes on a sampled subset of the cases and keeps the fastest;
 results do not depend on the shape because each photon's Philox stream is
 keyed by (seed, case, photon).
 """
@@ -25,21 +25,33 @@ DEFAULT_CANDIDATES = [
 ]
 
 
-def autotune(walker: GpuWalker, candidates=None, photons_cap=20_000, repeats=3, seed=1):
-    """Pick the fastest launch shape for the cases currently set."""
+def autotune(walker: GpuWalker, candidates=None, sample_cases=1024, repeats=3, seed=1, cases=None):
+    """Pick the fastest launch shape.  Candidates are timed on a seeded random
+    subset of `sample_cases` cases at their full photon counts (capping the
+    photons per case changes how long walks and chunk tails balance, and with
+    it the ranking); the full case set is restored afterwards."""
     cands = list(candidates or DEFAULT_CANDIDATES)
-    best, times = walker.autotune(cands, photons_cap=photons_cap, repeats=repeats, seed=seed)
+    cases = list(cases if cases is not None else walker.cases)
+    if len(cases) > sample_cases:
+        rng = np.random.default_rng(seed)
+        subset = [cases[k] for k in sorted(rng.choice(len(cases), sample_cases, replace=False))]
+    else:
+        subset = cases
+    walker.set_cases(subset)
+    best, times = walker.autotune(cands, photons_cap=0, repeats=repeats, seed=seed)
+    walker.set_cases(cases)
+    walker.set_launch(*best)
     return best, dict(zip(cands, times.tolist()))
 
 
 def run(walker: GpuWalker, cases, seed=0, use_graph=True, tune=True, records=False,
-        candidates=None, photons_cap=20_000):
+        candidates=None, sample_cases=1024):
     """Walk all cases; returns dict(tallies, counts, brf, fractions, seconds,
     kernel_ms, launch, tune_times, steps, photons)."""
     walker.set_cases(cases)
     tune_times = None
     if tune:
-        _, tune_times = autotune(walker, candidates, photons_cap=photons_cap)
+        _, tune_times = autotune(walker, candidates, sample_cases=sample_cases, cases=cases)
     t0 = time.perf_counter()
     walker.walk(mode='native', seed=seed, records=records, use_graph=use_graph)
     t1 = time.perf_counter()

@@ -59,7 +59,7 @@ def test_first_case_of_batch_equals_lone_launch(walker):
 
 def test_autotune_and_graph_replays_are_invariant(walker):
     cases, _ = small_sweep()
-    res = sweep.run(walker, cases, seed=9, use_graph=True, tune=True, photons_cap=2000)
+    res = sweep.run(walker, cases, seed=9, use_graph=True, tune=True, sample_cases=8)
     assert res['launch'] in [tuple(c) for c in sweep.DEFAULT_CANDIDATES]
     ref_counts = res['counts'].copy()
     ref_t = res['tallies'].copy()
