@@ -35,7 +35,10 @@ def main():
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-photons', type=int, default=10_000)
     ap.add_argument('--cpu-procs', type=int, default=0)
-    ap.add_argument('--no-sweep', action='store_true', help='skip the batched 2048-case sweep block')
+    ap.add_argument('--no-sweep', action='store_true', help='skip the batched sweep block')
+    ap.add_argument('--launch', type=str, default='',
+                    help='fixed launch shape block,blocks_per_cu,chunk,refill_min (skips the autotuner; '
+                         'for profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU: the photon walk is built and measured on a single '
@@ -51,7 +54,11 @@ def main():
     w = GpuWalker(0)
     w.set_cases([snow_1m_case(args.photons)])
     # the project's autotuner picks the launch shape for this workload (untimed)
-    shape, tune_ms = w.autotune(SNOW_1M_CANDIDATES, photons_cap=0, repeats=3, seed=99)
+    if args.launch:
+        shape = tuple(int(v) for v in args.launch.split(','))
+        w.set_launch(*shape)
+    else:
+        shape, tune_ms = w.autotune(SNOW_1M_CANDIDATES, photons_cap=0, repeats=3, seed=99)
     for k in range(args.warmup):
         w.walk(mode='native', seed=10_000 + k, records=True)
     torch.cuda.synchronize()
@@ -67,6 +74,12 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     fr = fractions_from_tallies(w.tallies())
+    # PCIe-inclusive: the same walk plus the D2H copy of all records (not `value`)
+    t2 = time.perf_counter()
+    for k in range(3):
+        w.walk(mode='native', seed=500 + k, records=True)
+        w.records()
+    pcie_ms = (time.perf_counter() - t2) * 1e3 / 3
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     photons_per_s = args.photons / (ms_per_step * 1e-3)
     avg_kern = sum(kern_ms) / len(kern_ms)
@@ -91,6 +104,7 @@ def main():
                    'theta0_deg': SNOW_1M['theta0'], 'parallelism': 'photon-per-lane, 1 GPU'},
         'photon_steps_per_s': round(avg_steps / (ms_per_step * 1e-3), 1),
         'brf_cases_per_s': round(1e3 / ms_per_step, 3),
+        'pcie_inclusive_photons_per_s': round(args.photons / (pcie_ms * 1e-3), 1),
         'kernel_ms_avg': round(avg_kern, 4),
         'launch': list(shape),
         'albedo': round(float(fr['albedo'][0]), 6),

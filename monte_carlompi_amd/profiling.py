@@ -7,8 +7,8 @@ summaries of rocprofv3 CSV output (kernel stats, FETCH_SIZE / WRITE_SIZE).
     python -m monte_carlompi_amd.profiling sweep [--repeats 2]
         the batched sweep (16,384 cases) alone, to profile it
     python -m monte_carlompi_amd.profiling valu COUNTERS_CSV [--kernel k_walk]
-        VALU busy and lane utilisation from SQ_ACTIVE_INST_VALU,
-        SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE
+        VALU issue occupancy, dual issue and lane utilisation from SQ_INSTS_VALU,
+        SQ_ACTIVE_INST_VALU(2), SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE
     python -m monte_carlompi_amd.profiling stats KERNEL_STATS_CSV
     python -m monte_carlompi_amd.profiling traffic --kernel NAME --fetch F.csv
         --write W.csv --cal-fetch CF.csv --cal-write CW.csv [--mib 256] [--json OUT]
@@ -32,7 +32,7 @@ def copy_kernel(mib: int = 256):
     print('copied %d bytes x3' % (n * 4))
 
 
-def run_sweep(repeats=2, launch=(256, 4, 4096, 8)):
+def run_sweep(repeats=2, launch=(256, 6, 4096, 16)):
     """The batched sweep (16,384 cases) `repeats` times with a fixed launch shape
     (a workload to profile on its own)."""
     from .gpu import GpuWalker
@@ -48,23 +48,33 @@ def run_sweep(repeats=2, launch=(256, 4, 4096, 8)):
                                                                       st['steps']), flush=True)
 
 
-def valu_metrics(path, kernel, simds=1024):
-    """VALU busy / lane utilisation of `kernel` from SQ/GRBM counters:
-    busy = SQ_ACTIVE_INST_VALU * 4 / SIMDs / GRBM_GUI_ACTIVE (cycles a SIMD
-    issues VALU work, wave64 = 4 cycles per instruction), lane utilisation =
-    SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU * 64)."""
-    vals = collections.defaultdict(lambda: collections.defaultdict(list))
+def valu_metrics(path, kernel, simds=1024, n_xcd=8, clock_hz=2.4e9):
+    """VALU metrics of `kernel` per dispatch from one --pmc pass.  rocprofv3
+    writes one row per counter per dispatch, summed over all XCDs, so the
+    kernel's cycles are GRBM_GUI_ACTIVE / n_xcd.  Reported:
+      issue_busy   quad-cycles in which a SIMD issued >= 1 VALU instruction,
+                   (SQ_ACTIVE_INST_VALU2 + SQ_INSTS_VALU - 2 SQ_ACTIVE_INST_VALU2)
+                   / (cycles / 4 * SIMDs)
+      dual_issue   quad-cycles with two VALU issues, SQ_ACTIVE_INST_VALU2 / (cycles / 4 * SIMDs)
+      VALUUtilization  rocprofv3's formula: 100 SQ_THREAD_CYCLES_VALU / (SQ_ACTIVE_INST_VALU 64)
+                   (share of the 64 lanes active per VALU instruction)."""
+    vals = collections.defaultdict(dict)
     for r in csv.DictReader(open(path)):
         if kernel in r['Kernel_Name']:
-            vals[r['Dispatch_Id']][r['Counter_Name']].append(float(r['Counter_Value']))
+            vals[r['Dispatch_Id']][r['Counter_Name']] = float(r['Counter_Value'])
     out = []
-    for d, c in vals.items():
-        g = {k: sum(v) for k, v in c.items()}
-        res = dict(dispatch=int(d), **{k: v for k, v in g.items()})
-        if g.get('GRBM_GUI_ACTIVE') and g.get('SQ_ACTIVE_INST_VALU') is not None:
-            res['valu_busy'] = g['SQ_ACTIVE_INST_VALU'] * 4 / simds / g['GRBM_GUI_ACTIVE']
+    for d, g in vals.items():
+        res = dict(dispatch=int(d), **g)
+        if g.get('GRBM_GUI_ACTIVE'):
+            cycles = g['GRBM_GUI_ACTIVE'] / n_xcd
+            quads = cycles / 4 * simds
+            res['kernel_s'] = cycles / clock_hz
+            if 'SQ_INSTS_VALU' in g and 'SQ_ACTIVE_INST_VALU2' in g:
+                dual = g['SQ_ACTIVE_INST_VALU2']
+                res['issue_busy'] = (dual + g['SQ_INSTS_VALU'] - 2 * dual) / quads
+                res['dual_issue'] = dual / quads
         if g.get('SQ_ACTIVE_INST_VALU'):
-            res['valu_lane_util'] = g.get('SQ_THREAD_CYCLES_VALU', 0) / (g['SQ_ACTIVE_INST_VALU'] * 64)
+            res['VALUUtilization'] = 100 * g.get('SQ_THREAD_CYCLES_VALU', 0) / (g['SQ_ACTIVE_INST_VALU'] * 64)
         out.append(res)
     return out
 

@@ -25,11 +25,12 @@ DEFAULT_CANDIDATES = [
 ]
 
 
-def autotune(walker: GpuWalker, candidates=None, sample_cases=1024, repeats=3, seed=1, cases=None):
-    """Pick the fastest launch shape.  Candidates are timed on a seeded random
-    subset of `sample_cases` cases at their full photon counts (capping the
-    photons per case changes how long walks and chunk tails balance, and with
-    it the ranking); the full case set is restored afterwards."""
+def autotune(walker: GpuWalker, candidates=None, sample_cases=1024, repeats=3, seed=1, cases=None,
+             finalists=3):
+    """Pick the fastest launch shape in two stages: every candidate is timed on
+    a seeded random subset of `sample_cases` cases at their full photon counts
+    (capping photons per case changes how long walks and chunk tails balance),
+    then the `finalists` best are timed once more on the full case set."""
     cands = list(candidates or DEFAULT_CANDIDATES)
     cases = list(cases if cases is not None else walker.cases)
     if len(cases) > sample_cases:
@@ -40,8 +41,14 @@ def autotune(walker: GpuWalker, candidates=None, sample_cases=1024, repeats=3, s
     walker.set_cases(subset)
     best, times = walker.autotune(cands, photons_cap=0, repeats=repeats, seed=seed)
     walker.set_cases(cases)
+    result = dict(zip(cands, times.tolist()))
+    if len(subset) < len(cases) and finalists > 1:
+        order = np.argsort(times)[:finalists]
+        fin = [cands[k] for k in order]
+        best, ftimes = walker.autotune(fin, photons_cap=0, repeats=1, seed=seed)
+        result.update({('full',) + tuple(c): t for c, t in zip(fin, ftimes.tolist())})
     walker.set_launch(*best)
-    return best, dict(zip(cands, times.tolist()))
+    return best, result
 
 
 def run(walker: GpuWalker, cases, seed=0, use_graph=True, tune=True, records=False,
