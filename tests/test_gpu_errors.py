@@ -1,0 +1,84 @@
+"""Error handling through the C ABI (every check raises before any launch, or
+is reported after a normal kernel exit): invalid cases and bands, call order,
+replay inputs, the per-photon step cap, and the context staying usable."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from monte_carlompi_amd import _lib
+from monte_carlompi_amd._lib import McsError
+from monte_carlompi_amd.gpu import Band, Case, GpuWalker
+
+pytestmark = pytest.mark.gpu
+
+
+def good_case(n=1000, **kw):
+    b = dict(wvl=1.0, g=0.85, ssa_ice=0.99, ext_cff=5000.0)
+    b.update({k: kw.pop(k) for k in list(kw) if k in b or k in ('ssa_imp', 'p_ext_imp')})
+    c = dict(tau_tot=5.0, theta0_deg=30.0, n_photons=n)
+    c.update(kw)
+    return Case(bands=[Band(**b)], **c)
+
+
+@pytest.fixture(scope='module')
+def walker():
+    w = GpuWalker(0)
+    yield w
+    w.close()
+
+
+@pytest.mark.parametrize('bad', [dict(g=1.0), dict(g=float('nan')), dict(ssa_ice=1.5), dict(ext_cff=-1.0),
+                                 dict(tau_tot=0.0), dict(tau_tot=float('inf')), dict(p_ext_imp=-0.1),
+                                 dict(r_lambertian=1.2), dict(n_photons=-1)])
+def test_invalid_inputs_are_rejected(walker, bad):
+    with pytest.raises(McsError, match='ARG'):
+        walker.set_cases([good_case(**bad)])
+
+
+def test_call_order_and_arguments(walker):
+    w2 = GpuWalker(0)
+    with pytest.raises(McsError, match='STATE'):
+        w2.walk()                                  # no cases yet
+    w2.set_cases([good_case(100)])
+    with pytest.raises(McsError, match='STATE'):
+        w2.records()                               # nothing walked with records
+    w2.walk(records=False)
+    with pytest.raises(McsError, match='STATE'):
+        w2.records()
+    with pytest.raises(McsError, match='ARG'):
+        w2.walk(photon_band=np.full(100, 3, np.uint8))     # band 3 of a 1-band case
+    with pytest.raises(McsError, match='ARG'):
+        w2.walk(mode='replay')                     # replay needs deviates + bands
+    with pytest.raises(McsError, match='ARG'):
+        w2.set_launch(block=100)                   # not a multiple of 64
+    w2.close()
+
+
+def test_replay_slice_shorter_than_triple(walker):
+    walker.set_cases([good_case(2)])
+    vals = np.full(5, 0.5)
+    offs = np.array([0, 3, 5], np.int64)           # second slice has 2 deviates
+    with pytest.raises(McsError, match='ARG'):
+        walker.walk(mode='replay', replay=(vals, offs), photon_band=np.zeros(2, np.uint8))
+
+
+def test_step_cap_is_reported_and_context_recovers(walker):
+    # a non-absorbing thick layer: walks are long; cap them at 50 steps
+    walker.set_cases([good_case(20_000, ssa_ice=1.0, tau_tot=1e4)])
+    with pytest.raises(McsError, match='STEP_CAP'):
+        walker.walk(records=True, max_steps=50)
+    c = walker.counts()[0]
+    assert c[6] > 0 and c[0] == 20_000            # capped photons counted, all accounted for
+    r = walker.records()
+    assert np.all(r['n_scat'][r['condition'] == 0] == 49)
+    walker.set_cases([good_case(20_000)])
+    walker.walk(records=True)                      # usable afterwards
+    assert walker.counts()[0][0] == 20_000
+
+
+def test_destroy_null_and_errors_text():
+    L = _lib.lib()
+    assert L.mcs_destroy(None) == -1
+    ctx = ctypes.c_void_p()
+    assert L.mcs_create(999, ctypes.byref(ctx)) == -1 and not ctx.value

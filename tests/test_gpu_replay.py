@@ -81,3 +81,32 @@ def test_replay_rejects_short_slice(walker):
     # the context stays usable
     walker.walk(mode='replay', replay=(values, offsets), photon_band=pband)
     check_records(walker.records(), walk.records_to_arrays(records))
+
+
+def test_replay_config_case_1e6_streamed(walker):
+    """The 1e6-photon config case (1 m, 60 deg, omega 0.999, g 0.89, Lambertian
+    bottom 0.5) in replay mode, in ten chunks of 1e5 photons so the recorded
+    deviates stay under 1 GB at a time.  One MT19937 state (seed 11) runs on
+    through all ten oracle chunks; every photon's record must match."""
+    from oracle import coracle as C
+    ext = walk.sphere_ext_cff_mss(100.0, 917.0)
+    tau = 1.0 * ext * 300.0
+    n, k = 1_000_000, 100_000
+    rs = np.random.RandomState(11)
+    wvls = walk.draw_wavelengths(rs, n, 1.3, 1e-12)
+    mt = C.mt_from_numpy(rs)
+    par_all = walk.photon_params(wvls, dict(ssa=0.999, ext=ext, g=0.89), dict(ssa=0.0, ext=0.0),
+                                 0.0, tau, 300.0)
+    total_steps = 0
+    for s in range(0, n, k):
+        print('replay 1e6 chunk', s // k, flush=True)
+        par = {key: v[s:s + k] for key, v in par_all.items()}
+        out = C.walk(par, tau, 300.0, 60.0, True, 0.5, mt=mt, record=True)
+        out['wvn'] = 1. / par['wvl']
+        out['snow_depth'] = par['snow_depth']
+        gcase, pband = case_from_params(par, tau, 300.0, 60.0, True, 0.5)
+        walker.set_cases([gcase])
+        walker.walk(mode='replay', replay=out['stream'], photon_band=pband)
+        check_records(walker.records(), out)
+        total_steps += out['steps']
+    assert total_steps > 1.5e8
