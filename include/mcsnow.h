@@ -115,6 +115,11 @@ typedef struct {
     int32_t blocks_per_cu;     /* grid = blocks_per_cu * CUs                  */
     int32_t chunk;             /* photons per work-queue chunk (>= 64)        */
     int32_t refill_min;        /* idle lanes that trigger a wave refill 1..64 */
+    int32_t coop_live;         /* native mode, once the work queue is empty: a
+                                  wave with at most this many live photons
+                                  (0..32) walks them cooperatively, 64 / L
+                                  lanes per photon stepping it in parallel;
+                                  0 = off.  Default 16.                       */
 } mcs_launch;
 
 /* Host output arrays (each sum(n_photons) long, photon order; any may be
@@ -178,7 +183,8 @@ int mcs_read_tallies(mcs_ctx *ctx, double *out, int32_t n_cases);
 
 /* Per-case counts (n_cases x MCS_TALLY_U64):
  *   0 photons, 1..5 photons by condition, 6 capped, 7 phi on an axis,
- *   8 photon-steps, 9..15 reserved (0) */
+ *   8 photon-steps, 9 photon-steps walked in the cooperative tail mode
+ *   (mcs_launch.coop_live), 10..15 reserved (0) */
 int mcs_read_counts(mcs_ctx *ctx, uint64_t *out, int32_t n_cases);
 
 int mcs_last_stats(const mcs_ctx *ctx, mcs_stats *out);

@@ -51,12 +51,24 @@ class McsWalkArgs(ctypes.Structure):
                 ('use_graph', ctypes.c_int32)]
 
 
+DEFAULT_COOP_LIVE = 16
+
+
 class McsLaunch(ctypes.Structure):
     _fields_ = [('block', ctypes.c_int32), ('blocks_per_cu', ctypes.c_int32),
-                ('chunk', ctypes.c_int32), ('refill_min', ctypes.c_int32)]
+                ('chunk', ctypes.c_int32), ('refill_min', ctypes.c_int32),
+                ('coop_live', ctypes.c_int32)]
+
+    @classmethod
+    def of(cls, shape):
+        """(block, blocks_per_cu, chunk, refill_min[, coop_live])."""
+        shape = tuple(int(v) for v in shape)
+        if len(shape) == 4:
+            shape += (DEFAULT_COOP_LIVE,)
+        return cls(*shape)
 
     def as_tuple(self):
-        return (self.block, self.blocks_per_cu, self.chunk, self.refill_min)
+        return (self.block, self.blocks_per_cu, self.chunk, self.refill_min, self.coop_live)
 
 
 class McsRecords(ctypes.Structure):

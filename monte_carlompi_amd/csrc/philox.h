@@ -13,23 +13,6 @@ struct u32x4 {
     uint32_t x, y, z, w;
 };
 
-__host__ __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
-{
-#pragma unroll
-    for (int r = 0; r < 10; ++r) {
-        if (r) {
-            k0 += 0x9E3779B9u;
-            k1 += 0xBB67AE85u;
-        }
-        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
-        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
-        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
-        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
-        c = u32x4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
-    }
-    return c;
-}
-
 // Round keys pass through an empty volatile asm so that the schedule is
 // recomputed with two scalar adds per round inside the walk loop, instead of
 // twenty round keys being held live in SGPRs (which spills them to VGPR lanes).
@@ -45,6 +28,23 @@ __device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t k)
     uint32_t r;
     asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "s"(k));
     return r;
+}
+
+// Philox4x32-10 of counter c under key (k0, k1).
+__device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) {
+            k0 = sched(k0 + 0x9E3779B9u);
+            k1 = sched(k1 + 0xBB67AE85u);
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+        c = u32x4{xor3((uint32_t)(p1 >> 32), c.y, k0), (uint32_t)p1,
+                  xor3((uint32_t)(p0 >> 32), c.w, k1), (uint32_t)p0};
+    }
+    return c;
 }
 
 // Per-photon part of round 1 for counters (photon, case, step, stream):

@@ -46,7 +46,7 @@ constexpr int T_QCOND = MCS_BRF_BINS + 1;    // 649..653 by condition 1..5
 constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
 constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
 // counts (uint64)
-constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8;
+constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9;
 
 struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
@@ -84,7 +84,7 @@ struct KParams {
     uint32_t refill_min;
     uint32_t key0, key1;
     uint32_t max_steps;
-    int32_t pad;
+    int32_t coop_live;           // tail mode at <= this many live photons per wave (0: off)
     const uint8_t *photon_band;  // nullable
     const double *rdev;          // replay
     const int64_t *roff;
@@ -150,10 +150,12 @@ __device__ __forceinline__ T lds_load(const T *p)
 // walks of a launch.
 struct NativeWalker {
     using Real = float;
+    static constexpr bool COOP = true;        // has the wave-cooperative tail mode below
     static constexpr int MAX_BLOCK = 1024;
     static constexpr int WAVES_PER_EU = 5;   // <= 96 VGPRs: 5 waves per SIMD without spills
     float z, mux, muy, muz, tau_acc;
     uint32_t i, p, c;
+    uint32_t ci;                 // steps walked in the cooperative tail mode
     int band;
     bool bounce;
     u32x4 r;                     // uniforms of the next step
@@ -198,6 +200,7 @@ struct NativeWalker {
         t_refl = C.t_refl;
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0; p = photon; c = (uint32_t)case_id;
+        ci = 0;
         bounce = false;
         cond = 0;
         pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
@@ -294,6 +297,7 @@ struct NativeWalker {
 // product and sum rounds as numpy's float64 scalars do.
 struct ReplayWalker {
     using Real = double;
+    static constexpr bool COOP = false;
     static constexpr int MAX_BLOCK = 256;
     static constexpr int WAVES_PER_EU = 2;
     double z, mux, muy, muz, muz2, path;
@@ -440,6 +444,228 @@ struct ReplayWalker {
     }
 };
 
+
+// Lanes are EMPTY (no photon), LIVE or DONE (finished, record pending).
+enum { EMPTY = 0, LIVE = 1, DONE = 2 };
+
+// ------------------------------------------------- wave-cooperative tail mode
+// Once a block's queue is empty, a wave's last photons would walk one step
+// per wave instruction stream (~1,100 cycles per step for a lone wave), and
+// the longest walks of a launch (~1e4 steps) set its duration.  Here the
+// G = 64 / L lanes of a group walk G consecutive steps of one of the L live
+// photons at once:
+//  * lane j of the group draws the Philox block of step k = i + 1 + j -- the
+//    same counter the one-lane walk uses, so every step keeps its uniforms;
+//  * a step is "scatter by (theta_k, phi_k) about the current direction, then
+//    move dtau_k": with the direction carried as the third column of a frame
+//    quaternion F, F_k = F_{k-1} (x) L_k (L_k = Rz(phi_k) Ry(theta_k)), so all
+//    G frames are one inclusive product scan (DPP row shifts + row
+//    broadcasts, lower lanes on the left); depth z_k and optical path are
+//    prefix sums of dtau_k * uz_k and dtau_k;
+//  * the first lane whose step exits or absorbs (or reaches max_steps) ends
+//    the segment; the photon's lane takes that lane's state, otherwise the
+//    last lane's, and continues.
+// Azimuths are measured in the transported frame instead of the lab-axis
+// frame of mc3D:1255-1281.  phi_k is uniform and independent of the history,
+// so the walk has the same law; events use the uniforms of their own step,
+// so no step is re-drawn.
+
+struct Quat {
+    float w, x, y, z;
+};
+
+__device__ __forceinline__ Quat qmul(const Quat &A, const Quat &B)
+{
+    return Quat{fmaf(A.w, B.w, -fmaf(A.x, B.x, fmaf(A.y, B.y, A.z * B.z))),
+                fmaf(A.w, B.x, fmaf(A.x, B.w, fmaf(A.y, B.z, -A.z * B.y))),
+                fmaf(A.w, B.y, fmaf(-A.x, B.z, fmaf(A.y, B.w, A.z * B.x))),
+                fmaf(A.w, B.z, fmaf(A.x, B.y, fmaf(-A.y, B.x, A.z * B.w)))};
+}
+
+// DPP lane move: CTRL 0x111..0x118 row_shr:1..8, 0x142 row_bcast:15,
+// 0x143 row_bcast:31; rows outside ROWS and lanes without a source get `old`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ float dppf(float old, float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, ROWS,
+                                                      0xF, false));
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void qlevel(Quat &q, bool take)
+{
+    const Quat y{dppf<CTRL, ROWS>(1.f, q.w), dppf<CTRL, ROWS>(0.f, q.x), dppf<CTRL, ROWS>(0.f, q.y),
+                 dppf<CTRL, ROWS>(0.f, q.z)};
+    const Quat m = qmul(y, q);
+    q.w = take ? m.w : q.w;
+    q.x = take ? m.x : q.x;
+    q.y = take ? m.y : q.y;
+    q.z = take ? m.z : q.z;
+}
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ void slevel(float &s0, float &s1, bool take)
+{
+    const float t0 = dppf<CTRL, ROWS>(0.f, s0), t1 = dppf<CTRL, ROWS>(0.f, s1);
+    s0 = take ? s0 + t0 : s0;
+    s1 = take ? s1 + t1 : s1;
+}
+
+// Inclusive scans within groups of G = 2^lg lanes (jj = lane within group):
+// q <- q_0 (x) q_1 (x) ... (x) q_jj.  Levels 1..8 stay inside 16-lane rows;
+// row_bcast:15 then joins row pairs (rows 1, 3) and row_bcast:31 the halves.
+__device__ __forceinline__ void group_scan_q(Quat &q, int lg, int jj, int lane)
+{
+    if (lg >= 1) qlevel<0x111, 0xF>(q, jj >= 1);
+    if (lg >= 2) qlevel<0x112, 0xF>(q, jj >= 2);
+    if (lg >= 3) qlevel<0x114, 0xF>(q, jj >= 4);
+    if (lg >= 4) qlevel<0x118, 0xF>(q, jj >= 8);
+    if (lg >= 5) qlevel<0x142, 0xA>(q, (lane & 16) != 0);
+    if (lg >= 6) qlevel<0x143, 0xC>(q, lane >= 32);
+}
+
+// the same for two prefix sums
+__device__ __forceinline__ void group_scan_s(float &s0, float &s1, int lg, int jj, int lane)
+{
+    if (lg >= 1) slevel<0x111, 0xF>(s0, s1, jj >= 1);
+    if (lg >= 2) slevel<0x112, 0xF>(s0, s1, jj >= 2);
+    if (lg >= 3) slevel<0x114, 0xF>(s0, s1, jj >= 4);
+    if (lg >= 4) slevel<0x118, 0xF>(s0, s1, jj >= 8);
+    if (lg >= 5) slevel<0x142, 0xA>(s0, s1, (lane & 16) != 0);
+    if (lg >= 6) slevel<0x143, 0xC>(s0, s1, lane >= 32);
+}
+
+__device__ __forceinline__ float bperm(int src_lane, float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
+}
+__device__ __forceinline__ uint32_t bperm(int src_lane, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
+}
+
+// One cooperative round of a wave (all 64 lanes call; no divergence around
+// the lane permutes).  `own` is the wave's 32-entry LDS table.
+__device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *own)
+{
+    const int lane = lane_id();
+    // photons that cannot join take one ordinary step: the first step keeps
+    // the incident direction, a Lambertian re-emission and the surface mode
+    // have their own direction law
+    const bool live = state == LIVE;
+    const bool elig = live && w.i >= 1u && !w.bounce && !w.surf;
+    if (__ballot(live && !elig)) {
+        if (live && !elig) {
+            const bool d = w.step(P);
+            if (d || w.i >= P.max_steps) {
+                if (!d) w.cond = 0;
+                state = DONE;
+            }
+        }
+    }
+    const unsigned long long em = __ballot(elig);
+    if (em == 0ull) return;
+    const int L = __popcll(em);
+    const int lg = L <= 1 ? 6 : 6 - (32 - __clz(L - 1));   // G = 2^lg lanes per photon
+    const int G = 1 << lg;
+    const int g = lane >> lg;
+    const int jj = lane & (G - 1);
+    const bool member = g < L;
+    // owner table: group g walks the live photon of rank g
+    const int rk = lane_rank(em);
+    if (elig) __atomic_store_n(&own[rk], lane, __ATOMIC_RELAXED);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int owner = member ? __atomic_load_n(&own[g], __ATOMIC_RELAXED) : lane;
+    // the photon's state, from its lane
+    PhiloxPre pre{bperm(owner, w.pre.ck0), bperm(owner, w.pre.h0), bperm(owner, w.pre.l0)};
+    const float a = bperm(owner, w.a), b = bperm(owner, w.b), cc = bperm(owner, w.cc);
+    const float d = bperm(owner, w.d), e = bperm(owner, w.e);
+    const uint32_t t_abs = bperm(owner, w.t_abs_all);
+    const float tau_tot = bperm(owner, w.tau_tot);
+    const float z0 = bperm(owner, w.z), tau0 = bperm(owner, w.tau_acc);
+    const float dx = bperm(owner, w.mux), dy = bperm(owner, w.muy), dz = bperm(owner, w.muz);
+    const uint32_t k = bperm(owner, w.i) + 1u + (uint32_t)jj;
+    // step k's uniforms and local rotation (same draws as NativeWalker::step)
+    const u32x4 cur = philox_step(pre, k, P.key0, P.key1);
+    const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
+    const float u0 = u01f(cur.x);
+    const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u0, d));
+    const float cost = (e == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+    const float hc = __builtin_amdgcn_sqrtf(fmaxf(fmaf(0.5f, cost, 0.5f), 0.f));   // cos(theta/2)
+    const float hs = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-0.5f, cost, 0.5f), 0.f));  // sin(theta/2)
+    const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
+    const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
+    Quat q{ha * hc, -hb * hs, ha * hs, hb * hc};
+    if (jj == 0) {
+        // frame of the current direction d: the rotation taking e_z to d
+        const bool up = dz >= 0.f;
+        Quat f = up ? Quat{1.f + dz, -dy, dx, 0.f} : Quat{-dy, 1.f - dz, 0.f, dx};
+        const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
+        f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
+        q = qmul(f, q);
+    }
+    // frames first: the depth change of step k needs its direction
+    group_scan_q(q, lg, jj, lane);
+    const float uz = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
+    float sz = dtau * uz, sp = dtau;
+    group_scan_s(sz, sp, lg, jj, lane);
+    const float zk = z0 + sz, pk = tau0 + sp;
+    const bool top = zk > 0.f && uz > 0.f;
+    const bool bot = zk < -tau_tot && uz < 0.f;
+    const bool ab = cur.w < t_abs;
+    const bool cap = k >= P.max_steps;
+    const unsigned long long ev = __ballot(member && (top || bot || ab || cap));
+    // this lane's outcome, should its step end the segment
+    const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
+    const float corr = top ? zk : zk + tau_tot;
+    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
+    const float zout = bot ? -tau_tot : zk;
+    const float ox = 2.f * fmaf(q.x, q.z, q.w * q.y);
+    const float oy = 2.f * fmaf(q.y, q.z, -q.w * q.x);
+    // the photon's lane reads the first event lane of its group (or the last)
+    int src = lane;
+    if (elig) {
+        const int base = rk << lg;
+        const unsigned long long gm = (ev >> base) & (lg == 6 ? ~0ull : ((1ull << G) - 1ull));
+        src = base + (gm ? (__ffsll((long long)gm) - 1) : G - 1);
+    }
+    const int r_type = (int)bperm(src, (uint32_t)type);
+    const uint32_t r_k = bperm(src, k);
+    const uint32_t r_w = bperm(src, cur.w);
+    const float r_z = bperm(src, zout), r_p = bperm(src, pout);
+    const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uz);
+    if (elig) {
+        w.ci += r_k - w.i;
+        w.i = r_k;
+        w.z = r_z;
+        w.tau_acc = r_p;
+        w.mux = r_x; w.muy = r_y; w.muz = r_uz;
+        if (r_type == 1) {
+            w.cond = 1;
+            state = DONE;
+        } else if (r_type == 2) {
+            bool refl = false;
+            if (w.lamb) {
+                const u32x4 qq = philox4x32_10(u32x4{w.p, w.c, w.i, 1u}, P.key0, P.key1);
+                refl = (unsigned long long)qq.y < w.t_refl;
+            }
+            if (refl && w.i < P.max_steps) {
+                w.bounce = true;
+                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
+            } else {
+                w.cond = refl ? 0 : 2;
+                state = DONE;
+            }
+        } else if (r_type == 3) {
+            w.cond = (r_w < w.t_abs5) ? 5 : 4;
+            state = DONE;
+        } else if (r_type == 4) {
+            w.cond = 0;
+            state = DONE;
+        }
+    }
+}
+
 // ------------------------------------------------------------ chunk queue
 // Take chunk `seq` of this block into slot seq % SLOTS (whole wave calls).
 // Returns false when the global queue is empty.
@@ -517,9 +743,6 @@ __device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint6
     ((R *)P.rec_path)[pos] = pa;
 }
 
-// Lanes are EMPTY (no photon), LIVE or DONE (finished, record pending).
-enum { EMPTY = 0, LIVE = 1, DONE = 2 };
-
 // Record + tally the DONE lanes of the wave; flush chunks that complete.
 template <class W>
 __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, int &state,
@@ -549,6 +772,9 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
             unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
             atomicAdd(&T.u[U_N], 1ull);
             atomicAdd(&T.u[U_STEPS], (unsigned long long)w.i);
+            if constexpr (W::COOP) {
+                if (w.ci) atomicAdd(&T.u[U_COOP], (unsigned long long)w.ci);
+            }
             if (w.cond >= 1) {
                 unsafeAtomicAdd(&T.f[T_QCOND + w.cond - 1], wvn);
                 atomicAdd(&T.u[w.cond], 1ull);
@@ -659,6 +885,7 @@ __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_
 k_walk(const KParams P)
 {
     __shared__ BlockState bs;
+    __shared__ int own_tab[W::COOP ? W::MAX_BLOCK / 64 : 1][32];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     // ---- init: zero LDS, wave 0 takes the first chunk
@@ -692,8 +919,18 @@ k_walk(const KParams P)
             if (++spins > (1 << 28)) break;  // cannot happen: chunks always finish
             continue;
         }
-        // leave the hot loop once `thresh` lanes are not live
-        const int thresh = lds_load(&bs.exhausted) ? 64 : min(64, (64 - __popcll(live)) + (int)P.refill_min);
+        if constexpr (W::COOP) {
+            // queue empty and few photons left: walk them cooperatively
+            if (lds_load(&bs.exhausted) && __popcll(live) <= P.coop_live) {
+                __builtin_amdgcn_s_setprio(3);
+                while (__ballot(state == LIVE)) coop_round(w, state, P, own_tab[wave]);
+                continue;
+            }
+        }
+        // leave the hot loop once `thresh` lanes are not live (with the queue
+        // empty: once the cooperative tail can take over)
+        const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - P.coop_live : 64)
+                                                   : min(64, (64 - __popcll(live)) + (int)P.refill_min);
         // ---------------- hot: step every live photon
         for (uint32_t iter = 1;; ++iter) {
             if (state == LIVE) {
@@ -769,7 +1006,7 @@ struct mcs_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     char err[512] = {0};
-    mcs_launch launch{256, 4, 2048, 8};
+    mcs_launch launch{256, 4, 2048, 8, 16};
     // host copies
     std::vector<mcs_case> cases;
     std::vector<mcs_band> bands;
@@ -966,7 +1203,7 @@ int mcs_set_launch(mcs_ctx *c, const mcs_launch *l)
 {
     if (!c || !l) return MCS_ERR_ARG;
     if (l->block < 64 || l->block > 1024 || l->block % 64 || l->blocks_per_cu < 1 || l->blocks_per_cu > 64 ||
-        l->chunk < 64 || l->refill_min < 1 || l->refill_min > 64)
+        l->chunk < 64 || l->refill_min < 1 || l->refill_min > 64 || l->coop_live < 0 || l->coop_live > 32)
         return set_err(c, MCS_ERR_ARG, "launch shape out of range");
     c->launch = *l;
     c->gkey = 0;
@@ -1108,6 +1345,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     P.chunk = chunk;
     P.n_chunks = (uint32_t)nch;
     P.refill_min = (uint32_t)c->launch.refill_min;
+    P.coop_live = c->launch.coop_live;
     P.key0 = (uint32_t)a->seed;
     P.key1 = (uint32_t)(a->seed >> 32);
     P.max_steps = a->max_steps ? (uint32_t)a->max_steps : (1u << 24);

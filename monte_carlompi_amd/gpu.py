@@ -22,7 +22,7 @@ T_QDOWN = BRF_BINS
 T_QCOND = BRF_BINS + 1          # +cond-1 for condition 1..5
 T_QCAP = BRF_BINS + 6
 T_QPATH = BRF_BINS + 7
-U_N, U_CAP, U_AXIS, U_STEPS = 0, 6, 7, 8
+U_N, U_CAP, U_AXIS, U_STEPS, U_COOP = 0, 6, 7, 8, 9
 
 
 @dataclass
@@ -122,8 +122,9 @@ class GpuWalker:
         self._band_wvn = [np.array([1.0 / b.wvl for b in c.bands]) for c in cases]
         self._band_depth = [np.array([c.tau_tot / b.ext_cff for b in c.bands]) for c in cases]
 
-    def set_launch(self, block=256, blocks_per_cu=4, chunk=2048, refill_min=8):
-        L = McsLaunch(block, blocks_per_cu, chunk, refill_min)
+    def set_launch(self, block=256, blocks_per_cu=4, chunk=2048, refill_min=8,
+                   coop_live=_lib.DEFAULT_COOP_LIVE):
+        L = McsLaunch(block, blocks_per_cu, chunk, refill_min, coop_live)
         self._check(lib().mcs_set_launch(self._ctx, ctypes.byref(L)))
 
     def get_launch(self):
@@ -204,7 +205,7 @@ class GpuWalker:
                     phi_unbound=s.phi_unbound, kernel_ms=s.kernel_ms)
 
     def autotune(self, candidates, photons_cap=0, repeats=3, seed=1):
-        arr = (McsLaunch * len(candidates))(*[McsLaunch(*c) for c in candidates])
+        arr = (McsLaunch * len(candidates))(*[McsLaunch.of(c) for c in candidates])
         times = np.zeros(len(candidates))
         best = McsLaunch()
         self._check(lib().mcs_autotune(self._ctx, arr, len(candidates), int(photons_cap),

@@ -37,7 +37,7 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(_lib.McsCase) == 56
     assert ctypes.sizeof(_lib.McsBand) == 64
     assert ctypes.sizeof(_lib.McsWalkArgs) == 56
-    assert ctypes.sizeof(_lib.McsLaunch) == 16
+    assert ctypes.sizeof(_lib.McsLaunch) == 20
     assert ctypes.sizeof(_lib.McsRecords) == 48
     assert ctypes.sizeof(_lib.McsStats) == 40
 

@@ -1,0 +1,128 @@
+"""The cooperative tail mode (mcs_launch.coop_live): once a block's queue is
+empty, waves with few live photons walk each photon with 64 / L lanes, one
+step per lane, through a quaternion frame scan.  Launches with about one
+chunk per wave put most photon-steps into that mode; their results must match
+the CPU oracle and the one-lane walk at 3 sigma / chi^2, and the known
+answers of van de Hulst."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from oracle import coracle
+from tests.gpu_cases import flat_params
+from tests.test_gpu_native import EXT100, TAU_1M, brf_counts, gpu_case, two_sample_z
+
+pytestmark = pytest.mark.gpu
+
+U_STEPS, U_COOP = 8, 9
+
+
+@pytest.fixture(scope='module')
+def walker():
+    from monte_carlompi_amd.gpu import GpuWalker
+    w = GpuWalker(0)
+    yield w
+    w.close()
+
+
+def coop_runs(walker, case_fn, seeds, coop_live, records=True):
+    """One 64-photon chunk per wave (64-thread blocks, one per CU)."""
+    n = 64 * walker.cus
+    walker.set_cases([case_fn(n)])
+    walker.set_launch(64, 1, 64, 1, coop_live)
+    recs, counts = [], np.zeros(16, np.uint64)
+    for s in seeds:
+        walker.walk(mode='native', seed=s, records=records)
+        counts += walker.counts()[0]
+        if records:
+            recs.append(walker.records())
+    walker.set_launch()
+    out = {k: np.concatenate([r[k] for r in recs]) for k in recs[0]} if records else {}
+    return out, counts
+
+
+def test_coop_dominated_config_case_matches_oracle(walker):
+    """1 m snow, 60 deg, omega 0.999, g 0.89, Lambertian bottom 0.5."""
+    case = lambda n: gpu_case(n, TAU_1M, 60.0, True, 0.5)
+    print('coop config case: gpu', flush=True)
+    g, cnt = coop_runs(walker, case, range(100, 112), 32)
+    frac = float(cnt[U_COOP]) / float(cnt[U_STEPS])
+    print('steps walked cooperatively: %.3f' % frac, flush=True)
+    assert frac > 0.5
+    print('coop config case: cpu oracle', flush=True)
+    cpu = coracle.run_ranks(300, 8, 25_000,
+                            lambda n: flat_params(n, 1.3, 0.89, 0.999, EXT100, TAU_1M, 300.0),
+                            TAU_1M, 300.0, 60.0, True, 0.5)
+    n_g, n_c = g['condition'].size, cpu['condition'].size
+    for cond in (1, 2, 4):
+        z = two_sample_z(int(np.sum(g['condition'] == cond)), n_g, int(np.sum(cpu['condition'] == cond)), n_c)
+        assert abs(z) < 3.0, (cond, z)
+    hg = brf_counts(g['theta_n'][g['condition'] == 1], g['phi_n'][g['condition'] == 1])
+    hc = brf_counts(cpu['theta_n'][cpu['condition'] == 1], cpu['phi_n'][cpu['condition'] == 1])
+    keep = (hg + hc) >= 10
+    A, B = n_g, n_c
+    chi2 = np.sum((np.sqrt(B / A) * hg[keep] - np.sqrt(A / B) * hc[keep]) ** 2 / (hg[keep] + hc[keep]))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, chi2
+    # step counts and path lengths: distributions, not only means
+    bins = np.unique(np.round(np.geomspace(1, 2e4, 40)).astype(int))
+    cg = np.histogram(g['n_scat'] + 1, bins)[0]
+    cc = np.histogram(cpu['n_scat'] + 1, bins)[0]
+    keep = (cg + cc) >= 10
+    chi2 = np.sum((np.sqrt(B / A) * cg[keep] - np.sqrt(A / B) * cc[keep]) ** 2 / (cg[keep] + cc[keep]))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, chi2
+    refl_g = g['path_length'][g['condition'] == 1]
+    refl_c = cpu['path_length'][cpu['condition'] == 1]
+    assert stats.ks_2samp(refl_g, refl_c).pvalue > 1e-3
+    assert int(cnt[U_STEPS]) == int(np.sum(g['n_scat'] + 1))
+
+
+def test_coop_and_one_lane_walks_agree(walker):
+    """Same launch shape, tail mode on (32) and off (0): exit fractions, BRF
+    and step counts agree at 3 sigma / chi^2."""
+    case = lambda n: gpu_case(n, 40.0, 30.0, False, 0.0, g=0.85, ssa=0.9995)
+    a, ca = coop_runs(walker, case, range(10), 32)
+    b, cb = coop_runs(walker, case, range(10, 20), 0)
+    assert cb[U_COOP] == 0 and ca[U_COOP] > 0.5 * ca[U_STEPS]
+    n_a, n_b = a['condition'].size, b['condition'].size
+    for cond in (1, 2, 4):
+        z = two_sample_z(int(np.sum(a['condition'] == cond)), n_a, int(np.sum(b['condition'] == cond)), n_b)
+        assert abs(z) < 3.0, (cond, z)
+    ha = brf_counts(a['theta_n'][a['condition'] == 1], a['phi_n'][a['condition'] == 1])
+    hb = brf_counts(b['theta_n'][b['condition'] == 1], b['phi_n'][b['condition'] == 1])
+    keep = (ha + hb) >= 10
+    chi2 = np.sum((np.sqrt(n_b / n_a) * ha[keep] - np.sqrt(n_a / n_b) * hb[keep]) ** 2 / (ha[keep] + hb[keep]))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, chi2
+    assert stats.ks_2samp(a['n_scat'], b['n_scat']).pvalue > 1e-3
+    # transmitted photons: exit cosines
+    ta, tb = a['theta_n'][a['condition'] == 2], b['theta_n'][b['condition'] == 2]
+    assert stats.ks_2samp(ta, tb).pvalue > 1e-3
+
+
+def test_coop_van_de_hulst_known_answer(walker):
+    """tau 2, omega 0.9, g 0.75, mu0 1, free bottom: R 0.09739, T 0.66096."""
+    case = lambda n: gpu_case(n, 2.0, 0.0, False, 0.0, wvl=0.5, g=0.75, ssa=0.9)
+    _, c = coop_runs(walker, case, range(40), 32, records=False)
+    n = float(c[0])
+    R, T = c[1] / n, (c[2] + c[3]) / n
+    assert c[U_COOP] > 0.2 * c[U_STEPS]
+    assert abs(R - 0.09739) < 3 * np.sqrt(0.09739 * (1 - 0.09739) / n) + 2e-5
+    assert abs(T - 0.66096) < 3 * np.sqrt(0.66096 * (1 - 0.66096) / n) + 2e-5
+
+
+def test_coop_step_cap(walker):
+    """A photon capped inside a cooperative segment stops at exactly max_steps."""
+    from monte_carlompi_amd._lib import McsError
+    case = gpu_case(64 * walker.cus, 1e4, 0.0, False, 0.0, ssa=1.0)
+    walker.set_cases([case])
+    walker.set_launch(64, 1, 64, 1, 32)
+    try:
+        with pytest.raises(McsError, match='STEP_CAP'):
+            walker.walk(mode='native', seed=9, records=True, max_steps=300)
+        r = walker.records()
+        c = walker.counts()[0]
+        assert np.all(r['n_scat'] <= 299)
+        assert int(np.sum(r['condition'] == 0)) == int(c[6]) > 0
+        assert np.all(r['n_scat'][r['condition'] == 0] == 299)
+        assert c[U_COOP] > 0
+    finally:
+        walker.set_launch()

@@ -52,6 +52,8 @@ def test_call_order_and_arguments(walker):
         w2.walk(mode='replay')                     # replay needs deviates + bands
     with pytest.raises(McsError, match='ARG'):
         w2.set_launch(block=100)                   # not a multiple of 64
+    with pytest.raises(McsError, match='ARG'):
+        w2.set_launch(coop_live=33)                # tail mode serves at most 32 photons
     w2.close()
 
 
