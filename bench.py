@@ -21,8 +21,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon MC'
-SNOW_1M_CANDIDATES = [(256, 4, 2048, 8), (256, 5, 2048, 8), (256, 4, 1024, 4), (512, 2, 2048, 8),
-                      (256, 5, 1024, 16), (128, 8, 1024, 8)]
+# (block, blocks_per_cu, chunk, refill_min, coop_live): small chunks spread the
+# 1e6 photons over every block; the cooperative tail mode finishes the walks
+# still running when the queue is empty
+SNOW_1M_CANDIDATES = [(256, 5, 512, 16, 16), (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 16),
+                      (256, 4, 1024, 16, 16), (256, 5, 512, 8, 16), (256, 5, 1024, 16, 16),
+                      (1024, 1, 4096, 16, 16), (256, 5, 512, 16, 24)]
 N_PHOTONS = 1_000_000
 
 
@@ -37,8 +41,8 @@ def main():
     ap.add_argument('--cpu-procs', type=int, default=0)
     ap.add_argument('--no-sweep', action='store_true', help='skip the batched sweep block')
     ap.add_argument('--launch', type=str, default='',
-                    help='fixed launch shape block,blocks_per_cu,chunk,refill_min (skips the autotuner; '
-                         'for profiling runs)')
+                    help='fixed launch shape block,blocks_per_cu,chunk,refill_min[,coop_live] (skips the '
+                         'autotuner; for profiling runs)')
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get('WORLD_SIZE', '1')) > 1:
         print('bench.py measures one GPU: the photon walk is built and measured on a single '
@@ -73,6 +77,8 @@ def main():
     w.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    cnt = w.counts()[0]
+    coop_frac = float(cnt[9]) / max(float(cnt[8]), 1.0)
     fr = fractions_from_tallies(w.tallies())
     # PCIe-inclusive: the same walk plus the D2H copy of all records (not `value`)
     t2 = time.perf_counter()
@@ -107,6 +113,7 @@ def main():
         'pcie_inclusive_photons_per_s': round(args.photons / (pcie_ms * 1e-3), 1),
         'kernel_ms_avg': round(avg_kern, 4),
         'launch': list(shape),
+        'coop_steps_frac': round(coop_frac, 4),
         'albedo': round(float(fr['albedo'][0]), 6),
         'roofline': {
             'bound': 'valu', 'achieved': round(rf['fp32_tflops'], 4),

@@ -108,8 +108,13 @@ typedef struct {
     int32_t use_graph;         /* 1: launch through a captured hipGraph       */
 } mcs_walk_args;
 
-/* Launch shape of the persistent walk kernel (autotuned; results do not
- * depend on it: each photon's Philox stream is keyed by (case, photon)). */
+/* Launch shape of the persistent walk kernel (autotuned).  Each photon's
+ * Philox stream is keyed by (seed, case, photon): with coop_live = 0 records
+ * are bit-identical for every shape.  With coop_live > 0, the photons that are
+ * still walking when the queue runs dry finish in the cooperative mode, whose
+ * azimuths are measured in a transported frame and whose depths are prefix
+ * sums: the same law (statistics agree), but which photons take that path
+ * depends on scheduling, so records are not bit-reproducible. */
 typedef struct {
     int32_t block;             /* threads per block: 64..1024, multiple of 64 */
     int32_t blocks_per_cu;     /* grid = blocks_per_cu * CUs                  */

@@ -13,7 +13,8 @@
 //
 // Two walkers share that skeleton:
 //  * NativeWalker: FP32, Philox4x32-10 keyed by the seed, counter
-//    (photon, case, step, stream) -- results do not depend on launch shape;
+//    (photon, case, step, stream); once the queue is empty, sparse waves
+//    finish their photons in the wave-cooperative mode below;
 //  * ReplayWalker: FP64 with the reference's exact step semantics
 //    (monte_carloMPI/monte_carlo3D.py:1212-1490), consuming the reference's
 //    uniform deviates from CSR slices.
@@ -154,15 +155,14 @@ struct NativeWalker {
     static constexpr int MAX_BLOCK = 1024;
     static constexpr int WAVES_PER_EU = 5;   // <= 96 VGPRs: 5 waves per SIMD without spills
     float z, mux, muy, muz, tau_acc;
-    uint32_t i, p, c;
-    uint32_t ci;                 // steps walked in the cooperative tail mode
+    uint32_t i;
     int band;
     bool bounce;
     u32x4 r;                     // uniforms of the next step
-    // per-photon parameters
-    float a, b, cc, d, e, tau_tot, inv_ext;
+    // per-photon parameters (the photon and case ids live in `pre`; the
+    // bottom reflectance and 1 / ext_cff are read from the tables when needed)
+    float a, b, cc, d, e, tau_tot;
     uint32_t t_abs5, t_abs_all;  // absorbed as condition 5 iff bits < t_abs5, else 4 iff < t_abs_all
-    unsigned long long t_refl;
     bool iso, lamb, surf;
     int cond;
     PhiloxPre pre;
@@ -184,7 +184,6 @@ struct NativeWalker {
         band = bnd;
         a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
         iso = B.iso != 0;
-        inv_ext = B.inv_ext;
         surf = C.surf != 0;
         if (surf) {
             // Lambertian surface: absorbed with probability 1 - R, split by species
@@ -197,10 +196,8 @@ struct NativeWalker {
         }
         tau_tot = C.tau_tot_f;
         lamb = C.lamb != 0;
-        t_refl = C.t_refl;
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
-        i = 0; p = photon; c = (uint32_t)case_id;
-        ci = 0;
+        i = 0;
         bounce = false;
         cond = 0;
         pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
@@ -259,12 +256,9 @@ struct NativeWalker {
         if (bot) {                                   // mc3D:1399-1459
             tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
-            if (lamb) {
-                const u32x4 q = philox4x32_10(u32x4{p, c, i, 1u}, P.key0, P.key1);
-                if ((unsigned long long)q.y < t_refl) {
-                    bounce = true;
-                    return false;
-                }
+            if (lamb && reflects(P)) {
+                bounce = true;
+                return false;
             }
             cond = first ? 3 : 2;
             return true;
@@ -276,10 +270,24 @@ struct NativeWalker {
         return true;
     }
 
-    __device__ void result(float &theta, float &phi, float &path, bool &axis) const
+    // ids from the Philox words: ck0 = case ^ k0, l0 = lo(M0 * photon) with
+    // M0 odd, so photon = l0 * M0^-1 (mod 2^32), M0^-1 = 0x991A7CDB
+    __device__ __forceinline__ uint32_t case_id(const KParams &P) const { return pre.ck0 ^ P.key0; }
+    __device__ __forceinline__ uint32_t photon_id() const { return pre.l0 * 0x991A7CDBu; }
+
+    // Lambertian bottom: reflected with probability R, from Philox block
+    // (photon, case, step, 1)
+    __device__ __forceinline__ bool reflects(const KParams &P) const
+    {
+        const uint32_t cs = case_id(P);
+        const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 1u}, P.key0, P.key1);
+        return (unsigned long long)q.y < P.cases[cs].t_refl;
+    }
+
+    __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const
     {
         theta = acosf(fminf(fmaxf(muz, -1.f), 1.f));
-        path = tau_acc * inv_ext;
+        path = tau_acc * P.bands[P.cases[case_id].band_begin + band].inv_ext;
         axis = false;
         if (i == 1) {
             phi = 0.f;
@@ -429,7 +437,7 @@ struct ReplayWalker {
         return err;
     }
 
-    __device__ void result(double &theta, double &phi, double &pth, bool &axis) const
+    __device__ void result(const KParams &, int, double &theta, double &phi, double &pth, bool &axis) const
     {
 #pragma clang fp contract(off)
         theta = acos(muz);
@@ -544,53 +552,104 @@ __device__ __forceinline__ uint32_t bperm(int src_lane, uint32_t v)
     return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
 }
 
+// Per-wave state of the cooperative mode.  Groups are sticky: while no photon
+// of the wave exits, is absorbed or joins, each group keeps its photon's state
+// in its own lanes and carries it from its last lane to the next segment; the
+// photon's lane is updated (and groups rebuilt) only after a round with an
+// event.  The photon's constants sit in the wave's LDS table (one 64-byte
+// entry per group), read by the group's lanes each round.
+struct CoopEntry {
+    uint32_t ck0, h0, l0, t_abs;     // Philox words, absorption threshold
+    float a, b, cc, d;               // HG constants
+    float e, tau_tot, z, tau;        // HG constant, layer depth; state at grouping
+    float dx, dy, dz;
+    uint32_t i;
+};
+constexpr int COOP_TAB = 32;         // entries per wave (coop_live <= 32)
+
+struct CoopGroups {
+    unsigned long long em;   // photon lanes of the current grouping (0: rebuild)
+    int lg;                  // G = 2^lg lanes per photon
+    PhiloxPre pre;           // the group's photon's Philox words
+    float z0, tau0, dx, dy, dz;  // its state before the next segment
+    uint32_t i0;
+};
+
+__device__ __forceinline__ float rdlane63(float v)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // One cooperative round of a wave (all 64 lanes call; no divergence around
-// the lane permutes).  `own` is the wave's 32-entry LDS table.
-__device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *own)
+// the lane permutes).  `tab` is the wave's LDS table of COOP_TAB entries.
+__device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab, CoopGroups &cg,
+                           unsigned long long *coop_steps)
 {
     const int lane = lane_id();
     // photons that cannot join take one ordinary step: the first step keeps
     // the incident direction, a Lambertian re-emission and the surface mode
     // have their own direction law
-    const bool live = state == LIVE;
-    const bool elig = live && w.i >= 1u && !w.bounce && !w.surf;
-    if (__ballot(live && !elig)) {
-        if (live && !elig) {
-            const bool d = w.step(P);
-            if (d || w.i >= P.max_steps) {
-                if (!d) w.cond = 0;
-                state = DONE;
+    {
+        const bool live = state == LIVE;
+        const bool solo = live && (w.i < 1u || w.bounce || w.surf);
+        if (__ballot(solo)) {
+            if (solo) {
+                const bool d = w.step(P);
+                if (d || w.i >= P.max_steps) {
+                    if (!d) w.cond = 0;
+                    state = DONE;
+                }
             }
         }
     }
+    const bool elig = state == LIVE && w.i >= 1u && !w.bounce && !w.surf;
     const unsigned long long em = __ballot(elig);
-    if (em == 0ull) return;
-    const int L = __popcll(em);
-    const int lg = L <= 1 ? 6 : 6 - (32 - __clz(L - 1));   // G = 2^lg lanes per photon
+    if (em != cg.em) {
+        if (cg.em) {
+            // a photon joined: hand the carried state back before regrouping
+            const bool was = (cg.em >> lane) & 1ull;
+            const int src = was ? (lane_rank(cg.em) << cg.lg) : lane;
+            const float z = bperm(src, cg.z0), t = bperm(src, cg.tau0);
+            const float x = bperm(src, cg.dx), y = bperm(src, cg.dy), u = bperm(src, cg.dz);
+            const uint32_t i = bperm(src, cg.i0);
+            if (was) {
+                atomicAdd(coop_steps, (unsigned long long)(i - w.i));
+                w.i = i; w.z = z; w.tau_acc = t; w.mux = x; w.muy = y; w.muz = u;
+            }
+        }
+        cg.em = em;
+        if (em == 0ull) return;
+        const int L = __popcll(em);
+        cg.lg = L <= 1 ? 6 : 6 - (32 - __clz(L - 1));
+        // group g walks the live photon of rank g: that lane fills entry g
+        if (elig) {
+            CoopEntry &E = tab[lane_rank(em)];
+            E.ck0 = w.pre.ck0; E.h0 = w.pre.h0; E.l0 = w.pre.l0; E.t_abs = w.t_abs_all;
+            E.a = w.a; E.b = w.b; E.cc = w.cc; E.d = w.d;
+            E.e = w.e; E.tau_tot = w.tau_tot; E.z = w.z; E.tau = w.tau_acc;
+            E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int g = min(lane >> cg.lg, L - 1);
+        const CoopEntry &E = tab[g];
+        cg.pre = PhiloxPre{E.ck0, E.h0, E.l0};
+        cg.z0 = E.z; cg.tau0 = E.tau; cg.dx = E.dx; cg.dy = E.dy; cg.dz = E.dz; cg.i0 = E.i;
+    }
+    const int lg = cg.lg;
     const int G = 1 << lg;
-    const int g = lane >> lg;
     const int jj = lane & (G - 1);
-    const bool member = g < L;
-    // owner table: group g walks the live photon of rank g
-    const int rk = lane_rank(em);
-    if (elig) __atomic_store_n(&own[rk], lane, __ATOMIC_RELAXED);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int owner = member ? __atomic_load_n(&own[g], __ATOMIC_RELAXED) : lane;
-    // the photon's state, from its lane
-    PhiloxPre pre{bperm(owner, w.pre.ck0), bperm(owner, w.pre.h0), bperm(owner, w.pre.l0)};
-    const float a = bperm(owner, w.a), b = bperm(owner, w.b), cc = bperm(owner, w.cc);
-    const float d = bperm(owner, w.d), e = bperm(owner, w.e);
-    const uint32_t t_abs = bperm(owner, w.t_abs_all);
-    const float tau_tot = bperm(owner, w.tau_tot);
-    const float z0 = bperm(owner, w.z), tau0 = bperm(owner, w.tau_acc);
-    const float dx = bperm(owner, w.mux), dy = bperm(owner, w.muy), dz = bperm(owner, w.muz);
-    const uint32_t k = bperm(owner, w.i) + 1u + (uint32_t)jj;
+    const int L = __popcll(em);
+    const bool member = (lane >> lg) < L;
+    const CoopEntry &E = tab[min(lane >> lg, L - 1)];
+    const float ca = E.a, cb = E.b, ccc = E.cc, cd = E.d, ce = E.e, ctau = E.tau_tot;
+    const uint32_t ct_abs = E.t_abs;
+    const uint32_t k = cg.i0 + 1u + (uint32_t)jj;
     // step k's uniforms and local rotation (same draws as NativeWalker::step)
-    const u32x4 cur = philox_step(pre, k, P.key0, P.key1);
+    const u32x4 cur = philox_step(cg.pre, k, P.key0, P.key1);
     const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
     const float u0 = u01f(cur.x);
-    const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u0, d));
-    const float cost = (e == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, u0, cd));
+    const float cost = (ce == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
     const float hc = __builtin_amdgcn_sqrtf(fmaxf(fmaf(0.5f, cost, 0.5f), 0.f));   // cos(theta/2)
     const float hs = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-0.5f, cost, 0.5f), 0.f));  // sin(theta/2)
     const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
@@ -598,6 +657,7 @@ __device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *o
     Quat q{ha * hc, -hb * hs, ha * hs, hb * hc};
     if (jj == 0) {
         // frame of the current direction d: the rotation taking e_z to d
+        const float dx = cg.dx, dy = cg.dy, dz = cg.dz;
         const bool up = dz >= 0.f;
         Quat f = up ? Quat{1.f + dz, -dy, dx, 0.f} : Quat{-dy, 1.f - dz, 0.f, dx};
         const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
@@ -609,23 +669,36 @@ __device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *o
     const float uz = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
     float sz = dtau * uz, sp = dtau;
     group_scan_s(sz, sp, lg, jj, lane);
-    const float zk = z0 + sz, pk = tau0 + sp;
+    const float zk = cg.z0 + sz, pk = cg.tau0 + sp;
     const bool top = zk > 0.f && uz > 0.f;
-    const bool bot = zk < -tau_tot && uz < 0.f;
-    const bool ab = cur.w < t_abs;
+    const bool bot = zk < -ctau && uz < 0.f;
+    const bool ab = cur.w < ct_abs;
     const bool cap = k >= P.max_steps;
     const unsigned long long ev = __ballot(member && (top || bot || ab || cap));
-    // this lane's outcome, should its step end the segment
-    const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
-    const float corr = top ? zk : zk + tau_tot;
-    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
-    const float zout = bot ? -tau_tot : zk;
     const float ox = 2.f * fmaf(q.x, q.z, q.w * q.y);
     const float oy = 2.f * fmaf(q.y, q.z, -q.w * q.x);
-    // the photon's lane reads the first event lane of its group (or the last)
+    if (ev == 0ull) {
+        // no event: every group walked G steps; carry its last lane's state
+        if (lg == 6) {
+            cg.z0 = rdlane63(zk); cg.tau0 = rdlane63(pk);
+            cg.dx = rdlane63(ox); cg.dy = rdlane63(oy); cg.dz = rdlane63(uz);
+        } else {
+            const int src = lane | (G - 1);
+            cg.z0 = bperm(src, zk); cg.tau0 = bperm(src, pk);
+            cg.dx = bperm(src, ox); cg.dy = bperm(src, oy); cg.dz = bperm(src, uz);
+        }
+        cg.i0 += (uint32_t)G;
+        return;
+    }
+    // this lane's outcome, should its step end the segment
+    const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
+    const float corr = top ? zk : zk + ctau;
+    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
+    const float zout = bot ? -ctau : zk;
+    // each photon's lane reads the first event lane of its group (or the last)
     int src = lane;
     if (elig) {
-        const int base = rk << lg;
+        const int base = lane_rank(em) << lg;
         const unsigned long long gm = (ev >> base) & (lg == 6 ? ~0ull : ((1ull << G) - 1ull));
         src = base + (gm ? (__ffsll((long long)gm) - 1) : G - 1);
     }
@@ -634,8 +707,9 @@ __device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *o
     const uint32_t r_w = bperm(src, cur.w);
     const float r_z = bperm(src, zout), r_p = bperm(src, pout);
     const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uz);
+    cg.em = 0ull;   // photons' lanes hold the state again: regroup next round
     if (elig) {
-        w.ci += r_k - w.i;
+        atomicAdd(coop_steps, (unsigned long long)(r_k - w.i));
         w.i = r_k;
         w.z = r_z;
         w.tau_acc = r_p;
@@ -644,11 +718,7 @@ __device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *o
             w.cond = 1;
             state = DONE;
         } else if (r_type == 2) {
-            bool refl = false;
-            if (w.lamb) {
-                const u32x4 qq = philox4x32_10(u32x4{w.p, w.c, w.i, 1u}, P.key0, P.key1);
-                refl = (unsigned long long)qq.y < w.t_refl;
-            }
+            const bool refl = w.lamb && w.reflects(P);
             if (refl && w.i < P.max_steps) {
                 w.bounce = true;
                 w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
@@ -664,6 +734,34 @@ __device__ void coop_round(NativeWalker &w, int &state, const KParams &P, int *o
             state = DONE;
         }
     }
+}
+
+// The cooperative phase of a wave: runs until every photon of the wave is
+// finished.  Not inlined, so that its registers are allocated apart from the
+// one-lane walk loop; it is entered once per wave.  Everything passes by
+// value (an escaping address would send the caller's walker and kernel
+// parameters to scratch memory), and the arguments arrive in vector
+// registers, so the wave-uniform ones are restated with readfirstlane.
+struct CoopResult {
+    NativeWalker w;
+    int state;
+};
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __attribute__((noinline)) CoopResult coop_finish(NativeWalker w, int state, uint32_t key0, uint32_t key1,
+                                                            uint32_t max_steps, uint32_t cases_lo, uint32_t cases_hi,
+                                                            CoopEntry *tab, unsigned long long *coop_steps)
+{
+    KParams P{};   // the fields the walk reads
+    P.key0 = uni(key0);
+    P.key1 = uni(key1);
+    P.max_steps = uni(max_steps);
+    P.cases = reinterpret_cast<const DevCase *>(((uint64_t)uni(cases_hi) << 32) | uni(cases_lo));
+    CoopGroups cg;
+    cg.em = 0ull;
+    while (__ballot(state == LIVE)) coop_round(w, state, P, tab, cg, coop_steps);
+    return CoopResult{w, state};
 }
 
 // ------------------------------------------------------------ chunk queue
@@ -765,16 +863,13 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
         if (mine) {
             typename W::Real th, ph, pa;
             bool axis;
-            w.result(th, ph, pa, axis);
+            w.result(P, H.case_id, th, ph, pa, axis);
             if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
             const DevBand &B = P.bands[P.cases[H.case_id].band_begin + w.band];
             const double wvn = B.wvn;
             unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
             atomicAdd(&T.u[U_N], 1ull);
             atomicAdd(&T.u[U_STEPS], (unsigned long long)w.i);
-            if constexpr (W::COOP) {
-                if (w.ci) atomicAdd(&T.u[U_COOP], (unsigned long long)w.ci);
-            }
             if (w.cond >= 1) {
                 unsafeAtomicAdd(&T.f[T_QCOND + w.cond - 1], wvn);
                 atomicAdd(&T.u[w.cond], 1ull);
@@ -885,7 +980,7 @@ __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_
 k_walk(const KParams P)
 {
     __shared__ BlockState bs;
-    __shared__ int own_tab[W::COOP ? W::MAX_BLOCK / 64 : 1][32];
+    extern __shared__ CoopEntry coop_tab[];   // COOP_TAB entries per wave (native mode)
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     // ---- init: zero LDS, wave 0 takes the first chunk
@@ -923,7 +1018,12 @@ k_walk(const KParams P)
             // queue empty and few photons left: walk them cooperatively
             if (lds_load(&bs.exhausted) && __popcll(live) <= P.coop_live) {
                 __builtin_amdgcn_s_setprio(3);
-                while (__ballot(state == LIVE)) coop_round(w, state, P, own_tab[wave]);
+                const uint64_t cp = reinterpret_cast<uint64_t>(P.cases);
+                const CoopResult cr = coop_finish(w, state, P.key0, P.key1, P.max_steps, (uint32_t)cp,
+                                                  (uint32_t)(cp >> 32), coop_tab + wave * COOP_TAB,
+                                                  &bs.tal[my_slot].u[U_COOP]);
+                w = cr.w;
+                state = cr.state;
                 continue;
             }
         }
@@ -1244,10 +1344,12 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     // events bracket the walk kernel alone: kernel_ms is its device time
     if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
-    if (mode == MCS_NATIVE)
-        hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), 0, c->stream, P);
-    else
+    if (mode == MCS_NATIVE) {
+        const size_t tab = (size_t)(block / 64) * COOP_TAB * sizeof(CoopEntry);
+        hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), tab, c->stream, P);
+    } else {
         hipLaunchKernelGGL(k_walk<ReplayWalker>, dim3(grid), dim3(block), 0, c->stream, P);
+    }
     if (events && hipEventRecord(c->ev1, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (records && n > 0) {

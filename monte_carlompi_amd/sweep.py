@@ -7,9 +7,10 @@ photons of whatever case is next instead of waiting at case boundaries, and
 each chunk's FP64 BRF histogram lives in LDS until the chunk completes.  The
 launch (tally clears + walk + record scatter) is captured once in a hipGraph
 and replayed for repeated sweeps of the same shape.  ``autotune`` times a
-grid of launch shapes on a sampled subset of the cases and keeps the fastest;
-results do not depend on the shape because each photon's Philox stream is
-keyed by (seed, case, photon).
+grid of launch shapes on a sampled subset of the cases and keeps the fastest.
+Each photon's Philox stream is keyed by (seed, case, photon), so with the
+cooperative tail mode off (coop_live 0) results do not depend on the shape;
+with it on they agree statistically.
 """
 from __future__ import annotations
 
