@@ -39,7 +39,6 @@ constexpr int TU = MCS_TALLY_U64;
 constexpr double PI_D = 3.141592653589793;
 constexpr float TWO_PI_F = 6.283185307179586f;
 constexpr float LN2_F = 0.6931471805599453f;
-constexpr float HALF_PI_F = 1.5707963267948966f;
 
 // tally layout (doubles), see mcs_read_tallies
 constexpr int T_QDOWN = MCS_BRF_BINS;        // 648
@@ -180,8 +179,20 @@ struct NativeWalker {
             for (int k = 0; k < C.n_bands - 1; ++k)
                 if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
         }
-        const DevBand &B = P.bands[C.band_begin + bnd];
         band = bnd;
+        load_params(P, C, bnd);
+        z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
+        i = 0;
+        bounce = false;
+        cond = 0;
+        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
+        r = philox_step(pre, 1u, P.key0, P.key1);
+    }
+
+    // per-photon constants of (case, band)
+    __device__ __forceinline__ void load_params(const KParams &P, const DevCase &C, int bnd)
+    {
+        const DevBand &B = P.bands[C.band_begin + bnd];
         a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
         iso = B.iso != 0;
         surf = C.surf != 0;
@@ -196,12 +207,6 @@ struct NativeWalker {
         }
         tau_tot = C.tau_tot_f;
         lamb = C.lamb != 0;
-        z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
-        i = 0;
-        bounce = false;
-        cond = 0;
-        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
-        r = philox_step(pre, 1u, P.key0, P.key1);
     }
 
     // one step; returns true when the photon has left or been absorbed
@@ -552,12 +557,14 @@ __device__ __forceinline__ uint32_t bperm(int src_lane, uint32_t v)
     return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
 }
 
-// Per-wave state of the cooperative mode.  Groups are sticky: while no photon
-// of the wave exits, is absorbed or joins, each group keeps its photon's state
-// in its own lanes and carries it from its last lane to the next segment; the
-// photon's lane is updated (and groups rebuilt) only after a round with an
-// event.  The photon's constants sit in the wave's LDS table (one 64-byte
-// entry per group), read by the group's lanes each round.
+// Per-wave state of the cooperative mode.  On entry every live photon of the
+// wave writes its constants once into its own entry of the wave's LDS table
+// (index = its rank at entry), so the walker's constants and its pipelined
+// Philox block need no registers during the phase.  Groups are sticky: while
+// no photon of the wave exits, is absorbed or joins, each group keeps its
+// photon's state in its own lanes and carries it from its last lane to the
+// next segment; the photon's lane is updated (and groups rebuilt) only after a
+// round with an event.
 struct CoopEntry {
     uint32_t ck0, h0, l0, t_abs;     // Philox words, absorption threshold
     float a, b, cc, d;               // HG constants
@@ -565,12 +572,12 @@ struct CoopEntry {
     float dx, dy, dz;
     uint32_t i;
 };
-constexpr int COOP_TAB = 32;         // entries per wave (coop_live <= 32)
 
 struct CoopGroups {
     unsigned long long em;   // photon lanes of the current grouping (0: rebuild)
     int lg;                  // G = 2^lg lanes per photon
-    PhiloxPre pre;           // the group's photon's Philox words
+    int ent;                 // table entry of this lane's group's photon
+    PhiloxPre pre;           // that photon's Philox words
     float z0, tau0, dx, dy, dz;  // its state before the next segment
     uint32_t i0;
 };
@@ -581,9 +588,10 @@ __device__ __forceinline__ float rdlane63(float v)
 }
 
 // One cooperative round of a wave (all 64 lanes call; no divergence around
-// the lane permutes).  `tab` is the wave's LDS table of COOP_TAB entries.
-__device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab, CoopGroups &cg,
-                           unsigned long long *coop_steps)
+// the lane permutes).  `tab`: the wave's entries; `map`: group -> entry;
+// `me`: this lane's photon's entry.
+__device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab,
+                                           int *map, int me, CoopGroups &cg, unsigned long long *coop_steps)
 {
     const int lane = lane_id();
     // photons that cannot join take one ordinary step: the first step keeps
@@ -594,6 +602,8 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
         const bool solo = live && (w.i < 1u || w.bounce || w.surf);
         if (__ballot(solo)) {
             if (solo) {
+                w.load_params(P, P.cases[w.case_id(P)], w.band);
+                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
                 const bool d = w.step(P);
                 if (d || w.i >= P.max_steps) {
                     if (!d) w.cond = 0;
@@ -621,26 +631,23 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
         if (em == 0ull) return;
         const int L = __popcll(em);
         cg.lg = L <= 1 ? 6 : 6 - (32 - __clz(L - 1));
-        // group g walks the live photon of rank g: that lane fills entry g
+        // group g walks the live photon of rank g: it posts its entry and state
         if (elig) {
-            CoopEntry &E = tab[lane_rank(em)];
-            E.ck0 = w.pre.ck0; E.h0 = w.pre.h0; E.l0 = w.pre.l0; E.t_abs = w.t_abs_all;
-            E.a = w.a; E.b = w.b; E.cc = w.cc; E.d = w.d;
-            E.e = w.e; E.tau_tot = w.tau_tot; E.z = w.z; E.tau = w.tau_acc;
-            E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
+            CoopEntry &E = tab[me];
+            E.z = w.z; E.tau = w.tau_acc; E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
+            __atomic_store_n(&map[lane_rank(em)], me, __ATOMIC_RELAXED);
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int g = min(lane >> cg.lg, L - 1);
-        const CoopEntry &E = tab[g];
+        cg.ent = __atomic_load_n(&map[min(lane >> cg.lg, L - 1)], __ATOMIC_RELAXED);
+        const CoopEntry &E = tab[cg.ent];
         cg.pre = PhiloxPre{E.ck0, E.h0, E.l0};
         cg.z0 = E.z; cg.tau0 = E.tau; cg.dx = E.dx; cg.dy = E.dy; cg.dz = E.dz; cg.i0 = E.i;
     }
     const int lg = cg.lg;
     const int G = 1 << lg;
     const int jj = lane & (G - 1);
-    const int L = __popcll(em);
-    const bool member = (lane >> lg) < L;
-    const CoopEntry &E = tab[min(lane >> lg, L - 1)];
+    const bool member = (lane >> lg) < __popcll(em);
+    const CoopEntry &E = tab[cg.ent];
     const float ca = E.a, cb = E.b, ccc = E.cc, cd = E.d, ce = E.e, ctau = E.tau_tot;
     const uint32_t ct_abs = E.t_abs;
     const uint32_t k = cg.i0 + 1u + (uint32_t)jj;
@@ -718,16 +725,17 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
             w.cond = 1;
             state = DONE;
         } else if (r_type == 2) {
-            const bool refl = w.lamb && w.reflects(P);
+            const uint32_t cs = w.case_id(P);
+            const bool refl = P.cases[cs].lamb && w.reflects(P);
             if (refl && w.i < P.max_steps) {
-                w.bounce = true;
-                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
+                w.bounce = true;   // the next (solo) step re-emits from the bottom
             } else {
                 w.cond = refl ? 0 : 2;
                 state = DONE;
             }
         } else if (r_type == 3) {
-            w.cond = (r_w < w.t_abs5) ? 5 : 4;
+            const uint32_t cs = w.case_id(P);
+            w.cond = (r_w < P.bands[P.cases[cs].band_begin + w.band].t_abs5) ? 5 : 4;
             state = DONE;
         } else if (r_type == 4) {
             w.cond = 0;
@@ -736,32 +744,24 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     }
 }
 
-// The cooperative phase of a wave: runs until every photon of the wave is
-// finished.  Not inlined, so that its registers are allocated apart from the
-// one-lane walk loop; it is entered once per wave.  Everything passes by
-// value (an escaping address would send the caller's walker and kernel
-// parameters to scratch memory), and the arguments arrive in vector
-// registers, so the wave-uniform ones are restated with readfirstlane.
-struct CoopResult {
-    NativeWalker w;
-    int state;
-};
-
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-__device__ __attribute__((noinline)) CoopResult coop_finish(NativeWalker w, int state, uint32_t key0, uint32_t key1,
-                                                            uint32_t max_steps, uint32_t cases_lo, uint32_t cases_hi,
-                                                            CoopEntry *tab, unsigned long long *coop_steps)
+// The cooperative phase of a wave: every photon of the wave is finished when
+// it returns.  `tab`: coop_live entries; `map`: coop_live group slots.
+__device__ __forceinline__ void coop_phase(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab, int *map,
+                                           unsigned long long *coop_steps)
 {
-    KParams P{};   // the fields the walk reads
-    P.key0 = uni(key0);
-    P.key1 = uni(key1);
-    P.max_steps = uni(max_steps);
-    P.cases = reinterpret_cast<const DevCase *>(((uint64_t)uni(cases_hi) << 32) | uni(cases_lo));
+    const unsigned long long live0 = __ballot(state == LIVE);
+    const int me = lane_rank(live0);
+    if (state == LIVE) {
+        CoopEntry &E = tab[me];
+        E.ck0 = w.pre.ck0; E.h0 = w.pre.h0; E.l0 = w.pre.l0; E.t_abs = w.t_abs_all;
+        E.a = w.a; E.b = w.b; E.cc = w.cc; E.d = w.d;
+        E.e = w.e; E.tau_tot = w.tau_tot;
+    }
     CoopGroups cg;
     cg.em = 0ull;
-    while (__ballot(state == LIVE)) coop_round(w, state, P, tab, cg, coop_steps);
-    return CoopResult{w, state};
+    cg.lg = 0;
+    cg.ent = 0;
+    while (__ballot(state == LIVE)) coop_round(w, state, P, tab, map, me, cg, coop_steps);
 }
 
 // ------------------------------------------------------------ chunk queue
@@ -980,7 +980,8 @@ __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_
 k_walk(const KParams P)
 {
     __shared__ BlockState bs;
-    extern __shared__ CoopEntry coop_tab[];   // COOP_TAB entries per wave (native mode)
+    // native mode: coop_live table entries per wave, then coop_live group slots per wave
+    extern __shared__ CoopEntry coop_tab[];
     const int lane = lane_id();
     const int wave = threadIdx.x >> 6;
     // ---- init: zero LDS, wave 0 takes the first chunk
@@ -996,6 +997,7 @@ k_walk(const KParams P)
     }
     __syncthreads();
 
+    int *const coop_map = reinterpret_cast<int *>(coop_tab + (blockDim.x >> 6) * P.coop_live);
     W w;
     int state = EMPTY;
     int my_slot = 0;
@@ -1018,13 +1020,11 @@ k_walk(const KParams P)
             // queue empty and few photons left: walk them cooperatively
             if (lds_load(&bs.exhausted) && __popcll(live) <= P.coop_live) {
                 __builtin_amdgcn_s_setprio(3);
-                const uint64_t cp = reinterpret_cast<uint64_t>(P.cases);
-                const CoopResult cr = coop_finish(w, state, P.key0, P.key1, P.max_steps, (uint32_t)cp,
-                                                  (uint32_t)(cp >> 32), coop_tab + wave * COOP_TAB,
-                                                  &bs.tal[my_slot].u[U_COOP]);
-                w = cr.w;
-                state = cr.state;
-                continue;
+                coop_phase(w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
+                           &bs.tal[my_slot].u[U_COOP]);
+                // the queue is empty and every photon of the wave is finished
+                retire(bs, P, w, state, my_slot, my_gid);
+                break;
             }
         }
         // leave the hot loop once `thresh` lanes are not live (with the queue
@@ -1345,7 +1345,7 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (mode == MCS_NATIVE) {
-        const size_t tab = (size_t)(block / 64) * COOP_TAB * sizeof(CoopEntry);
+        const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), tab, c->stream, P);
     } else {
         hipLaunchKernelGGL(k_walk<ReplayWalker>, dim3(grid), dim3(block), 0, c->stream, P);
