@@ -156,17 +156,18 @@ struct NativeWalker {
     float z, mux, muy, muz, tau_acc;
     uint32_t i;
     int band;
-    bool bounce;
     u32x4 r;                     // uniforms of the next step
     // per-photon parameters (the photon and case ids live in `pre`; the
     // bottom reflectance and 1 / ext_cff are read from the tables when needed)
     float a, b, cc, d, e, tau_tot;
     uint32_t t_abs5, t_abs_all;  // absorbed as condition 5 iff bits < t_abs5, else 4 iff < t_abs_all
-    bool iso, lamb, surf;
+    bool iso, lamb;
     int cond;
     PhiloxPre pre;
 
-    __device__ void start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    // Set up photon `photon` of case `case_id` and take its first step (or
+    // its whole walk in the Lambertian surface mode); true when it is done.
+    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
     {
         const DevCase &C = P.cases[case_id];
         int bnd = 0;
@@ -183,10 +184,29 @@ struct NativeWalker {
         load_params(P, C, bnd);
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0;
-        bounce = false;
         cond = 0;
         pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
         r = philox_step(pre, 1u, P.key0, P.key1);
+        if (C.surf) return surface_walk(P, C);
+        // first step: along the incident direction (mc3D:1344-1347)
+        ++i;
+        const u32x4 cur = r;
+        r = philox_step(pre, 2u, P.key0, P.key1);
+        const float dtau = free_path(cur);
+        z = dtau * muz;
+        tau_acc = dtau;
+        if (z < -tau_tot) {                          // unscattered through the layer
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
+            z = -tau_tot;
+            if (lamb && reflects(P)) return reemit(P);
+            cond = 3;
+            return true;
+        }
+        if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            return true;
+        }
+        return false;
     }
 
     // per-photon constants of (case, band)
@@ -195,8 +215,7 @@ struct NativeWalker {
         const DevBand &B = P.bands[C.band_begin + bnd];
         a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
         iso = B.iso != 0;
-        surf = C.surf != 0;
-        if (surf) {
+        if (C.surf) {
             // Lambertian surface: absorbed with probability 1 - R, split by species
             const double q = 1.0 - C.r_lamb;
             t_abs5 = prob_bits_dev(B.p_ext_imp * q);
@@ -209,44 +228,118 @@ struct NativeWalker {
         lamb = C.lamb != 0;
     }
 
-    // one step; returns true when the photon has left or been absorbed
+    __device__ __forceinline__ static float free_path(const u32x4 &q)
+    {
+        return -LN2_F * __builtin_amdgcn_logf(u01f(q.z));
+    }
+
+    // Cosine-weighted upward direction, cos = sqrt(u): the pdf 2 sin cos of
+    // the rejection loop in mc3D:1242-1250 (Lambertian re-emission).
+    __device__ __forceinline__ void lambertian_direction(const u32x4 &q)
+    {
+        const float cost = __builtin_amdgcn_sqrtf(u01f(q.x));
+        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        const float rev = u01f(q.y);
+        mux = sint * __builtin_amdgcn_cosf(rev);
+        muy = sint * __builtin_amdgcn_sinf(rev);
+        muz = cost;
+    }
+
+    // After a Lambertian bottom reflection at step i (z = -tau_tot): the next
+    // step re-emits cosine-weighted upward and moves (mc3D:1420-1459); true
+    // when the photon is done.
+    __device__ bool reemit(const KParams &P)
+    {
+        if (i >= P.max_steps) {
+            cond = 0;
+            return true;
+        }
+        ++i;
+        const u32x4 cur = r;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
+        lambertian_direction(cur);
+        const float dtau = free_path(cur);
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        if (z > 0.f) {
+            tau_acc -= fminf(z / muz, dtau);
+            cond = 1;
+            return true;
+        }
+        if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            return true;
+        }
+        return false;
+    }
+
+    // Lambertian surface (mc3D:1228-1238, 1385-1387): first event at z = 0,
+    // absorbed with probability 1 - R, else re-emitted cosine-weighted upward
+    // until the photon leaves.
+    __device__ bool surface_walk(const KParams &P, const DevCase &C)
+    {
+        (void)C;
+        ++i;
+        u32x4 cur = r;
+        if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            return true;
+        }
+        for (;;) {
+            if (i >= P.max_steps) {
+                cond = 0;
+                return true;
+            }
+            ++i;
+            cur = philox_step(pre, i, P.key0, P.key1);
+            lambertian_direction(cur);
+            const float dtau = free_path(cur);
+            z = fmaf(dtau, muz, z);
+            tau_acc += dtau;
+            if (z > 0.f) {
+                tau_acc -= fminf(z / muz, dtau);
+                cond = 1;
+                return true;
+            }
+            if (cur.w < t_abs_all) {
+                cond = (cur.w < t_abs5) ? 5 : 4;
+                return true;
+            }
+        }
+    }
+
+    // one step (i >= 1: after the first); true when the photon has left or
+    // been absorbed
     __device__ __forceinline__ bool step(const KParams &P)
     {
         ++i;
         const u32x4 cur = r;
         r = philox_step(pre, i + 1u, P.key0, P.key1);
-        const bool first = (i == 1u);
-        const bool lam = bounce || surf;
-        // free path; a Lambertian surface takes its first event at z = 0
-        float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
-        dtau = (surf && first) ? 0.f : dtau;
-        // polar cosine: HG inverse CDF (mc3D:790-800), isotropic for g = 0, or
-        // the cosine-weighted hemisphere cos = sqrt(u) whose pdf 2 sin cos is
-        // that of the rejection loop in mc3D:1242-1250
+        const float dtau = free_path(cur);
+        // polar cosine: HG inverse CDF (mc3D:790-800), isotropic for g = 0
         const float u0 = u01f(cur.x);
         const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u0, d));
-        float cost = iso ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-        cost = lam ? __builtin_amdgcn_sqrtf(u0) : cost;
+        const float cost = iso ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
         const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
         // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
         const float rev = u01f(cur.y);
         const float sinp = __builtin_amdgcn_sinf(rev);
         const float cosp = __builtin_amdgcn_cosf(rev);
-        // rotation (mc3D:1255-1281); near-vertical incoming directions and
-        // Lambertian re-emission (incoming (0, 0, 1)) take the axis form
+        // rotation (mc3D:1255-1281); near-vertical incoming directions take the
+        // axis form (rare: a branch, not a select)
         const float s2 = fmaf(-muz, muz, 1.f);
-        const bool vert = lam || (s2 < 1e-7f);
-        const float sg = lam ? 1.f : (muz > 0.f ? 1.f : -1.f);
-        const float st = sint * __builtin_amdgcn_rsqf(fmaxf(s2, 1e-30f));
-        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-        const float gz = fmaf(-s2 * st, cosp, muz * cost);
-        const float vx = sint * cosp, vy = sg * sint * sinp, vz = sg * cost;
-        // the first step keeps the incident direction (mc3D:1344-1347)
-        mux = first ? mux : (vert ? vx : gx);
-        muy = first ? muy : (vert ? vy : gy);
-        muz = first ? muz : (vert ? vz : gz);
-        bounce = false;
+        if (s2 < 1e-7f) {
+            const float sg = muz > 0.f ? 1.f : -1.f;
+            mux = sint * cosp;
+            muy = sg * sint * sinp;
+            muz = sg * cost;
+        } else {
+            const float st = sint * __builtin_amdgcn_rsqf(s2);
+            const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+            const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+            const float gz = fmaf(-s2 * st, cosp, muz * cost);
+            mux = gx; muy = gy; muz = gz;
+        }
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
         const bool top = z > 0.f;
@@ -261,11 +354,8 @@ struct NativeWalker {
         if (bot) {                                   // mc3D:1399-1459
             tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
-            if (lamb && reflects(P)) {
-                bounce = true;
-                return false;
-            }
-            cond = first ? 3 : 2;
+            if (lamb && reflects(P)) return reemit(P);
+            cond = 2;
             return true;
         }
         // absorbed: one uniform decides species and absorption jointly, with
@@ -331,7 +421,8 @@ struct ReplayWalker {
         return P.rdev[pos++];
     }
 
-    __device__ void start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    // set up the photon; its steps all run in step() (returns false)
+    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
     {
         (void)photon;
         const DevCase &C = P.cases[case_id];
@@ -353,6 +444,7 @@ struct ReplayWalker {
         i = 0;
         bounce = false;
         cond = 0;
+        return false;
     }
 
     __device__ bool step(const KParams &P)
@@ -594,25 +686,7 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
                                            int *map, int me, CoopGroups &cg, unsigned long long *coop_steps)
 {
     const int lane = lane_id();
-    // photons that cannot join take one ordinary step: the first step keeps
-    // the incident direction, a Lambertian re-emission and the surface mode
-    // have their own direction law
-    {
-        const bool live = state == LIVE;
-        const bool solo = live && (w.i < 1u || w.bounce || w.surf);
-        if (__ballot(solo)) {
-            if (solo) {
-                w.load_params(P, P.cases[w.case_id(P)], w.band);
-                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
-                const bool d = w.step(P);
-                if (d || w.i >= P.max_steps) {
-                    if (!d) w.cond = 0;
-                    state = DONE;
-                }
-            }
-        }
-    }
-    const bool elig = state == LIVE && w.i >= 1u && !w.bounce && !w.surf;
+    const bool elig = state == LIVE;
     const unsigned long long em = __ballot(elig);
     if (em != cg.em) {
         if (cg.em) {
@@ -725,12 +799,12 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
             w.cond = 1;
             state = DONE;
         } else if (r_type == 2) {
-            const uint32_t cs = w.case_id(P);
-            const bool refl = P.cases[cs].lamb && w.reflects(P);
-            if (refl && w.i < P.max_steps) {
-                w.bounce = true;   // the next (solo) step re-emits from the bottom
+            if (P.cases[w.case_id(P)].lamb && w.reflects(P)) {
+                // re-emitted from the bottom: the next step, taken here
+                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
+                if (w.reemit(P)) state = DONE;
             } else {
-                w.cond = refl ? 0 : 2;
+                w.cond = 2;
                 state = DONE;
             }
         } else if (r_type == 3) {
@@ -934,8 +1008,7 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
                 const uint32_t photon = H.p0 + off + rk;
                 my_gid = H.gid0 + off + rk;
                 my_slot = s;
-                w.start(P, H.case_id, photon, my_gid);
-                state = LIVE;
+                state = w.start(P, H.case_id, photon, my_gid) ? DONE : LIVE;
             }
         }
         if (got == n) break;
@@ -1011,6 +1084,8 @@ k_walk(const KParams P)
         if (!exhausted && __ballot(state == EMPTY)) refill(bs, P, w, state, my_slot, my_gid);
         const unsigned long long live = __ballot(state == LIVE);
         if (live == 0ull) {
+            // photons can finish at hand-out (first step, surface mode): retire first
+            if (__ballot(state == DONE)) continue;
             if (lds_load(&bs.exhausted)) break;
             __builtin_amdgcn_s_sleep(2);
             if (++spins > (1 << 28)) break;  // cannot happen: chunks always finish
