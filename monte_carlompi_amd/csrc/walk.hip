@@ -670,9 +670,27 @@ struct CoopGroups {
     int lg;                  // G = 2^lg lanes per photon
     int ent;                 // table entry of this lane's group's photon
     PhiloxPre pre;           // that photon's Philox words
-    float z0, tau0, dx, dy, dz;  // its state before the next segment
-    uint32_t i0;
+    float z0, tau0;          // its state before the next segment: depth, path,
+    Quat f;                  // direction frame (third column = direction),
+    uint32_t i0;             // steps taken
 };
+
+// direction = third column of the rotation of unit quaternion q
+__device__ __forceinline__ void frame_dir(const Quat &q, float &x, float &y, float &z)
+{
+    x = 2.f * fmaf(q.x, q.z, q.w * q.y);
+    y = 2.f * fmaf(q.y, q.z, -q.w * q.x);
+    z = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
+}
+
+// a frame whose third column is the unit vector d: the rotation taking e_z to d
+__device__ __forceinline__ Quat frame_of(float dx, float dy, float dz)
+{
+    const bool up = dz >= 0.f;
+    const Quat f = up ? Quat{1.f + dz, -dy, dx, 0.f} : Quat{-dy, 1.f - dz, 0.f, dx};
+    const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
+    return Quat{f.w * n, f.x * n, f.y * n, f.z * n};
+}
 
 __device__ __forceinline__ float rdlane63(float v)
 {
@@ -694,7 +712,9 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
             const bool was = (cg.em >> lane) & 1ull;
             const int src = was ? (lane_rank(cg.em) << cg.lg) : lane;
             const float z = bperm(src, cg.z0), t = bperm(src, cg.tau0);
-            const float x = bperm(src, cg.dx), y = bperm(src, cg.dy), u = bperm(src, cg.dz);
+            float x, y, u;
+            frame_dir(cg.f, x, y, u);
+            x = bperm(src, x); y = bperm(src, y); u = bperm(src, u);
             const uint32_t i = bperm(src, cg.i0);
             if (was) {
                 atomicAdd(coop_steps, (unsigned long long)(i - w.i));
@@ -715,7 +735,8 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
         cg.ent = __atomic_load_n(&map[min(lane >> cg.lg, L - 1)], __ATOMIC_RELAXED);
         const CoopEntry &E = tab[cg.ent];
         cg.pre = PhiloxPre{E.ck0, E.h0, E.l0};
-        cg.z0 = E.z; cg.tau0 = E.tau; cg.dx = E.dx; cg.dy = E.dy; cg.dz = E.dz; cg.i0 = E.i;
+        cg.z0 = E.z; cg.tau0 = E.tau; cg.i0 = E.i;
+        cg.f = frame_of(E.dx, E.dy, E.dz);
     }
     const int lg = cg.lg;
     const int G = 1 << lg;
@@ -736,15 +757,7 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
     const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
     Quat q{ha * hc, -hb * hs, ha * hs, hb * hc};
-    if (jj == 0) {
-        // frame of the current direction d: the rotation taking e_z to d
-        const float dx = cg.dx, dy = cg.dy, dz = cg.dz;
-        const bool up = dz >= 0.f;
-        Quat f = up ? Quat{1.f + dz, -dy, dx, 0.f} : Quat{-dy, 1.f - dz, 0.f, dx};
-        const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
-        f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
-        q = qmul(f, q);
-    }
+    if (jj == 0) q = qmul(cg.f, q);   // the group's carried frame on the left
     // frames first: the depth change of step k needs its direction
     group_scan_q(q, lg, jj, lane);
     const float uz = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
@@ -756,21 +769,25 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     const bool ab = cur.w < ct_abs;
     const bool cap = k >= P.max_steps;
     const unsigned long long ev = __ballot(member && (top || bot || ab || cap));
-    const float ox = 2.f * fmaf(q.x, q.z, q.w * q.y);
-    const float oy = 2.f * fmaf(q.y, q.z, -q.w * q.x);
     if (ev == 0ull) {
         // no event: every group walked G steps; carry its last lane's state
+        // (the frame renormalised against drift)
+        Quat f;
         if (lg == 6) {
             cg.z0 = rdlane63(zk); cg.tau0 = rdlane63(pk);
-            cg.dx = rdlane63(ox); cg.dy = rdlane63(oy); cg.dz = rdlane63(uz);
+            f = Quat{rdlane63(q.w), rdlane63(q.x), rdlane63(q.y), rdlane63(q.z)};
         } else {
             const int src = lane | (G - 1);
             cg.z0 = bperm(src, zk); cg.tau0 = bperm(src, pk);
-            cg.dx = bperm(src, ox); cg.dy = bperm(src, oy); cg.dz = bperm(src, uz);
+            f = Quat{bperm(src, q.w), bperm(src, q.x), bperm(src, q.y), bperm(src, q.z)};
         }
+        const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
+        cg.f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
         cg.i0 += (uint32_t)G;
         return;
     }
+    const float ox = 2.f * fmaf(q.x, q.z, q.w * q.y);
+    const float oy = 2.f * fmaf(q.y, q.z, -q.w * q.x);
     // this lane's outcome, should its step end the segment
     const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
     const float corr = top ? zk : zk + ctau;
