@@ -25,8 +25,8 @@ METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon 
 # 1e6 photons over every block; the cooperative tail mode finishes the walks
 # still running when the queue is empty
 SNOW_1M_CANDIDATES = [(256, 5, 512, 16, 16), (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 16),
-                      (256, 4, 1024, 16, 16), (256, 5, 512, 8, 16), (256, 5, 1024, 16, 16),
-                      (1024, 1, 4096, 16, 16), (256, 5, 512, 16, 24)]
+                      (256, 5, 512, 8, 16), (256, 5, 512, 16, 20), (1024, 1, 2048, 16, 24),
+                      (1024, 1, 2048, 16, 32), (512, 2, 1024, 16, 24)]
 N_PHOTONS = 1_000_000
 
 
@@ -92,11 +92,8 @@ def main():
     avg_steps = sum(steps) / len(steps)
     rf = roofline.walk_roofline(avg_steps, avg_kern)
 
-    traffic = None
-    pmc = os.path.join(ROOT, 'profiles', 'k_walk_native_snow1m_traffic.json')
-    if os.path.exists(pmc):
-        with open(pmc) as fh:
-            traffic = json.load(fh).get('hbm_bytes_per_launch')
+    traffic = profiled('k_walk_native_snow1m_traffic.json', 'hbm_bytes_per_launch')
+    issue = profiled('k_walk_native_snow1m_valu.json', 'issue_busy')
 
     out = {
         'metric': METRIC, 'value': round(photons_per_s, 1), 'unit': 'photons/s',
@@ -119,9 +116,11 @@ def main():
             'bound': 'valu', 'achieved': round(rf['fp32_tflops'], 4),
             'peak': roofline.PEAK_FP32_TFLOPS, 'unit': 'TFLOP/s',
             'frac': round(rf['fp32_frac'], 5), 'traffic': traffic,
-            'valu_lane_ops_frac': round(rf['valu_frac'], 4),
-            'basis': '48 FP32 flops and 170 VALU lane-ops per photon-step x photon-steps '
-                     'per launch / walk-kernel HIP-event time',
+            'valu_issue_busy': issue,
+            'basis': '48 FP32 flops per photon-step x photon-steps per launch / walk-kernel '
+                     'HIP-event time; traffic (HBM bytes per launch) and valu_issue_busy (share '
+                     'of SIMD quad-cycles issuing VALU work) from rocprofv3 runs of the same '
+                     'workload, profiles/',
         },
     }
     if not args.no_sweep:
@@ -141,6 +140,15 @@ def main():
         }
         out['gpu_over_cpu'] = round(photons_per_s / cb['photons_per_s'], 1)
     print(json.dumps(out), flush=True)
+
+
+def profiled(name, key):
+    """A figure of a committed rocprofv3 summary (profiles/NAME), or None."""
+    path = os.path.join(ROOT, 'profiles', name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get(key)
 
 
 def sweep_block(w):
@@ -166,7 +174,8 @@ def sweep_block(w):
             'photons_per_s': round(st['photons'] / (ms * 1e-3), 1),
             'photon_steps_per_s': round(st['steps'] / (ms * 1e-3), 1),
             'launch': list(res['launch']), 'autotune_ms': {str(k): round(v, 3) for k, v in res['tune_times'].items()},
-            'valu_lane_ops_frac': round(rf['valu_frac'], 4), 'fp32_frac': round(rf['fp32_frac'], 5)}
+            'fp32_frac': round(rf['fp32_frac'], 5),
+            'valu_issue_busy': profiled('k_walk_native_sweep16384_valu.json', 'issue_busy')}
 
 
 if __name__ == '__main__':

@@ -6,9 +6,10 @@ summaries of rocprofv3 CSV output (kernel stats, FETCH_SIZE / WRITE_SIZE).
         FETCH_SIZE / WRITE_SIZE report for a known number of bytes)
     python -m monte_carlompi_amd.profiling sweep [--repeats 2]
         the batched sweep (16,384 cases) alone, to profile it
-    python -m monte_carlompi_amd.profiling valu COUNTERS_CSV [--kernel k_walk]
+    python -m monte_carlompi_amd.profiling valu COUNTERS_CSV [--kernel k_walk] [--json OUT]
         VALU issue occupancy, dual issue and lane utilisation from SQ_INSTS_VALU,
-        SQ_ACTIVE_INST_VALU(2), SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE
+        SQ_ACTIVE_INST_VALU(2), SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE (medians
+        over dispatches to OUT)
     python -m monte_carlompi_amd.profiling stats KERNEL_STATS_CSV
     python -m monte_carlompi_amd.profiling traffic --kernel NAME --fetch F.csv
         --write W.csv --cal-fetch CF.csv --cal-write CW.csv [--mib 256] [--json OUT]
@@ -19,6 +20,8 @@ import argparse
 import collections
 import csv
 import json
+
+import numpy as np
 
 
 def copy_kernel(mib: int = 256):
@@ -123,6 +126,7 @@ def main():
     c = sub.add_parser('copy'); c.add_argument('--mib', type=int, default=256)
     sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
     vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
+    vm.add_argument('--json', help='write the per-dispatch medians here')
     s = sub.add_parser('stats'); s.add_argument('csv')
     t = sub.add_parser('traffic')
     for k in ('kernel', 'fetch', 'write', 'cal_fetch', 'cal_write'):
@@ -135,8 +139,17 @@ def main():
     elif a.cmd == 'sweep':
         run_sweep(a.repeats)
     elif a.cmd == 'valu':
-        for r in valu_metrics(a.csv, a.kernel):
+        rows = valu_metrics(a.csv, a.kernel)
+        for r in rows:
             print(json.dumps(r))
+        if a.json:
+            keys = ('issue_busy', 'dual_issue', 'VALUUtilization', 'kernel_s', 'SQ_INSTS_VALU',
+                    'SQ_INSTS_VALU_TRANS_F32')
+            med = {k: float(np.median([r[k] for r in rows if k in r])) for k in keys
+                   if any(k in r for r in rows)}
+            med.update(kernel=a.kernel, dispatches=len(rows))
+            with open(a.json, 'w') as fh:
+                json.dump(med, fh, indent=1)
     elif a.cmd == 'stats':
         for r in kernel_stats(a.csv):
             print('%-60s calls %4d  avg %12.1f ns  min %12.1f  max %12.1f  %6.2f%%' % (
