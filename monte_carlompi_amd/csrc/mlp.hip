@@ -6,7 +6,8 @@
 // stored transposed, so forward (X W^T), input-gradient (dZ W) and
 // weight-gradient (dZ^T A) products are the same kernel.  Epilogues are
 // fused: bias + SiLU (forward, keeping the FP32 pre-activation for the
-// backward pass), SiLU' (backward), or a plain FP32 store.
+// backward pass), SiLU' (backward), a plain FP32 store, or an FP32 atomic add
+// of a split-K partial (weight gradients: few output tiles, K = batch rows).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,7 +29,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int BM = 64, BN = 64, BK = 32, LDSP = BK + 8;  // LDS row pitch (bf16)
 constexpr int K0 = 32;                                    // padded input width
 
-enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3 };
+enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3, EPI_ADD_F32 = 4 };
 
 __device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
 
@@ -41,6 +42,7 @@ struct GemmArgs {
     bf16 *Cb;           // EPI_SILU / EPI_SILU_NOZ: silu(z) [M][ldcb]; EPI_DSILU: dZ [M][ldcb]
     const float *Zin;   // EPI_DSILU: pre-activation [M][ldc]
     int ldc, ldcb;
+    int kchunk;         // K per blockIdx.z slice (split-K; EPI_ADD_F32 adds the partial tiles)
 };
 
 template <bool A_T, bool B_T, int EPI>
@@ -54,8 +56,10 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmArgs g)
     const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
     f32x16 acc;
     for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+    const int kb = blockIdx.z * g.kchunk;
+    const int ke = min(g.K, kb + g.kchunk);
 
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
+    for (int k0 = kb; k0 < ke; k0 += BK) {
         // ---- A tile -> As[m][k]
         if (!A_T) {
             const int row = t >> 2, kc = (t & 3) * 8;
@@ -121,6 +125,8 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmArgs g)
         const float v = acc[r] + bias;
         if (EPI == EPI_F32) {
             g.C[(size_t)m * g.ldc + n] = v;
+        } else if (EPI == EPI_ADD_F32) {
+            unsafeAtomicAdd(&g.C[(size_t)m * g.ldc + n], acc[r]);   // partial sum of one K slice
         } else if (EPI == EPI_SILU || EPI == EPI_SILU_NOZ) {
             if (EPI == EPI_SILU) g.C[(size_t)m * g.ldc + n] = v;
             g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * sigm(v));
@@ -177,14 +183,15 @@ __global__ void k_loss(const float *Y, const float *T, const float *w, int rows,
     if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
 }
 
-// db[c] = sum over rows of dZ[r][c]
+// db[c] += sum over rows [blockIdx.y * 256, +256) of dZ[r][c] (db zeroed first)
 __global__ void k_colsum(const bf16 *dZ, int rows, int cols, int ld, float *db)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= cols) return;
+    const int r0 = blockIdx.y * 256, r1 = min(rows, r0 + 256);
     float s = 0.f;
-    for (int r = 0; r < rows; ++r) s += (float)dZ[(size_t)r * ld + c];
-    db[c] = s;
+    for (int r = r0; r < r1; ++r) s += (float)dZ[(size_t)r * ld + c];
+    unsafeAtomicAdd(&db[c], s);
 }
 
 __global__ void k_fill_f32(float *p, size_t n, float v)
@@ -282,9 +289,19 @@ static hipError_t dalloc(T **p, size_t count)
 }
 
 template <bool AT, bool BT, int EPI>
-static void gemm(mcs_mlp *c, const GemmArgs &g)
+static void gemm(mcs_mlp *c, const GemmArgs &g0)
 {
-    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM);
+    GemmArgs g = g0;
+    const int tiles = ((g.N + BN - 1) / BN) * ((g.M + BM - 1) / BM);
+    int splits = 1;
+    if (EPI == EPI_ADD_F32) {
+        // long-K products with few output tiles (weight gradients): split K so
+        // that ~512 blocks run, slices a multiple of BK
+        splits = std::max(1, std::min(g.K / BK, (512 + tiles - 1) / tiles));
+    }
+    g.kchunk = ((g.K + splits - 1) / splits + BK - 1) / BK * BK;
+    splits = (g.K + g.kchunk - 1) / g.kchunk;
+    dim3 grid((g.N + BN - 1) / BN, (g.M + BM - 1) / BM, splits);
     hipLaunchKernelGGL((k_gemm<AT, BT, EPI>), grid, dim3(256), 0, c->stream, g);
 }
 
@@ -293,6 +310,7 @@ static GemmArgs ga(const bf16 *A, int lda, const bf16 *B, int ldb, int M, int N,
     GemmArgs g;
     memset(&g, 0, sizeof g);
     g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.M = M; g.N = N; g.K = K;
+    g.kchunk = K;
     return g;
 }
 
@@ -325,6 +343,9 @@ static void backward_pass(mcs_mlp *c, int rows)
 {
     const int L = c->d.n_hidden_layers;
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, c->loss, (size_t)1, 0.f);
+    // weight and bias gradients are sums of split-K / row-slice partials
+    hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, c->stream, c->gr, c->n_pad, 0.f);
+    const dim3 colsum_rows(1, (rows + 255) / 256);
     hipLaunchKernelGGL(k_loss, dim3(256), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY, c->loss);
     // output layer
     const Layer &lo = c->layers[L];
@@ -332,9 +353,9 @@ static void backward_pass(mcs_mlp *c, int rows)
     {
         GemmArgs g = ga(c->dY, c->np, Aprev, lo.pcol, lo.prow, lo.pcol, rows);  // dW = dY^T A
         g.C = c->gr + lo.w_off; g.ldc = lo.pcol;
-        gemm<true, true, EPI_F32>(c, g);
-        hipLaunchKernelGGL(k_colsum, dim3((lo.prow + 255) / 256), dim3(256), 0, c->stream, c->dY, rows, lo.prow,
-                           c->np, c->gr + lo.b_off);
+        gemm<true, true, EPI_ADD_F32>(c, g);
+        hipLaunchKernelGGL(k_colsum, dim3((lo.prow + 255) / 256, colsum_rows.y), dim3(256), 0, c->stream, c->dY,
+                           rows, lo.prow, c->np, c->gr + lo.b_off);
         GemmArgs h = ga(c->dY, c->np, c->wb + lo.wb_off, lo.pcol, rows, lo.pcol, lo.prow);  // dA = dY W
         h.Zin = c->Z[L - 1]; h.ldc = lo.pcol;
         h.Cb = c->dZa; h.ldcb = lo.pcol;
@@ -347,9 +368,9 @@ static void backward_pass(mcs_mlp *c, int rows)
         const int ldi = (l == 0) ? K0 : c->layers[l - 1].prow;
         GemmArgs g = ga(dz, ly.prow, inp, ldi, ly.prow, ly.pcol, rows);  // dW_l = dZ^T in
         g.C = c->gr + ly.w_off; g.ldc = ly.pcol;
-        gemm<true, true, EPI_F32>(c, g);
-        hipLaunchKernelGGL(k_colsum, dim3((ly.prow + 255) / 256), dim3(256), 0, c->stream, dz, rows, ly.prow,
-                           ly.prow, c->gr + ly.b_off);
+        gemm<true, true, EPI_ADD_F32>(c, g);
+        hipLaunchKernelGGL(k_colsum, dim3((ly.prow + 255) / 256, colsum_rows.y), dim3(256), 0, c->stream, dz,
+                           rows, ly.prow, ly.prow, c->gr + ly.b_off);
         if (l > 0) {
             GemmArgs h = ga(dz, ly.prow, c->wb + ly.wb_off, ly.pcol, rows, ly.pcol, ly.prow);
             h.Zin = c->Z[l - 1]; h.ldc = ly.pcol;
