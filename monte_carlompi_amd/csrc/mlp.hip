@@ -138,6 +138,90 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmArgs g)
     }
 }
 
+
+// Forward GEMM for large M (inference): C = A B^T with A [M][lda] and
+// B [N][ldb] row-major BF16 (no transposes), 128 x 128 output tiles per
+// 256-thread block, each wave a 64 x 64 quadrant = 2 x 2 MFMA 32x32x16
+// accumulators; K staged through two LDS buffers 32 at a time, the next stage
+// prefetched into registers while the current one feeds the MFMAs.
+// K must be a multiple of 32 (padded layer widths are).  Epilogues: EPI_F32
+// (bias, FP32 store, columns n < N) or EPI_SILU_NOZ (bias + SiLU, BF16 store).
+constexpr int FM = 128, FN = 128, FP = BK + 8;   // tile and LDS row pitch (bf16)
+
+template <int EPI>
+__global__ void __launch_bounds__(256) k_gemm_fwd(const GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16 As[2][FM][FP];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[2][FN][FP];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;        // quadrant of the 128 x 128 tile
+    const int m0 = blockIdx.y * FM, n0 = blockIdx.x * FN;
+    // each thread moves 2 x 16 B of A and 2 x 16 B of B per stage:
+    // rows r = t / 4 and t / 4 + 64, k chunk (t % 4) * 8
+    const int lr = t >> 2, lk = (t & 3) * 8;
+    auto load = [&](int k0, uint4 (&ra)[2], uint4 (&rb)[2]) {
+        for (int h = 0; h < 2; ++h) {
+            const int m = m0 + lr + 64 * h, n = n0 + lr + 64 * h;
+            ra[h] = (m < g.M) ? *reinterpret_cast<const uint4 *>(g.A + (size_t)m * g.lda + k0 + lk)
+                              : make_uint4(0, 0, 0, 0);
+            rb[h] = (n < g.N) ? *reinterpret_cast<const uint4 *>(g.B + (size_t)n * g.ldb + k0 + lk)
+                              : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto store = [&](int buf, const uint4 (&ra)[2], const uint4 (&rb)[2]) {
+        for (int h = 0; h < 2; ++h) {
+            *reinterpret_cast<uint4 *>(&As[buf][lr + 64 * h][lk]) = ra[h];
+            *reinterpret_cast<uint4 *>(&Bs[buf][lr + 64 * h][lk]) = rb[h];
+        }
+    };
+    f32x16 acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+    uint4 ra[2], rb[2];
+    load(0, ra, rb);
+    store(0, ra, rb);
+    __syncthreads();
+    const int nst = g.K / BK;
+    for (int st = 0; st < nst; ++st) {
+        const int cur = st & 1;
+        if (st + 1 < nst) load((st + 1) * BK, ra, rb);   // in flight during the MFMAs
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int kc = kk + (lane >> 5) * 8;
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8 *>(&As[cur][wm * 64 + (lane & 31)][kc]);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8 *>(&As[cur][wm * 64 + 32 + (lane & 31)][kc]);
+            const bf16x8 b0 = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn * 64 + (lane & 31)][kc]);
+            const bf16x8 b1 = *reinterpret_cast<const bf16x8 *>(&Bs[cur][wn * 64 + 32 + (lane & 31)][kc]);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (st + 1 < nst) store(cur ^ 1, ra, rb);   // the other buffer: last read two stages ago
+        __syncthreads();
+    }
+    // epilogue: item r of lane -> row (r/4)*8 + (lane/32)*4 + r%4, col lane%32 of a 32 x 32 tile
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (n >= g.N) continue;
+        const float bias = g.bias ? g.bias[n] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                if (m >= g.M) continue;
+                const float v = acc[i][j][r] + bias;
+                if (EPI == EPI_F32) g.C[(size_t)m * g.ldc + n] = v;
+                else g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * sigm(v));
+            }
+        }
+    }
+}
+
 // X rows (fp32, n_in wide) -> bf16 [rows][K0], zero padded
 __global__ void k_cast_x(const float *x, int n_in, int rows, bf16 *X)
 {
@@ -219,13 +303,6 @@ __global__ void k_adam(float *p, const float *gr, float *m, float *v, const uint
     }
 }
 
-__global__ void k_copy_out(const float *Y, int rows, int np, int n_out, float *y)
-{
-    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (i >= (size_t)rows * n_out) return;
-    const int r = (int)(i / n_out), c = (int)(i % n_out);
-    y[i] = Y[(size_t)r * np + c];
-}
 
 }  // namespace mlp
 
@@ -314,8 +391,37 @@ static GemmArgs ga(const bf16 *A, int lda, const bf16 *B, int ldb, int M, int N,
     return g;
 }
 
-// forward of `rows` rows already in c->X; keep Z for backward when train
-static void forward_pass(mcs_mlp *c, int rows, bool train)
+template <int EPI>
+static void gemm_fwd(mcs_mlp *c, const GemmArgs &g)
+{
+    dim3 grid((g.N + FN - 1) / FN, (g.M + FM - 1) / FM);
+    hipLaunchKernelGGL((k_gemm_fwd<EPI>), grid, dim3(256), 0, c->stream, g);
+}
+
+// inference of `rows` rows already in c->X; outputs [rows][n_out] into `y`
+static void infer_pass(mcs_mlp *c, int rows, float *y)
+{
+    const int L = c->d.n_hidden_layers;
+    const bf16 *in = c->X;
+    int ld_in = K0;
+    for (int l = 0; l < L; ++l) {
+        const Layer &ly = c->layers[l];
+        GemmArgs g = ga(in, ld_in, c->wb + ly.wb_off, ly.pcol, rows, ly.prow, ly.pcol);
+        g.bias = c->p + ly.b_off;
+        g.Cb = c->A[l]; g.ldcb = ly.prow;
+        gemm_fwd<EPI_SILU_NOZ>(c, g);
+        in = c->A[l];
+        ld_in = ly.prow;
+    }
+    const Layer &lo = c->layers[L];
+    GemmArgs g = ga(in, ld_in, c->wb + lo.wb_off, lo.pcol, rows, c->d.n_out, lo.pcol);
+    g.bias = c->p + lo.b_off;
+    g.C = y; g.ldc = c->d.n_out;
+    gemm_fwd<EPI_F32>(c, g);
+}
+
+// training forward of `rows` rows already in c->X, keeping Z for backward
+static void forward_pass(mcs_mlp *c, int rows)
 {
     const int L = c->d.n_hidden_layers;
     const bf16 *in = c->X;
@@ -326,8 +432,7 @@ static void forward_pass(mcs_mlp *c, int rows, bool train)
         g.bias = c->p + ly.b_off;
         g.C = c->Z[l]; g.ldc = ly.prow;
         g.Cb = c->A[l]; g.ldcb = ly.prow;
-        if (train) gemm<false, false, EPI_SILU>(c, g);
-        else gemm<false, false, EPI_SILU_NOZ>(c, g);
+        gemm<false, false, EPI_SILU>(c, g);
         in = c->A[l];
         ld_in = ly.prow;
     }
@@ -552,9 +657,7 @@ int mcs_mlp_forward(mcs_mlp *c, const float *x, int64_t n, float *y)
         MCHK(c, hipMemcpyAsync(c->xin, x + s * ni, sizeof(float) * rows * ni, hipMemcpyHostToDevice, c->stream));
         MCHK(c, hipEventRecord(c->e0, c->stream));
         hipLaunchKernelGGL(k_cast_x, dim3((rows * K0 + 255) / 256), dim3(256), 0, c->stream, c->xin, ni, rows, c->X);
-        forward_pass(c, rows, false);
-        hipLaunchKernelGGL(k_copy_out, dim3((unsigned)(((size_t)rows * no + 255) / 256)), dim3(256), 0, c->stream,
-                           c->Y, rows, c->np, no, c->yout);
+        infer_pass(c, rows, c->yout);
         MCHK(c, hipEventRecord(c->e1, c->stream));
         MCHK(c, hipGetLastError());
         MCHK(c, hipMemcpyAsync(y + s * no, c->yout, sizeof(float) * rows * no, hipMemcpyDeviceToHost, c->stream));
@@ -600,7 +703,7 @@ int mcs_mlp_grads(mcs_mlp *c, const float *x, const float *t, const float *w, in
     MCHK(c, hipMemcpy(c->xin, x, sizeof(float) * n * ni, hipMemcpyHostToDevice));
     MCHK(c, hipMemcpy(c->T, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_cast_x, dim3((n * K0 + 255) / 256), dim3(256), 0, c->stream, c->xin, ni, n, c->X);
-    forward_pass(c, n, true);
+    forward_pass(c, n);
     backward_pass(c, n);
     MCHK(c, hipGetLastError());
     MCHK(c, hipStreamSynchronize(c->stream));
@@ -648,7 +751,7 @@ int mcs_mlp_train_steps(mcs_mlp *c, const int32_t *idx, int32_t n_steps, int32_t
     for (int s = 0; s < n_steps; ++s) {
         hipLaunchKernelGGL(k_gather, dim3(batch), dim3(64), 0, c->stream, c->xd, c->td, didx + (size_t)s * batch,
                            c->d.n_in, c->d.n_out, c->np, batch, c->n_data, c->X, c->T);
-        forward_pass(c, batch, true);
+        forward_pass(c, batch);
         backward_pass(c, batch);
         // k_loss accumulates into loss[0]; keep step s's value in loss[1 + s]
         (void)hipMemcpyAsync(c->loss + 1 + s, c->loss, 4, hipMemcpyDeviceToDevice, c->stream);
