@@ -195,18 +195,25 @@ struct NativeWalker {
         const float dtau = free_path(cur);
         z = dtau * muz;
         tau_acc = dtau;
+        bool done = false;
         if (z < -tau_tot) {                          // unscattered through the layer
             tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
-            if (lamb && reflects(P)) return reemit(P);
-            cond = 3;
-            return true;
-        }
-        if (cur.w < t_abs_all) {
+            if (lamb && reflects(P)) {
+                done = reemit(P);
+            } else {
+                cond = 3;
+                done = true;
+            }
+        } else if (cur.w < t_abs_all) {
             cond = (cur.w < t_abs5) ? 5 : 4;
-            return true;
+            done = true;
         }
-        return false;
+        if (!done && i >= P.max_steps) {             // capped, as after any step
+            cond = 0;
+            done = true;
+        }
+        return done;
     }
 
     // per-photon constants of (case, band)
@@ -819,7 +826,12 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
             if (P.cases[w.case_id(P)].lamb && w.reflects(P)) {
                 // re-emitted from the bottom: the next step, taken here
                 w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
-                if (w.reemit(P)) state = DONE;
+                if (w.reemit(P)) {
+                    state = DONE;
+                } else if (w.i >= P.max_steps) {
+                    w.cond = 0;
+                    state = DONE;
+                }
             } else {
                 w.cond = 2;
                 state = DONE;

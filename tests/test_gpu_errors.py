@@ -84,3 +84,30 @@ def test_destroy_null_and_errors_text():
     assert L.mcs_destroy(None) == -1
     ctx = ctypes.c_void_p()
     assert L.mcs_create(999, ctypes.byref(ctx)) == -1 and not ctx.value
+
+
+@pytest.mark.parametrize('coop', [0, 32])
+@pytest.mark.parametrize('cap', [1, 2, 7])
+def test_step_cap_with_lambertian_reemission(walker, coop, cap):
+    """A thin layer over a white Lambertian bottom: photons bounce often; no
+    record exceeds max_steps steps and capped photons stop at exactly
+    max_steps (n_scat = max_steps - 1), in the one-lane walk (coop_live 0) and
+    with most steps cooperative (one chunk per wave)."""
+    case = good_case(64 * walker.cus, tau_tot=0.3, r_lambertian=1.0, lambertian_bottom=True, ssa_ice=1.0)
+    walker.set_cases([case])
+    walker.set_launch(64, 1, 64, 1, coop)
+    try:
+        try:
+            walker.walk(records=True, max_steps=cap)
+        except McsError as e:
+            assert 'STEP_CAP' in str(e)
+        r = walker.records()
+        c = walker.counts()[0]
+        assert c[0] == case.n_photons
+        assert np.all(r['n_scat'] <= cap - 1)
+        capped = r['condition'] == 0
+        assert int(capped.sum()) == int(c[6])
+        assert np.all(r['n_scat'][capped] == cap - 1)
+        assert int(c[8]) == int(np.sum(r['n_scat'] + 1))
+    finally:
+        walker.set_launch()
