@@ -26,7 +26,7 @@ METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon 
 # still running when the queue is empty
 SNOW_1M_CANDIDATES = [(256, 5, 512, 16, 16), (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 16),
                       (256, 5, 512, 8, 16), (256, 5, 512, 16, 20), (1024, 1, 2048, 16, 24),
-                      (1024, 1, 2048, 16, 32), (512, 2, 1024, 16, 24)]
+                      (256, 6, 512, 16, 16), (512, 3, 1024, 16, 16)]
 N_PHOTONS = 1_000_000
 
 

@@ -152,7 +152,7 @@ struct NativeWalker {
     using Real = float;
     static constexpr bool COOP = true;        // has the wave-cooperative tail mode below
     static constexpr int MAX_BLOCK = 1024;
-    static constexpr int WAVES_PER_EU = 5;   // <= 96 VGPRs: 5 waves per SIMD without spills
+    // k_walk is built for 5 (96 VGPRs) and 6 (80 VGPRs) waves per SIMD
     float z, mux, muy, muz, tau_acc;
     uint32_t i;
     int band;
@@ -1067,18 +1067,21 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
     }
 }
 
-// Wave issue priority from the oldest live photon: long walks set the tail
-// of a launch, so their waves win issue slots over young ones.
-__device__ __forceinline__ void set_priority(uint32_t max_i)
+// Wave issue priority from the oldest live photon (i = its step count, 0 for
+// other lanes): long walks set the tail of a launch, so their waves win issue
+// slots over young ones.
+__device__ __forceinline__ void set_priority(uint32_t i)
 {
-    if (max_i > 6000u) __builtin_amdgcn_s_setprio(3);
-    else if (max_i > 3000u) __builtin_amdgcn_s_setprio(2);
-    else if (max_i > 1500u) __builtin_amdgcn_s_setprio(1);
+    if (__ballot(i > 6000u)) __builtin_amdgcn_s_setprio(3);
+    else if (__ballot(i > 3000u)) __builtin_amdgcn_s_setprio(2);
+    else if (__ballot(i > 1500u)) __builtin_amdgcn_s_setprio(1);
     else __builtin_amdgcn_s_setprio(0);
 }
 
-template <class W>
-__global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(W::WAVES_PER_EU)))
+// WPE: waves per SIMD the register allocation targets (the launch picks the
+// instance that fits its shape).
+template <class W, int WPE>
+__global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_walk(const KParams P)
 {
     __shared__ BlockState bs;
@@ -1146,16 +1149,7 @@ k_walk(const KParams P)
             }
             const unsigned long long nl = __ballot(state == LIVE);
             if (nl == 0ull || 64 - __popcll(nl) >= thresh) break;
-            if ((iter & 255u) == 0u) {
-                uint32_t mi = (state == LIVE) ? w.i : 0u;
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 1));
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 2));
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 4));
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 8));
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 16));
-                mi = max(mi, (uint32_t)__shfl_xor((int)mi, 32));
-                set_priority(mi);
-            }
+            if ((iter & 255u) == 0u) set_priority(state == LIVE ? w.i : 0u);
         }
     }
 }
@@ -1450,9 +1444,16 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
-        hipLaunchKernelGGL(k_walk<NativeWalker>, dim3(grid), dim3(block), tab, c->stream, P);
+        // shapes asking for more than 5 waves per SIMD get the 80-VGPR instance
+        // (6 waves per SIMD; its spills sit outside the step loop)
+        const int waves_per_simd = (c->launch.blocks_per_cu * (block / 64) + 3) / 4;
+        if (waves_per_simd > 5)
+            hipLaunchKernelGGL((k_walk<NativeWalker, 6>), dim3(grid), dim3(block), tab, c->stream, P);
+        else
+            hipLaunchKernelGGL((k_walk<NativeWalker, 5>), dim3(grid), dim3(block), tab, c->stream, P);
     } else {
-        hipLaunchKernelGGL(k_walk<ReplayWalker>, dim3(grid), dim3(block), 0, c->stream, P);
+        hipLaunchKernelGGL((k_walk<ReplayWalker, ReplayWalker::WAVES_PER_EU>), dim3(grid), dim3(block), 0,
+                           c->stream, P);
     }
     if (events && hipEventRecord(c->ev1, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
