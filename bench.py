@@ -23,10 +23,11 @@ sys.path.insert(0, ROOT)
 METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon MC'
 # (block, blocks_per_cu, chunk, refill_min, coop_live): small chunks spread the
 # 1e6 photons over every block; the cooperative tail mode finishes the walks
-# still running when the queue is empty
-SNOW_1M_CANDIDATES = [(256, 5, 512, 16, 16), (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 16),
-                      (256, 5, 512, 8, 16), (256, 5, 512, 16, 20), (1024, 1, 2048, 16, 24),
-                      (256, 6, 512, 16, 16), (512, 3, 1024, 16, 16)]
+# still running when the queue is empty, and 16-wave blocks spread those
+# photons over the waves of the block that run out of work
+SNOW_1M_CANDIDATES = [(1024, 1, 2048, 16, 24), (1024, 1, 2048, 16, 20), (1024, 1, 2048, 16, 32),
+                      (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 24), (512, 2, 1024, 16, 16),
+                      (256, 5, 512, 16, 16), (512, 3, 1024, 16, 16)]
 N_PHOTONS = 1_000_000
 
 
@@ -79,6 +80,7 @@ def main():
     t1 = time.perf_counter()
     cnt = w.counts()[0]
     coop_frac = float(cnt[9]) / max(float(cnt[8]), 1.0)
+    handed_over = int(cnt[10])
     fr = fractions_from_tallies(w.tallies())
     # PCIe-inclusive: the same walk plus the D2H copy of all records (not `value`)
     t2 = time.perf_counter()
@@ -111,6 +113,7 @@ def main():
         'kernel_ms_avg': round(avg_kern, 4),
         'launch': list(shape),
         'coop_steps_frac': round(coop_frac, 4),
+        'photons_handed_over': handed_over,
         'albedo': round(float(fr['albedo'][0]), 6),
         'roofline': {
             'bound': 'valu', 'achieved': round(rf['fp32_tflops'], 4),

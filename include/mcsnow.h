@@ -189,7 +189,8 @@ int mcs_read_tallies(mcs_ctx *ctx, double *out, int32_t n_cases);
 /* Per-case counts (n_cases x MCS_TALLY_U64):
  *   0 photons, 1..5 photons by condition, 6 capped, 7 phi on an axis,
  *   8 photon-steps, 9 photon-steps walked in the cooperative tail mode
- *   (mcs_launch.coop_live), 10..15 reserved (0) */
+ *   (mcs_launch.coop_live), 10 photons handed from one wave's cooperative
+ *   tail to an idle wave of the same block, 11..15 reserved (0) */
 int mcs_read_counts(mcs_ctx *ctx, uint64_t *out, int32_t n_cases);
 
 int mcs_last_stats(const mcs_ctx *ctx, mcs_stats *out);

@@ -46,7 +46,7 @@ constexpr int T_QCOND = MCS_BRF_BINS + 1;    // 649..653 by condition 1..5
 constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
 constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
 // counts (uint64)
-constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9;
+constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED = 10;
 
 struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
@@ -91,7 +91,7 @@ struct KParams {
     double *tally_f;
     unsigned long long *tally_u;
     unsigned int *chunk_counter;
-    unsigned long long *ctl;     // [0] replay slice errors
+    unsigned long long *ctl;     // [0] replay slice errors, [1] failed hand-overs
     uint32_t *rec_gid, *rec_nscat, *rec_cb;
     void *rec_theta, *rec_phi, *rec_path;
 };
@@ -117,6 +117,13 @@ struct BlockState {
     unsigned long long cur;  // (seq << 32) | next photon offset in chunk
     int lock;
     int exhausted;
+    // hand-over of cooperative-tail photons between the block's waves
+    int holders;             // waves not waiting for photons
+    int idle;                // waves waiting for photons
+    int mb_state;            // mailbox: 0 free, 3 being filled, 1 posted, 2 being read
+    int mb_wave;             // posting wave (its coop table holds the photons)
+    unsigned int mb_mask;    // their table entries
+    int mb_pad;
     SlotHdr hdr[SLOTS];
     SlotTally tal[SLOTS];
 };
@@ -170,16 +177,7 @@ struct NativeWalker {
     __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
     {
         const DevCase &C = P.cases[case_id];
-        int bnd = 0;
-        if (P.photon_band) {
-            bnd = P.photon_band[gid];
-        } else if (C.n_bands > 1) {
-            const u32x4 q = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
-            const double u = ((double)q.x + 0.5) * 2.3283064365386963e-10;
-            bnd = C.n_bands - 1;
-            for (int k = 0; k < C.n_bands - 1; ++k)
-                if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
-        }
+        const int bnd = select_band(P, C, case_id, photon, gid);
         band = bnd;
         load_params(P, C, bnd);
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
@@ -214,6 +212,38 @@ struct NativeWalker {
             done = true;
         }
         return done;
+    }
+
+    // the photon's wavelength bin: given per photon, or drawn from the case's
+    // band CDF with Philox block (photon, case, 0, 2)
+    __device__ static int select_band(const KParams &P, const DevCase &C, int case_id, uint32_t photon, uint64_t gid)
+    {
+        int bnd = 0;
+        if (P.photon_band) {
+            bnd = P.photon_band[gid];
+        } else if (C.n_bands > 1) {
+            const u32x4 q = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
+            const double u = ((double)q.x + 0.5) * 2.3283064365386963e-10;
+            bnd = C.n_bands - 1;
+            for (int k = 0; k < C.n_bands - 1; ++k)
+                if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
+        }
+        return bnd;
+    }
+
+    // Continue a photon handed over by another wave of the block: its Philox
+    // words and walk state; band and constants are derived again.
+    __device__ void resume(const KParams &P, const PhiloxPre &p, uint64_t gid, float z_, float tau_,
+                           float dx, float dy, float dz, uint32_t i_)
+    {
+        pre = p;
+        const uint32_t cs = case_id(P);
+        const DevCase &C = P.cases[cs];
+        band = select_band(P, C, (int)cs, photon_id(), gid);
+        load_params(P, C, band);
+        z = z_; tau_acc = tau_; mux = dx; muy = dy; muz = dz; i = i_;
+        cond = 0;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
     }
 
     // per-photon constants of (case, band)
@@ -704,6 +734,24 @@ __device__ __forceinline__ float rdlane63(float v)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 
+// Groups carrying a photon's state (no event in their last rounds) hand it
+// back to the photon's lane.
+__device__ __forceinline__ void coop_handback(NativeWalker &w, const CoopGroups &cg, unsigned long long *coop_steps)
+{
+    const int lane = lane_id();
+    const bool was = (cg.em >> lane) & 1ull;
+    const int src = was ? (lane_rank(cg.em) << cg.lg) : lane;
+    const float z = bperm(src, cg.z0), t = bperm(src, cg.tau0);
+    float x, y, u;
+    frame_dir(cg.f, x, y, u);
+    x = bperm(src, x); y = bperm(src, y); u = bperm(src, u);
+    const uint32_t i = bperm(src, cg.i0);
+    if (was) {
+        atomicAdd(coop_steps, (unsigned long long)(i - w.i));
+        w.i = i; w.z = z; w.tau_acc = t; w.mux = x; w.muy = y; w.muz = u;
+    }
+}
+
 // One cooperative round of a wave (all 64 lanes call; no divergence around
 // the lane permutes).  `tab`: the wave's entries; `map`: group -> entry;
 // `me`: this lane's photon's entry.
@@ -714,20 +762,8 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     const bool elig = state == LIVE;
     const unsigned long long em = __ballot(elig);
     if (em != cg.em) {
-        if (cg.em) {
-            // a photon joined: hand the carried state back before regrouping
-            const bool was = (cg.em >> lane) & 1ull;
-            const int src = was ? (lane_rank(cg.em) << cg.lg) : lane;
-            const float z = bperm(src, cg.z0), t = bperm(src, cg.tau0);
-            float x, y, u;
-            frame_dir(cg.f, x, y, u);
-            x = bperm(src, x); y = bperm(src, y); u = bperm(src, u);
-            const uint32_t i = bperm(src, cg.i0);
-            if (was) {
-                atomicAdd(coop_steps, (unsigned long long)(i - w.i));
-                w.i = i; w.z = z; w.tau_acc = t; w.mux = x; w.muy = y; w.muz = u;
-            }
-        }
+        // a photon joined: hand the carried state back before regrouping
+        if (cg.em) coop_handback(w, cg, coop_steps);
         cg.em = em;
         if (em == 0ull) return;
         const int L = __popcll(em);
@@ -847,10 +883,48 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     }
 }
 
-// The cooperative phase of a wave: every photon of the wave is finished when
-// it returns.  `tab`: coop_live entries; `map`: coop_live group slots.
-__device__ __forceinline__ void coop_phase(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab, int *map,
-                                           unsigned long long *coop_steps)
+// Hand-over within the block.  A wave that has finished its photons waits
+// (`idle`) for a mailbox post; a wave in the cooperative phase with L >= 2
+// photons posts the upper half of them while a sibling waits: their walk state
+// goes into their table entries, the entry mask and the wave into the
+// mailbox, and their lanes are emptied.  The photons keep their chunk slot
+// (found again from case and photon id), so tallies, records and chunk
+// completion are unchanged; each photon still walks its own Philox stream.
+// Spread over the block's four SIMDs, the last long walks get more lanes each.
+__device__ __forceinline__ void coop_donate(BlockState &bs, NativeWalker &w, int &state, CoopEntry *tab, int me,
+                                            CoopGroups &cg, unsigned long long *coop_steps, unsigned long long *moved,
+                                            int wave)
+{
+    const int lane = lane_id();
+    const unsigned long long live = __ballot(state == LIVE);
+    const int L = __popcll(live);
+    if (L < 2 || lds_load(&bs.idle) == 0 || lds_load(&bs.mb_state) != 0) return;
+    int ok = 0;
+    if (lane == 0) ok = atomicCAS(&bs.mb_state, 0, 3) == 0;
+    if (!bcast(ok, 0)) return;
+    if (cg.em) coop_handback(w, cg, coop_steps);
+    cg.em = 0ull;   // regroup next round
+    const int nd = L >> 1;
+    if (lane == 0) bs.mb_mask = 0u;
+    if (state == LIVE && lane_rank(live) >= L - nd) {
+        CoopEntry &E = tab[me];
+        E.z = w.z; E.tau = w.tau_acc; E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
+        atomicOr(&bs.mb_mask, 1u << me);
+        atomicAdd(moved, 1ull);
+        state = EMPTY;
+    }
+    if (lane == 0) bs.mb_wave = wave;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) __atomic_store_n(&bs.mb_state, 1, __ATOMIC_RELAXED);
+}
+
+// The cooperative phase of a wave: every photon the wave still holds is
+// finished (or, with HAND, handed over) when it returns.  `tab`: coop_live
+// entries; `map`: coop_live group slots.
+template <bool HAND>
+__device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int &state, const KParams &P,
+                                           CoopEntry *tab, int *map, unsigned long long *coop_steps,
+                                           unsigned long long *moved, int wave)
 {
     const unsigned long long live0 = __ballot(state == LIVE);
     const int me = lane_rank(live0);
@@ -864,7 +938,73 @@ __device__ __forceinline__ void coop_phase(NativeWalker &w, int &state, const KP
     cg.em = 0ull;
     cg.lg = 0;
     cg.ent = 0;
-    while (__ballot(state == LIVE)) coop_round(w, state, P, tab, map, me, cg, coop_steps);
+    for (uint32_t round = 1; __ballot(state == LIVE); ++round) {
+        coop_round(w, state, P, tab, map, me, cg, coop_steps);
+        if constexpr (HAND) {
+            if ((round & 3u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
+        }
+    }
+}
+
+// Wait for photons posted by a sibling wave (all lanes EMPTY on entry).  True
+// with the posted photons LIVE on the first lanes; false once no sibling holds
+// photons and nothing is posted.
+__device__ bool coop_receive(BlockState &bs, const KParams &P, NativeWalker &w, int &state, int &my_slot,
+                             uint64_t &my_gid, const CoopEntry *coop_tab)
+{
+    const int lane = lane_id();
+    if (lane == 0) {
+        atomicAdd(&bs.idle, 1);
+        atomicSub(&bs.holders, 1);
+    }
+    __builtin_amdgcn_s_setprio(0);
+    for (int spin = 0; spin < (1 << 24); ++spin) {
+        if (lds_load(&bs.mb_state) == 1) {
+            int ok = 0;
+            if (lane == 0) ok = atomicCAS(&bs.mb_state, 1, 2) == 1;
+            if (!bcast(ok, 0)) continue;
+            if (lane == 0) {
+                atomicAdd(&bs.holders, 1);
+                atomicSub(&bs.idle, 1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            const int dw = lds_load(&bs.mb_wave);
+            const unsigned int mask = lds_load(&bs.mb_mask);
+            if (lane < __popc(mask)) {
+                unsigned int m = mask;
+                for (int k = 0; k < lane; ++k) m &= m - 1u;
+                const CoopEntry &E = coop_tab[dw * P.coop_live + (__ffs(m) - 1)];
+                const PhiloxPre pre{E.ck0, E.h0, E.l0};
+                const uint32_t cs = pre.ck0 ^ P.key0, ph = pre.l0 * 0x991A7CDBu;
+                int slot = -1;
+                for (int s = 0; s < SLOTS; ++s) {
+                    const SlotHdr &H = bs.hdr[s];
+                    if (lds_load(&H.state) == 1 && (uint32_t)H.case_id == cs &&
+                        ph - H.p0 < (uint32_t)lds_load(&H.seqcount))
+                        slot = s;
+                }
+                if (slot >= 0) {
+                    my_slot = slot;
+                    my_gid = bs.hdr[slot].gid0 + (ph - bs.hdr[slot].p0);
+                    w.resume(P, pre, my_gid, E.z, E.tau, E.dx, E.dy, E.dz, E.i);
+                    state = LIVE;
+                } else {
+                    atomicAdd(P.ctl + 1, 1ull);   // cannot happen: the chunk waits for the photon
+                }
+            }
+            // the entries are read before the mailbox is free again
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            if (lane == 0) __atomic_store_n(&bs.mb_state, 0, __ATOMIC_RELAXED);
+            return true;
+        }
+        if (lds_load(&bs.holders) == 0 && lds_load(&bs.mb_state) == 0) {
+            if (lane == 0) atomicSub(&bs.idle, 1);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(4);
+    }
+    if (lane == 0) atomicAdd(P.ctl + 1, 1ull);   // cannot happen: siblings finish
+    return false;
 }
 
 // ------------------------------------------------------------ chunk queue
@@ -1079,11 +1219,14 @@ __device__ __forceinline__ void set_priority(uint32_t i)
 }
 
 // WPE: waves per SIMD the register allocation targets (the launch picks the
-// instance that fits its shape).
+// instance that fits its shape).  The hand-over of cooperative-tail photons
+// between waves is left out of the 6-wave build: at 80 VGPRs its code would
+// spill in the step loop.
 template <class W, int WPE>
 __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_walk(const KParams P)
 {
+    constexpr bool HAND = W::COOP && WPE <= 5;
     __shared__ BlockState bs;
     // native mode: coop_live table entries per wave, then coop_live group slots per wave
     extern __shared__ CoopEntry coop_tab[];
@@ -1095,6 +1238,7 @@ k_walk(const KParams P)
         for (int k = threadIdx.x; k < (int)(sizeof(BlockState) / 8); k += blockDim.x) w64[k] = 0ull;
     }
     __syncthreads();
+    if (threadIdx.x == 0) bs.holders = blockDim.x >> 6;
     if (wave == 0) {
         if (!acquire_chunk(bs, 0u, P)) {
             if (lane == 0) bs.hdr[0].seqcount = 0ull;  // seq 0, count 0
@@ -1108,6 +1252,7 @@ k_walk(const KParams P)
     int my_slot = 0;
     uint64_t my_gid = 0;
     int spins = 0;
+    bool to_coop = false;
 
     for (;;) {
         // ---------------- cold: retire finished lanes, refill empty ones
@@ -1126,11 +1271,14 @@ k_walk(const KParams P)
         if constexpr (W::COOP) {
             // queue empty and few photons left: walk them cooperatively
             if (lds_load(&bs.exhausted) && __popcll(live) <= P.coop_live) {
-                __builtin_amdgcn_s_setprio(3);
-                coop_phase(w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
-                           &bs.tal[my_slot].u[U_COOP]);
-                // the queue is empty and every photon of the wave is finished
-                retire(bs, P, w, state, my_slot, my_gid);
+                if constexpr (HAND) {
+                    to_coop = true;   // below, with the hand-over
+                } else {
+                    __builtin_amdgcn_s_setprio(3);
+                    coop_phase<false>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
+                                      &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
+                    retire(bs, P, w, state, my_slot, my_gid);
+                }
                 break;
             }
         }
@@ -1150,6 +1298,24 @@ k_walk(const KParams P)
             const unsigned long long nl = __ballot(state == LIVE);
             if (nl == 0ull || 64 - __popcll(nl) >= thresh) break;
             if ((iter & 255u) == 0u) set_priority(state == LIVE ? w.i : 0u);
+        }
+    }
+    if constexpr (HAND) {
+        // the cooperative tail: the wave's last photons, then photons taken
+        // over from siblings (at most coop_live / 2 at a time) until the block
+        // has none left
+        if (P.coop_live > 0) {
+            for (;;) {
+                if (to_coop) {
+                    __builtin_amdgcn_s_setprio(3);
+                    coop_phase<true>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
+                                     &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
+                    // every photon the wave held is finished or handed over
+                    retire(bs, P, w, state, my_slot, my_gid);
+                }
+                if (!coop_receive(bs, P, w, state, my_slot, my_gid, coop_tab)) break;
+                to_coop = true;
+            }
         }
     }
 }
@@ -1626,6 +1792,9 @@ static int finish_walk(mcs_ctx *c)
     }
     s.kernel_ms = ms;
     c->stats = s;
+    if (ctl[1])
+        return set_err(c, MCS_ERR_HIP, "internal: %llu failed photon hand-overs between waves",
+                       (unsigned long long)ctl[1]);
     if (s.photons != (uint64_t)c->last_n)
         return set_err(c, MCS_ERR_HIP, "walk finished %llu of %lld photons",
                        (unsigned long long)s.photons, (long long)c->last_n);

@@ -22,7 +22,7 @@ T_QDOWN = BRF_BINS
 T_QCOND = BRF_BINS + 1          # +cond-1 for condition 1..5
 T_QCAP = BRF_BINS + 6
 T_QPATH = BRF_BINS + 7
-U_N, U_CAP, U_AXIS, U_STEPS, U_COOP = 0, 6, 7, 8, 9
+U_N, U_CAP, U_AXIS, U_STEPS, U_COOP, U_MOVED = 0, 6, 7, 8, 9, 10
 
 
 @dataclass

@@ -126,3 +126,73 @@ def test_coop_step_cap(walker):
         assert c[U_COOP] > 0
     finally:
         walker.set_launch()
+
+
+U_MOVED = 10
+
+
+def handover_runs(walker, case_fn, seeds, coop_live, records=True):
+    """One 1024-photon chunk per 16-wave block: the queue is empty at once, and
+    waves that finish early take over photons of their siblings' cooperative
+    tails."""
+    n = 1024 * walker.cus
+    walker.set_cases([case_fn(n)])
+    walker.set_launch(1024, 1, 1024, 16, coop_live)
+    recs, counts = [], np.zeros(16, np.uint64)
+    try:
+        for s in seeds:
+            walker.walk(mode='native', seed=s, records=records)
+            counts += walker.counts()[0]
+            if records:
+                recs.append(walker.records())
+    finally:
+        walker.set_launch()
+    out = {k: np.concatenate([r[k] for r in recs]) for k in recs[0]} if records else {}
+    return out, counts
+
+
+def test_hand_over_between_waves_keeps_the_law(walker):
+    """Photons handed from one wave's cooperative tail to an idle sibling keep
+    their walk: every photon is recorded once, and exit fractions, BRF and step
+    counts agree with the one-lane walk at 3 sigma / chi^2."""
+    case = lambda n: gpu_case(n, 40.0, 30.0, False, 0.0, g=0.85, ssa=0.9995)
+    a, ca = handover_runs(walker, case, range(3), 32)
+    b, cb = handover_runs(walker, case, range(3, 6), 0)
+    print('handed over: %d of %d photons' % (int(ca[U_MOVED]), int(ca[0])), flush=True)
+    assert ca[U_MOVED] > 0 and cb[U_MOVED] == 0
+    assert int(ca[U_STEPS]) == int(np.sum(a['n_scat'] + 1))
+    assert np.all(a['condition'] >= 1)   # no photon lost or left unwritten
+    n_a, n_b = a['condition'].size, b['condition'].size
+    for cond in (1, 2, 4):
+        z = two_sample_z(int(np.sum(a['condition'] == cond)), n_a, int(np.sum(b['condition'] == cond)), n_b)
+        assert abs(z) < 3.0, (cond, z)
+    ha = brf_counts(a['theta_n'][a['condition'] == 1], a['phi_n'][a['condition'] == 1])
+    hb = brf_counts(b['theta_n'][b['condition'] == 1], b['phi_n'][b['condition'] == 1])
+    keep = (ha + hb) >= 10
+    chi2 = np.sum((np.sqrt(n_b / n_a) * ha[keep] - np.sqrt(n_a / n_b) * hb[keep]) ** 2 / (ha[keep] + hb[keep]))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, chi2
+    assert stats.ks_2samp(a['n_scat'], b['n_scat']).pvalue > 1e-3
+    assert stats.ks_2samp(a['path_length'][a['condition'] == 1],
+                          b['path_length'][b['condition'] == 1]).pvalue > 1e-3
+
+
+def test_hand_over_with_lambertian_bottom_and_step_cap(walker):
+    """Handed-over photons that bounce off a Lambertian bottom or reach
+    max_steps stop exactly there, in their own chunk's records."""
+    from monte_carlompi_amd._lib import McsError
+    case = lambda n: gpu_case(n, 30.0, 0.0, True, 0.9, g=0.85, ssa=1.0)
+    walker.set_cases([case(1024 * walker.cus)])
+    walker.set_launch(1024, 1, 1024, 16, 32)
+    try:
+        with pytest.raises(McsError, match='STEP_CAP'):
+            walker.walk(mode='native', seed=21, records=True, max_steps=600)
+        r = walker.records()
+        c = walker.counts()[0]
+        assert c[U_MOVED] > 0
+        assert np.all(r['n_scat'] <= 599)
+        assert int(np.sum(r['condition'] == 0)) == int(c[6]) > 0
+        assert np.all(r['n_scat'][r['condition'] == 0] == 599)
+        assert int(c[U_STEPS]) == int(np.sum(r['n_scat'] + 1))
+        assert set(np.unique(r['condition']).tolist()) <= {0, 1, 2, 3}   # no absorption at omega 1
+    finally:
+        walker.set_launch()
