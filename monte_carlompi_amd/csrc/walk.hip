@@ -1338,12 +1338,93 @@ __global__ void k_scatter_records(const uint32_t *gid, const uint32_t *nscat, co
     }
 }
 
-// fill (graph-safe replacement for hipMemsetAsync)
-__global__ void k_fill_u64(unsigned long long *p, uint64_t n, unsigned long long v)
+// records: compacted order -> photon order, one chunk per block pass.  A
+// chunk's records sit at its own id range [gid0, gid0 + count) in any order
+// (retirement order), so the block gathers them into LDS by id and writes
+// them out contiguously: coalesced reads and writes.  LDS: chunk x (8 + 3
+// sizeof(R)) bytes.
+template <class R>
+__global__ void k_order_records(const DevCase *cases, const int32_t *chunk0, int n_cases, int chunk,
+                                uint32_t n_chunks, const uint32_t *gid, const uint32_t *nscat, const uint32_t *cb,
+                                const R *th, const R *ph, const R *pa, uint32_t *o_nscat, uint32_t *o_cb, R *o_th,
+                                R *o_ph, R *o_pa)
 {
+    extern __shared__ uint32_t ord_lds[];
+    uint32_t *s_ns = ord_lds, *s_cb = ord_lds + chunk;
+    R *s_th = reinterpret_cast<R *>(ord_lds + 2 * chunk), *s_ph = s_th + chunk, *s_pa = s_ph + chunk;
+    for (uint32_t j = blockIdx.x; j < n_chunks; j += gridDim.x) {
+        int lo = 0, hi = n_cases;   // chunk0[lo] <= j < chunk0[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)chunk0[mid] <= j) lo = mid; else hi = mid;
+        }
+        const DevCase &C = cases[lo];
+        const int64_t p0 = (int64_t)(j - (uint32_t)chunk0[lo]) * chunk;
+        const int count = (int)min<int64_t>(chunk, C.n_photons - p0);
+        const uint64_t g0 = (uint64_t)(C.gid_base + p0);
+        for (int t = threadIdx.x; t < count; t += blockDim.x) {
+            const uint64_t pos = g0 + t;
+            const uint32_t k = gid[pos] - (uint32_t)g0;
+            if (k < (uint32_t)count) {
+                s_ns[k] = nscat[pos];
+                s_cb[k] = cb[pos];
+                s_th[k] = th[pos];
+                s_ph[k] = ph[pos];
+                s_pa[k] = pa[pos];
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < count; t += blockDim.x) {
+            const uint64_t pos = g0 + t;
+            o_nscat[pos] = s_ns[t];
+            o_cb[pos] = s_cb[t];
+            o_th[pos] = s_th[t];
+            o_ph[pos] = s_ph[t];
+            o_pa[pos] = s_pa[t];
+        }
+        __syncthreads();
+    }
+}
+
+// Zero the tallies, counts and control words of a walk (one launch; a kernel,
+// not a memset, so that the walk can be captured in a graph).
+__global__ void k_clear(unsigned long long *a, uint64_t na, unsigned long long *b, uint64_t nb,
+                        unsigned long long *c, uint64_t nc)
+{
+    const uint64_t n = na + nb + nc;
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n;
-         j += (uint64_t)gridDim.x * blockDim.x)
-        p[j] = v;
+         j += (uint64_t)gridDim.x * blockDim.x) {
+        if (j < na) a[j] = 0ull;
+        else if (j < na + nb) b[j - na] = 0ull;
+        else c[j - na - nb] = 0ull;
+    }
+}
+
+// The walk's checks for the host: photons, photon-steps, capped, phi on an
+// axis (summed over cases), control words 0 and 1, and a written-flag, into
+// pinned host memory (one block of 256 threads).
+__global__ void k_summary(const unsigned long long *tu, int nc, const unsigned long long *ctl,
+                          unsigned long long *out)
+{
+    __shared__ unsigned long long red[4][256];
+    const int t = threadIdx.x;
+    unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = t; k < nc; k += 256) {
+        const unsigned long long *u = tu + (size_t)k * TU;
+        a0 += u[U_N]; a1 += u[U_STEPS]; a2 += u[U_CAP]; a3 += u[U_AXIS];
+    }
+    red[0][t] = a0; red[1][t] = a1; red[2][t] = a2; red[3][t] = a3;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[0] = red[0][0]; out[1] = red[1][0]; out[2] = red[2][0]; out[3] = red[3][0];
+        out[4] = ctl[0]; out[5] = ctl[1];
+        out[7] = 1ull;
+    }
 }
 
 __global__ void k_philox(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out, uint64_t n)
@@ -1390,6 +1471,11 @@ struct mcs_ctx {
     hipGraphExec_t gexec = nullptr;
     hipGraph_t graph = nullptr;
     uint64_t gkey = 0;
+    // chunk table on the device (uploaded when it changes)
+    std::vector<int32_t> chunk0;
+    bool chunk0_valid = false;
+    // pinned host words written by k_summary (h_sum) and its device address
+    unsigned long long *h_sum = nullptr, *d_sum = nullptr;
 };
 
 static int set_err(mcs_ctx *c, int code, const char *fmt, ...)
@@ -1454,8 +1540,10 @@ int mcs_create(int device, mcs_ctx **out)
     if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete c; return MCS_ERR_HIP; }
     c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_sum, 8 * sizeof(unsigned long long), hipHostMallocMapped) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&c->d_sum, c->h_sum, 0) != hipSuccess) {
+        (void)mcs_destroy(c);
         return MCS_ERR_HIP;
     }
     *out = c;
@@ -1476,6 +1564,7 @@ int mcs_destroy(mcs_ctx *c)
     for (auto &b : c->d_out) if (b.p) (void)hipFree(b.p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->h_sum) (void)hipHostFree(c->h_sum);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MCS_OK;
@@ -1549,6 +1638,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
     if ((r = ensure(c, c->d_tally_f, sizeof(double) * TF * n_cases))) return r;
     if ((r = ensure(c, c->d_tally_u, sizeof(unsigned long long) * TU * n_cases))) return r;
     if ((r = ensure(c, c->d_chunk0, sizeof(int32_t) * (n_cases + 1)))) return r;
+    c->chunk0_valid = false;
     if ((r = ensure(c, c->d_ctl, sizeof(unsigned long long) * 16))) return r;
     HIPCHK(c, hipMemcpyAsync(c->d_cases.p, dc.data(), sizeof(DevCase) * n_cases, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_bands.p, db.data(), sizeof(DevBand) * n_bands, hipMemcpyHostToDevice, c->stream));
@@ -1594,13 +1684,12 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                         bool events)
 {
     const int nc = (int)c->cases.size();
-    const unsigned blocks_fill = 256;
-    hipLaunchKernelGGL(k_fill_u64, dim3(blocks_fill), dim3(256), 0, c->stream,
-                       (unsigned long long *)c->d_tally_f.p, (uint64_t)TF * nc, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(64), dim3(256), 0, c->stream,
-                       (unsigned long long *)c->d_tally_u.p, (uint64_t)TU * nc, 0ull);
-    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, c->stream,
-                       (unsigned long long *)c->d_ctl.p, (uint64_t)16, 0ull);
+    const uint64_t nclear = (uint64_t)(TF + TU) * nc + 16;
+    const unsigned blocks_clear = (unsigned)std::min<uint64_t>((nclear + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_clear, dim3(blocks_clear), dim3(256), 0, c->stream,
+                       (unsigned long long *)c->d_tally_f.p, (uint64_t)TF * nc,
+                       (unsigned long long *)c->d_tally_u.p, (uint64_t)TU * nc,
+                       (unsigned long long *)c->d_ctl.p, (uint64_t)16);
     const int block = (mode == MCS_REPLAY) ? std::min(c->launch.block, ReplayWalker::MAX_BLOCK)
                                            : c->launch.block;
     int grid = c->launch.blocks_per_cu * c->cus;
@@ -1623,7 +1712,26 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     }
     if (events && hipEventRecord(c->ev1, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
-    if (records && n > 0) {
+    const size_t rsz = (mode == MCS_NATIVE) ? sizeof(float) : sizeof(double);
+    const size_t ord_lds = (size_t)c->launch.chunk * (8 + 3 * rsz);
+    if (records && n > 0 && ord_lds <= 65536) {
+        const unsigned g2 = std::min<uint32_t>(n_chunks, 4096u);
+        if (mode == MCS_NATIVE)
+            hipLaunchKernelGGL(k_order_records<float>, dim3(g2), dim3(256), ord_lds, c->stream,
+                               P.cases, P.case_chunk0, nc, c->launch.chunk, n_chunks,
+                               (uint32_t *)c->d_rec[0].p, (uint32_t *)c->d_rec[1].p, (uint32_t *)c->d_rec[2].p,
+                               (float *)c->d_rec[3].p, (float *)c->d_rec[4].p, (float *)c->d_rec[5].p,
+                               (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (float *)c->d_out[2].p,
+                               (float *)c->d_out[3].p, (float *)c->d_out[4].p);
+        else
+            hipLaunchKernelGGL(k_order_records<double>, dim3(g2), dim3(256), ord_lds, c->stream,
+                               P.cases, P.case_chunk0, nc, c->launch.chunk, n_chunks,
+                               (uint32_t *)c->d_rec[0].p, (uint32_t *)c->d_rec[1].p, (uint32_t *)c->d_rec[2].p,
+                               (double *)c->d_rec[3].p, (double *)c->d_rec[4].p, (double *)c->d_rec[5].p,
+                               (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (double *)c->d_out[2].p,
+                               (double *)c->d_out[3].p, (double *)c->d_out[4].p);
+    } else if (records && n > 0) {
+        // chunks too large for LDS: scatter record by record
         const unsigned g2 = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
         if (mode == MCS_NATIVE)
             hipLaunchKernelGGL(k_scatter_records<float>, dim3(g2), dim3(256), 0, c->stream,
@@ -1638,6 +1746,8 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                                (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (double *)c->d_out[2].p,
                                (double *)c->d_out[3].p, (double *)c->d_out[4].p);
     }
+    hipLaunchKernelGGL(k_summary, dim3(1), dim3(256), 0, c->stream, (const unsigned long long *)c->d_tally_u.p, nc,
+                       (const unsigned long long *)c->d_ctl.p, c->d_sum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_err(c, MCS_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
     return MCS_OK;
@@ -1691,7 +1801,13 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     }
     chunk0[nc] = (int32_t)nch;
     int r;
-    HIPCHK(c, hipMemcpyAsync(c->d_chunk0.p, chunk0.data(), sizeof(int32_t) * (nc + 1), hipMemcpyHostToDevice, c->stream));
+    if (!c->chunk0_valid || c->chunk0 != chunk0) {
+        c->chunk0 = chunk0;
+        c->chunk0_valid = false;
+        HIPCHK(c, hipMemcpyAsync(c->d_chunk0.p, c->chunk0.data(), sizeof(int32_t) * (nc + 1), hipMemcpyHostToDevice,
+                                 c->stream));
+        c->chunk0_valid = true;
+    }
     if (a->photon_band) {
         if ((r = ensure(c, c->d_pband, (size_t)n))) return r;
         HIPCHK(c, hipMemcpyAsync(c->d_pband.p, a->photon_band, (size_t)n, hipMemcpyHostToDevice, c->stream));
@@ -1737,6 +1853,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
         P.rec_phi = c->d_rec[4].p;
         P.rec_path = c->d_rec[5].p;
     }
+    c->h_sum[7] = 0ull;   // set again by this walk's k_summary
     if (a->use_graph) {
         uint64_t key = 0x1234;
         const uint64_t *w = (const uint64_t *)&P;
@@ -1779,17 +1896,15 @@ static int finish_walk(mcs_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
-    const int nc = (int)c->cases.size();
-    std::vector<unsigned long long> u((size_t)TU * nc), ctl(16);
-    HIPCHK(c, hipMemcpy(u.data(), c->d_tally_u.p, sizeof(unsigned long long) * TU * nc, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(ctl.data(), c->d_ctl.p, sizeof(unsigned long long) * 16, hipMemcpyDeviceToHost));
+    // k_summary wrote the sums into pinned host memory before the stream drained
+    const volatile unsigned long long *h = c->h_sum;
+    if (h[7] != 1ull) return set_err(c, MCS_ERR_HIP, "internal: walk summary not written");
+    unsigned long long ctl[2] = {h[4], h[5]};
     mcs_stats s{};
-    for (int k = 0; k < nc; ++k) {
-        s.photons += u[(size_t)k * TU + U_N];
-        s.steps += u[(size_t)k * TU + U_STEPS];
-        s.capped += u[(size_t)k * TU + U_CAP];
-        s.phi_unbound += u[(size_t)k * TU + U_AXIS];
-    }
+    s.photons = h[0];
+    s.steps = h[1];
+    s.capped = h[2];
+    s.phi_unbound = h[3];
     s.kernel_ms = ms;
     c->stats = s;
     if (ctl[1])
