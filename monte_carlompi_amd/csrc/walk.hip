@@ -34,6 +34,7 @@
 namespace mcs {
 
 constexpr int SLOTS = 4;
+static_assert((SLOTS & (SLOTS - 1)) == 0, "SLOTS is a power of two");
 constexpr int TF = MCS_TALLY_F64;
 constexpr int TU = MCS_TALLY_U64;
 constexpr double PI_D = 3.141592653589793;
@@ -633,48 +634,73 @@ __device__ __forceinline__ float dppf(float old, float v)
                                                       0xF, false));
 }
 
-template <int CTRL, int ROWS>
+// One scan level.  Lanes without a source (outside the row, or rows not in
+// ROWS) read the identity, so they keep q; `take` is needed only where the
+// source lane is in the row but in another group (groups under 16 lanes).
+template <int CTRL, int ROWS, bool SEL>
 __device__ __forceinline__ void qlevel(Quat &q, bool take)
 {
     const Quat y{dppf<CTRL, ROWS>(1.f, q.w), dppf<CTRL, ROWS>(0.f, q.x), dppf<CTRL, ROWS>(0.f, q.y),
                  dppf<CTRL, ROWS>(0.f, q.z)};
     const Quat m = qmul(y, q);
-    q.w = take ? m.w : q.w;
-    q.x = take ? m.x : q.x;
-    q.y = take ? m.y : q.y;
-    q.z = take ? m.z : q.z;
+    if constexpr (SEL) {
+        q.w = take ? m.w : q.w;
+        q.x = take ? m.x : q.x;
+        q.y = take ? m.y : q.y;
+        q.z = take ? m.z : q.z;
+    } else {
+        q = m;
+    }
 }
 
-template <int CTRL, int ROWS>
+template <int CTRL, int ROWS, bool SEL>
 __device__ __forceinline__ void slevel(float &s0, float &s1, bool take)
 {
     const float t0 = dppf<CTRL, ROWS>(0.f, s0), t1 = dppf<CTRL, ROWS>(0.f, s1);
-    s0 = take ? s0 + t0 : s0;
-    s1 = take ? s1 + t1 : s1;
+    if constexpr (SEL) {
+        s0 = take ? s0 + t0 : s0;
+        s1 = take ? s1 + t1 : s1;
+    } else {
+        s0 += t0;
+        s1 += t1;
+    }
 }
 
 // Inclusive scans within groups of G = 2^lg lanes (jj = lane within group):
 // q <- q_0 (x) q_1 (x) ... (x) q_jj.  Levels 1..8 stay inside 16-lane rows;
 // row_bcast:15 then joins row pairs (rows 1, 3) and row_bcast:31 the halves.
-__device__ __forceinline__ void group_scan_q(Quat &q, int lg, int jj, int lane)
+__device__ __forceinline__ void group_scan_q(Quat &q, int lg, int jj)
 {
-    if (lg >= 1) qlevel<0x111, 0xF>(q, jj >= 1);
-    if (lg >= 2) qlevel<0x112, 0xF>(q, jj >= 2);
-    if (lg >= 3) qlevel<0x114, 0xF>(q, jj >= 4);
-    if (lg >= 4) qlevel<0x118, 0xF>(q, jj >= 8);
-    if (lg >= 5) qlevel<0x142, 0xA>(q, (lane & 16) != 0);
-    if (lg >= 6) qlevel<0x143, 0xC>(q, lane >= 32);
+    if (lg < 4) {
+        if (lg >= 1) qlevel<0x111, 0xF, true>(q, jj >= 1);
+        if (lg >= 2) qlevel<0x112, 0xF, true>(q, jj >= 2);
+        if (lg >= 3) qlevel<0x114, 0xF, true>(q, jj >= 4);
+    } else {
+        // groups of whole rows: row scans, then row pairs and halves joined
+        qlevel<0x111, 0xF, false>(q, true);
+        qlevel<0x112, 0xF, false>(q, true);
+        qlevel<0x114, 0xF, false>(q, true);
+        qlevel<0x118, 0xF, false>(q, true);
+        if (lg >= 5) qlevel<0x142, 0xA, false>(q, true);
+        if (lg >= 6) qlevel<0x143, 0xC, false>(q, true);
+    }
 }
 
 // the same for two prefix sums
-__device__ __forceinline__ void group_scan_s(float &s0, float &s1, int lg, int jj, int lane)
+__device__ __forceinline__ void group_scan_s(float &s0, float &s1, int lg, int jj)
 {
-    if (lg >= 1) slevel<0x111, 0xF>(s0, s1, jj >= 1);
-    if (lg >= 2) slevel<0x112, 0xF>(s0, s1, jj >= 2);
-    if (lg >= 3) slevel<0x114, 0xF>(s0, s1, jj >= 4);
-    if (lg >= 4) slevel<0x118, 0xF>(s0, s1, jj >= 8);
-    if (lg >= 5) slevel<0x142, 0xA>(s0, s1, (lane & 16) != 0);
-    if (lg >= 6) slevel<0x143, 0xC>(s0, s1, lane >= 32);
+    if (lg < 4) {
+        if (lg >= 1) slevel<0x111, 0xF, true>(s0, s1, jj >= 1);
+        if (lg >= 2) slevel<0x112, 0xF, true>(s0, s1, jj >= 2);
+        if (lg >= 3) slevel<0x114, 0xF, true>(s0, s1, jj >= 4);
+    } else {
+        slevel<0x111, 0xF, false>(s0, s1, true);
+        slevel<0x112, 0xF, false>(s0, s1, true);
+        slevel<0x114, 0xF, false>(s0, s1, true);
+        slevel<0x118, 0xF, false>(s0, s1, true);
+        if (lg >= 5) slevel<0x142, 0xA, false>(s0, s1, true);
+        if (lg >= 6) slevel<0x143, 0xC, false>(s0, s1, true);
+    }
 }
 
 __device__ __forceinline__ float bperm(int src_lane, float v)
@@ -802,10 +828,10 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     Quat q{ha * hc, -hb * hs, ha * hs, hb * hc};
     if (jj == 0) q = qmul(cg.f, q);   // the group's carried frame on the left
     // frames first: the depth change of step k needs its direction
-    group_scan_q(q, lg, jj, lane);
+    group_scan_q(q, lg, jj);
     const float uz = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
     float sz = dtau * uz, sp = dtau;
-    group_scan_s(sz, sp, lg, jj, lane);
+    group_scan_s(sz, sp, lg, jj);
     const float zk = cg.z0 + sz, pk = cg.tau0 + sp;
     const bool top = zk > 0.f && uz > 0.f;
     const bool bot = zk < -ctau && uz < 0.f;
@@ -1008,12 +1034,17 @@ __device__ bool coop_receive(BlockState &bs, const KParams &P, NativeWalker &w, 
 }
 
 // ------------------------------------------------------------ chunk queue
-// Take chunk `seq` of this block into slot seq % SLOTS (whole wave calls).
+// The block's current chunk: BlockState::cur = (seq << 32) | (slot << 28) |
+// next photon offset (chunks are at most 2^24 photons, so hand-outs past the
+// end stay below 2^28).
+constexpr int CUR_SLOT_SHIFT = 28;
+constexpr uint32_t CUR_OFF_MASK = (1u << CUR_SLOT_SHIFT) - 1u;
+
+// Take the block's chunk number `seq` into free slot `slot` (whole wave calls).
 // Returns false when the global queue is empty.
-__device__ bool acquire_chunk(BlockState &bs, uint32_t seq, const KParams &P)
+__device__ bool acquire_chunk(BlockState &bs, uint32_t seq, int slot, const KParams &P)
 {
     const int lane = lane_id();
-    const int slot = seq % SLOTS;
     unsigned int j = 0;
     if (lane == 0) j = atomicAdd(P.chunk_counter, 1u);
     j = (unsigned int)bcast((int)j, 0);
@@ -1157,8 +1188,9 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
         if (n == 0) break;
         const unsigned long long cur = lds_load(&bs.cur);
         uint32_t seq = (uint32_t)(cur >> 32);
-        uint32_t off = (uint32_t)cur;
-        unsigned long long meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+        int cslot = (int)((uint32_t)cur >> CUR_SLOT_SHIFT) & (SLOTS - 1);
+        uint32_t off = (uint32_t)cur & CUR_OFF_MASK;
+        unsigned long long meta = lds_load(&bs.hdr[cslot].seqcount);
         uint32_t count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
         int got = 0;
         if (off < count) {
@@ -1166,13 +1198,14 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
             if (lane == 0) old = atomicAdd(&bs.cur, (unsigned long long)n);
             old = bcast64(old, 0);
             seq = (uint32_t)(old >> 32);
-            off = (uint32_t)old;
-            meta = lds_load(&bs.hdr[seq % SLOTS].seqcount);
+            cslot = (int)((uint32_t)old >> CUR_SLOT_SHIFT) & (SLOTS - 1);
+            off = (uint32_t)old & CUR_OFF_MASK;
+            meta = lds_load(&bs.hdr[cslot].seqcount);
             count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
             got = off < count ? (int)min((uint32_t)n, count - off) : 0;
             const int rk = lane_rank(idle);
             if (state == EMPTY && rk < got) {
-                const int s = seq % SLOTS;
+                const int s = cslot;
                 const SlotHdr &H = bs.hdr[s];
                 const uint32_t photon = H.p0 + off + rk;
                 my_gid = H.gid0 + off + rk;
@@ -1190,13 +1223,18 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
         const unsigned long long now = lds_load(&bs.cur);
         const uint32_t seq_now = (uint32_t)(now >> 32);
         if (seq_now == seq && !lds_load(&bs.exhausted)) {
-            const int ns = (seq_now + 1) % SLOTS;
-            if (lds_load(&bs.hdr[ns].state) == 0) {
-                if (acquire_chunk(bs, seq_now + 1, P)) {
-                    if (lane == 0)
-                        __atomic_store_n(&bs.cur, (unsigned long long)(seq_now + 1) << 32, __ATOMIC_RELAXED);
-                    advanced = true;
-                }
+            // any free slot: a chunk whose last photons still walk holds only its own
+            int ns = -1;
+            for (int d = 1; d <= SLOTS && ns < 0; ++d) {
+                const int s = (cslot + d) & (SLOTS - 1);
+                if (lds_load(&bs.hdr[s].state) == 0) ns = s;
+            }
+            if (ns >= 0 && acquire_chunk(bs, seq_now + 1, ns, P)) {
+                if (lane == 0)
+                    __atomic_store_n(&bs.cur,
+                                     ((unsigned long long)(seq_now + 1) << 32) | ((unsigned long long)ns << CUR_SLOT_SHIFT),
+                                     __ATOMIC_RELAXED);
+                advanced = true;
             }
         } else if (seq_now != seq) {
             advanced = true;  // someone else advanced: retry the handout
@@ -1240,7 +1278,7 @@ k_walk(const KParams P)
     __syncthreads();
     if (threadIdx.x == 0) bs.holders = blockDim.x >> 6;
     if (wave == 0) {
-        if (!acquire_chunk(bs, 0u, P)) {
+        if (!acquire_chunk(bs, 0u, 0, P)) {
             if (lane == 0) bs.hdr[0].seqcount = 0ull;  // seq 0, count 0
         }
     }
@@ -1657,7 +1695,7 @@ int mcs_set_launch(mcs_ctx *c, const mcs_launch *l)
 {
     if (!c || !l) return MCS_ERR_ARG;
     if (l->block < 64 || l->block > 1024 || l->block % 64 || l->blocks_per_cu < 1 || l->blocks_per_cu > 64 ||
-        l->chunk < 64 || l->refill_min < 1 || l->refill_min > 64 || l->coop_live < 0 || l->coop_live > 32)
+        l->chunk < 64 || l->chunk > (1 << 24) || l->refill_min < 1 || l->refill_min > 64 || l->coop_live < 0 || l->coop_live > 32)
         return set_err(c, MCS_ERR_ARG, "launch shape out of range");
     c->launch = *l;
     c->gkey = 0;

@@ -71,12 +71,18 @@ def dpp(old, v, ctrl, rows):
 
 
 def group_scan_q(q, lg):
-    """The schedule of walk.hip's group_scan_q on a (64, 4) wave."""
+    """The schedule of walk.hip's group_scan_q on a (64, 4) wave: groups under
+    16 lanes select per level; groups of whole rows take every level as is
+    (lanes without a DPP source read the identity)."""
     lanes = np.arange(64)
     jj = lanes & ((1 << lg) - 1)
     ident = np.tile([1.0, 0.0, 0.0, 0.0], (64, 1))
-    levels = [(1, 0x111, 0xF, jj >= 1), (2, 0x112, 0xF, jj >= 2), (3, 0x114, 0xF, jj >= 4),
-              (4, 0x118, 0xF, jj >= 8), (5, 0x142, 0xA, (lanes & 16) != 0), (6, 0x143, 0xC, lanes >= 32)]
+    if lg < 4:
+        levels = [(1, 0x111, 0xF, jj >= 1), (2, 0x112, 0xF, jj >= 2), (3, 0x114, 0xF, jj >= 4)]
+    else:
+        every = np.ones(64, bool)
+        levels = [(1, 0x111, 0xF, every), (1, 0x112, 0xF, every), (1, 0x114, 0xF, every),
+                  (1, 0x118, 0xF, every), (5, 0x142, 0xA, every), (6, 0x143, 0xC, every)]
     for need, ctrl, rows, take in levels:
         if lg >= need:
             y = dpp(ident, q, ctrl, rows)

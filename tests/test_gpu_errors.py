@@ -54,6 +54,8 @@ def test_call_order_and_arguments(walker):
         w2.set_launch(block=100)                   # not a multiple of 64
     with pytest.raises(McsError, match='ARG'):
         w2.set_launch(coop_live=33)                # tail mode serves at most 32 photons
+    with pytest.raises(McsError, match='ARG'):
+        w2.set_launch(chunk=(1 << 24) + 1)         # chunks hold at most 2^24 photons
     w2.close()
 
 

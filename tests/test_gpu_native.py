@@ -232,3 +232,4 @@ def test_reference_known_answer_through_drop_in(tmp_path, monkeypatch):
     assert abs(T - 0.66096) < 3 * np.sqrt(0.66096 * 0.33904 / n) + 2e-5
     assert abs(mc.calculate_albedo(rec) - R) < 1e-12
     mc.close()
+
