@@ -595,19 +595,22 @@ enum { EMPTY = 0, LIVE = 1, DONE = 2 };
 // Once a block's queue is empty, a wave's last photons would walk one step
 // per wave instruction stream (~1,100 cycles per step for a lone wave), and
 // the longest walks of a launch (~1e4 steps) set its duration.  Here the
-// G = 64 / L lanes of a group walk G consecutive steps of one of the L live
+// G = 64 / L lanes of a group walk 2G consecutive steps of one of the L live
 // photons at once:
-//  * lane j of the group draws the Philox block of step k = i + 1 + j -- the
-//    same counter the one-lane walk uses, so every step keeps its uniforms;
+//  * lane j of the group draws the Philox blocks of steps i + 1 + 2j and
+//    i + 2 + 2j -- the counters the one-lane walk uses, so every step keeps
+//    its uniforms;
 //  * a step is "scatter by (theta_k, phi_k) about the current direction, then
 //    move dtau_k": with the direction carried as the third column of a frame
-//    quaternion F, F_k = F_{k-1} (x) L_k (L_k = Rz(phi_k) Ry(theta_k)), so all
-//    G frames are one inclusive product scan (DPP row shifts + row
-//    broadcasts, lower lanes on the left); depth z_k and optical path are
-//    prefix sums of dtau_k * uz_k and dtau_k;
-//  * the first lane whose step exits or absorbs (or reaches max_steps) ends
-//    the segment; the photon's lane takes that lane's state, otherwise the
-//    last lane's, and continues.
+//    quaternion F, F_k = F_{k-1} (x) L_k (L_k = Rz(phi_k) Ry(theta_k)).  Each
+//    lane multiplies its two steps' L, one inclusive product scan over the
+//    group (DPP row shifts + row broadcasts, lower lanes on the left) gives
+//    the frame after each lane's second step, and the frame after its first
+//    is that one times the conjugate of its second L; depth and optical path
+//    are one prefix sum of both steps' dtau_k * uz_k and dtau_k;
+//  * the first step (in walk order) that exits or absorbs (or reaches
+//    max_steps) ends the segment; the photon's lane takes that step's state,
+//    otherwise the last lane's, and continues.
 // Azimuths are measured in the transported frame instead of the lab-axis
 // frame of mc3D:1255-1281.  phi_k is uniform and independent of the history,
 // so the walk has the same law; events use the uniforms of their own step,
@@ -778,6 +781,23 @@ __device__ __forceinline__ void coop_handback(NativeWalker &w, const CoopGroups 
     }
 }
 
+// A step's free path and local rotation L = Rz(phi) Ry(theta) (from the half
+// angles) from its Philox block, with the HG constants of the photon (the
+// same draws as NativeWalker::step).
+__device__ __forceinline__ void coop_local(const u32x4 &cur, float ca, float cb, float ccc, float cd, float ce,
+                                           float &dtau, Quat &L)
+{
+    dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
+    const float u0 = u01f(cur.x);
+    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, u0, cd));
+    const float cost = (ce == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
+    const float hc = __builtin_amdgcn_sqrtf(fmaxf(fmaf(0.5f, cost, 0.5f), 0.f));   // cos(theta/2)
+    const float hs = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-0.5f, cost, 0.5f), 0.f));  // sin(theta/2)
+    const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
+    const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
+    L = Quat{ha * hc, -hb * hs, ha * hs, hb * hc};
+}
+
 // One cooperative round of a wave (all 64 lanes call; no divergence around
 // the lane permutes).  `tab`: the wave's entries; `map`: group -> entry;
 // `me`: this lane's photon's entry.
@@ -814,50 +834,60 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     const CoopEntry &E = tab[cg.ent];
     const float ca = E.a, cb = E.b, ccc = E.cc, cd = E.d, ce = E.e, ctau = E.tau_tot;
     const uint32_t ct_abs = E.t_abs;
-    const uint32_t k = cg.i0 + 1u + (uint32_t)jj;
-    // step k's uniforms and local rotation (same draws as NativeWalker::step)
-    const u32x4 cur = philox_step(cg.pre, k, P.key0, P.key1);
-    const float dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
-    const float u0 = u01f(cur.x);
-    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, u0, cd));
-    const float cost = (ce == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
-    const float hc = __builtin_amdgcn_sqrtf(fmaxf(fmaf(0.5f, cost, 0.5f), 0.f));   // cos(theta/2)
-    const float hs = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-0.5f, cost, 0.5f), 0.f));  // sin(theta/2)
-    const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
-    const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
-    Quat q{ha * hc, -hb * hs, ha * hs, hb * hc};
+    // lane jj walks steps ka = i0 + 1 + 2 jj and kb = ka + 1 (the same Philox
+    // blocks the one-lane walk draws for them)
+    const uint32_t ka = cg.i0 + 1u + 2u * (uint32_t)jj, kb = ka + 1u;
+    const u32x4 ua = philox_step(cg.pre, ka, P.key0, P.key1);
+    const u32x4 ub = philox_step(cg.pre, kb, P.key0, P.key1);
+    float dta, dtb;
+    Quat La, Lb;
+    coop_local(ua, ca, cb, ccc, cd, ce, dta, La);
+    coop_local(ub, ca, cb, ccc, cd, ce, dtb, Lb);
+    Quat q = qmul(La, Lb);
     if (jj == 0) q = qmul(cg.f, q);   // the group's carried frame on the left
-    // frames first: the depth change of step k needs its direction
+    // frames first: the depth changes need the directions
     group_scan_q(q, lg, jj);
-    const float uz = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
-    float sz = dtau * uz, sp = dtau;
+    // frame after step ka: F_b (x) conj(L_b)
+    const Quat qa = qmul(q, Quat{Lb.w, -Lb.x, -Lb.y, -Lb.z});
+    const float uza = fmaf(-2.f, fmaf(qa.x, qa.x, qa.y * qa.y), 1.f);
+    const float uzb = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
+    const float dzb = dtb * uzb;
+    float sz = fmaf(dta, uza, dzb), sp = dta + dtb;
     group_scan_s(sz, sp, lg, jj);
-    const float zk = cg.z0 + sz, pk = cg.tau0 + sp;
-    const bool top = zk > 0.f && uz > 0.f;
-    const bool bot = zk < -ctau && uz < 0.f;
-    const bool ab = cur.w < ct_abs;
-    const bool cap = k >= P.max_steps;
-    const unsigned long long ev = __ballot(member && (top || bot || ab || cap));
+    const float zb = cg.z0 + sz, pb = cg.tau0 + sp;
+    const float za = zb - dzb, pa = pb - dtb;
+    const bool top_a = za > 0.f && uza > 0.f, bot_a = za < -ctau && uza < 0.f;
+    const bool top_b = zb > 0.f && uzb > 0.f, bot_b = zb < -ctau && uzb < 0.f;
+    const bool eva = member && (top_a || bot_a || ua.w < ct_abs || ka >= P.max_steps);
+    const bool evb = member && (top_b || bot_b || ub.w < ct_abs || kb >= P.max_steps);
+    const unsigned long long ev = __ballot(eva || evb);
     if (ev == 0ull) {
-        // no event: every group walked G steps; carry its last lane's state
+        // no event: every group walked 2G steps; carry its last lane's state
         // (the frame renormalised against drift)
         Quat f;
         if (lg == 6) {
-            cg.z0 = rdlane63(zk); cg.tau0 = rdlane63(pk);
+            cg.z0 = rdlane63(zb); cg.tau0 = rdlane63(pb);
             f = Quat{rdlane63(q.w), rdlane63(q.x), rdlane63(q.y), rdlane63(q.z)};
         } else {
             const int src = lane | (G - 1);
-            cg.z0 = bperm(src, zk); cg.tau0 = bperm(src, pk);
+            cg.z0 = bperm(src, zb); cg.tau0 = bperm(src, pb);
             f = Quat{bperm(src, q.w), bperm(src, q.x), bperm(src, q.y), bperm(src, q.z)};
         }
         const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
         cg.f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
-        cg.i0 += (uint32_t)G;
+        cg.i0 += 2u * (uint32_t)G;
         return;
     }
-    const float ox = 2.f * fmaf(q.x, q.z, q.w * q.y);
-    const float oy = 2.f * fmaf(q.y, q.z, -q.w * q.x);
-    // this lane's outcome, should its step end the segment
+    // this lane's outcome, should one of its steps end the segment: its first
+    // step if that one has an event, else its second
+    const bool A = eva;
+    const bool top = A ? top_a : top_b, bot = A ? bot_a : bot_b;
+    const uint32_t wv = A ? ua.w : ub.w, k = A ? ka : kb;
+    const bool ab = wv < ct_abs, cap = k >= P.max_steps;
+    const float zk = A ? za : zb, pk = A ? pa : pb, dtau = A ? dta : dtb, uz = A ? uza : uzb;
+    const Quat qe{A ? qa.w : q.w, A ? qa.x : q.x, A ? qa.y : q.y, A ? qa.z : q.z};
+    const float ox = 2.f * fmaf(qe.x, qe.z, qe.w * qe.y);
+    const float oy = 2.f * fmaf(qe.y, qe.z, -qe.w * qe.x);
     const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
     const float corr = top ? zk : zk + ctau;
     const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
@@ -871,7 +901,7 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     }
     const int r_type = (int)bperm(src, (uint32_t)type);
     const uint32_t r_k = bperm(src, k);
-    const uint32_t r_w = bperm(src, cur.w);
+    const uint32_t r_w = bperm(src, wv);
     const float r_z = bperm(src, zout), r_p = bperm(src, pout);
     const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uz);
     cg.em = 0ull;   // photons' lanes hold the state again: regroup next round

@@ -170,3 +170,44 @@ def test_transported_frame_walk_has_the_lab_frame_law():
     assert stats.ks_2samp(bad[:, 2], wrong[:, 2]).pvalue > 1e-3   # one step: same law
     swapped = frame_dir(qmul(local_quat(hg_cos(rb.uniform(size=n), g), rb.uniform(size=n)), frame_of(d0)))
     assert stats.ks_2samp(swapped[:, 2], wrong[:, 2]).pvalue < 1e-6  # L on the left: wrong law
+
+
+def test_two_steps_per_lane_give_every_frame_and_depth():
+    """Lane j of a group walks steps 2j+1 and 2j+2: the scan runs over
+    Q_j = L_a(j) (x) L_b(j) (the carried frame f on lane 0's left), the frame
+    after step 2j+2 is the inclusive product and the frame after step 2j+1 is
+    that product (x) conj(L_b(j)); depths follow from one prefix sum of
+    dtau_a uz_a + dtau_b uz_b.  Checked against the sequential walk for every
+    group size."""
+    rng = np.random.default_rng(4)
+    conj = lambda q: q * np.array([1.0, -1.0, -1.0, -1.0])
+    for lg in range(0, 7):
+        G = 1 << lg
+        d0 = rng.normal(size=3)
+        d0 /= np.linalg.norm(d0)
+        f = frame_of(d0[None])[0]
+        cost, rev = rng.uniform(-1, 1, (2, 64)), rng.uniform(0, 1, (2, 64))
+        La, Lb = local_quat(cost[0], rev[0]), local_quat(cost[1], rev[1])
+        dta, dtb = rng.exponential(size=64), rng.exponential(size=64)
+        lanes = np.arange(64)
+        Q = qmul(La, Lb)
+        Q[(lanes & (G - 1)) == 0] = qmul(np.broadcast_to(f, (64, 4)), Q)[(lanes & (G - 1)) == 0]
+        Fb = group_scan_q(Q.copy(), lg) if lg > 0 else Q
+        Fa = qmul(Fb, conj(Lb))
+        uza, uzb = frame_dir(Fa)[:, 2], frame_dir(Fb)[:, 2]
+        z0 = -1.5
+        for g0 in range(0, 64, G):
+            F = f.copy()
+            z = z0
+            for j in range(g0, g0 + G):
+                F = qmul(F, La[j])
+                np.testing.assert_allclose(Fa[j], F, atol=1e-12)
+                z += dta[j] * frame_dir(F)[2]
+                za = z
+                F = qmul(F, Lb[j])
+                np.testing.assert_allclose(Fb[j], F, atol=1e-12)
+                z += dtb[j] * frame_dir(F)[2]
+                # the kernel's depths: prefix sum of both steps, first step by difference
+                zb_k = z0 + np.sum(dta[g0:j + 1] * uza[g0:j + 1] + dtb[g0:j + 1] * uzb[g0:j + 1])
+                np.testing.assert_allclose(zb_k, z, atol=1e-10)
+                np.testing.assert_allclose(zb_k - dtb[j] * uzb[j], za, atol=1e-10)
