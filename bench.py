@@ -24,10 +24,16 @@ METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon 
 # (block, blocks_per_cu, chunk, refill_min, coop_live): small chunks spread the
 # 1e6 photons over every block; the cooperative tail mode finishes the walks
 # still running when the queue is empty, and 16-wave blocks spread those
-# photons over the waves of the block that run out of work
-SNOW_1M_CANDIDATES = [(1024, 1, 2048, 16, 24), (1024, 1, 2048, 16, 20), (1024, 1, 2048, 16, 32),
-                      (1024, 1, 2048, 16, 16), (512, 2, 1024, 16, 24), (512, 2, 1024, 16, 16),
-                      (256, 5, 512, 16, 16), (512, 3, 1024, 16, 16)]
+# photons over the waves of the block that run out of work.  Chunks sized for
+# exactly two per 1024-thread block keep any block from ending the queue a
+# whole chunk behind.
+def snow_1m_candidates(n_photons, cus):
+    from monte_carlompi_amd.gpu import balanced_chunk
+    b1 = balanced_chunk(n_photons, 1, cus)
+    return [(1024, 1, b1, 16, 32), (1024, 1, b1, 16, 24), (1024, 1, b1, 8, 32), (1024, 1, 2048, 16, 32),
+            (1024, 1, 2048, 16, 24), (512, 2, b1, 16, 32), (256, 5, 512, 16, 16), (512, 3, 1024, 16, 16)]
+
+
 N_PHOTONS = 1_000_000
 
 
@@ -59,11 +65,12 @@ def main():
     w = GpuWalker(0)
     w.set_cases([snow_1m_case(args.photons)])
     # the project's autotuner picks the launch shape for this workload (untimed)
+    tune_ms = None
     if args.launch:
         shape = tuple(int(v) for v in args.launch.split(','))
         w.set_launch(*shape)
     else:
-        shape, tune_ms = w.autotune(SNOW_1M_CANDIDATES, photons_cap=0, repeats=3, seed=99)
+        shape, tune_ms = w.autotune(snow_1m_candidates(args.photons, w.cus), photons_cap=0, repeats=9, seed=99)
     for k in range(args.warmup):
         w.walk(mode='native', seed=10_000 + k, records=True)
     torch.cuda.synchronize()
@@ -112,6 +119,8 @@ def main():
         'pcie_inclusive_photons_per_s': round(args.photons / (pcie_ms * 1e-3), 1),
         'kernel_ms_avg': round(avg_kern, 4),
         'launch': list(shape),
+        'autotune_ms': ({str(tuple(c)): round(float(t), 4) for c, t in zip(snow_1m_candidates(args.photons, w.cus), tune_ms)}
+                        if tune_ms is not None else None),
         'coop_steps_frac': round(coop_frac, 4),
         'photons_handed_over': handed_over,
         'albedo': round(float(fr['albedo'][0]), 6),

@@ -2080,8 +2080,10 @@ int mcs_autotune(mcs_ctx *c, const mcs_launch *cand, int32_t n, int64_t photons_
         a.mode = MCS_NATIVE;
         a.seed = seed;
         if ((r = mcs_walk(c, &a))) break;  // warm-up
+        // the same seeds for every candidate, a different walk for every repeat
         std::vector<double> t;
         for (int q = 0; q < repeats; ++q) {
+            a.seed = seed + 1 + (uint64_t)q;
             if ((r = mcs_walk(c, &a))) break;
             t.push_back(c->stats.kernel_ms);
         }

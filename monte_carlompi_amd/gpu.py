@@ -221,6 +221,15 @@ class GpuWalker:
         return out
 
 
+def balanced_chunk(n_photons, blocks_per_cu, cus, per_block=2):
+    """Chunk size that splits n_photons into per_block chunks for every block
+    of a blocks_per_cu x cus grid (the queue then ends with no block a whole
+    chunk behind the others).  A block keeps four chunks in flight, so chunks
+    should hold at least twice the block's lanes; the caller picks per_block
+    accordingly."""
+    return max(64, -(-int(n_photons) // (int(per_block) * int(blocks_per_cu) * int(cus))))
+
+
 def brf_from_tallies(t):
     """18 x 36 BRF grid of each case from its tallies (the 2-D form of
     post_processing.brf, pp:73-81): BRF_jk = H_jk / (Q_down w_jk) with

@@ -109,3 +109,24 @@ def test_unsupported_reference_modes_raise():
         mc.run(10, 1.3, 0.085, 100, shape='droxtal')
     with pytest.raises(NotImplementedError):
         mc.run(10, 1.3, 0.085, 100, debug=True)
+
+
+def test_balanced_chunk_gives_every_block_the_same_number_of_chunks():
+    from monte_carlompi_amd.gpu import balanced_chunk
+    for n, bpc, cus, per in [(1_000_000, 1, 256, 2), (1_000_000, 2, 256, 2), (12345, 1, 256, 2), (10_000_000, 1, 256, 4)]:
+        c = balanced_chunk(n, bpc, cus, per)
+        chunks = -(-n // c)
+        assert chunks <= per * bpc * cus                    # no block gets an extra chunk
+        assert c == 64 or -(-n // (c - 1)) > per * bpc * cus  # and the chunk is the smallest that does it
+    assert balanced_chunk(1_000_000, 1, 256) == 1954
+
+
+def test_bench_cli_parses_and_refuses_several_gpus():
+    import subprocess, sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--help'], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0 and '--launch' in r.stdout
+    r = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and 'one GPU' in (r.stdout + r.stderr)
