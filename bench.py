@@ -151,6 +151,15 @@ def main():
                                                          cb['steps_per_s'], cpu_baseline.cpu_model()),
         }
         out['gpu_over_cpu'] = round(photons_per_s / cb['photons_per_s'], 1)
+        # the same 1e6-photon case walked by the compiled C restatement (FP64)
+        cc = cpu_baseline.run_c(n_photon=args.photons, threads=args.cpu_procs or None, seed=1)
+        out['cpu_baseline']['c_restatement'] = {
+            'value': round(cc['photons_per_s'], 1), 'unit': 'photons/s', 'cores': cc['threads'],
+            'sample': '%d photons of the same case, %d host threads each walking its share with '
+                      'oracle/mcoracle.c (FP64, gcc -O2); walk phase %.2f s, %.3g photon-steps/s'
+                      % (cc['photons'], cc['threads'], cc['walk_s'], cc['steps_per_s']),
+        }
+        out['gpu_over_cpu_c'] = round(photons_per_s / cc['photons_per_s'], 1)
     print(json.dumps(out), flush=True)
 
 

@@ -67,6 +67,38 @@ def run(n_photon=10_000, processes=None, seed=0, wvl0=1.3, fwhm=1e-12, ssa=0.999
                 photons_per_s=len(records) / walk_s, steps_per_s=steps / walk_s)
 
 
+def run_c(n_photon=200_000, threads=None, seed=0, wvl=1.3, ssa=0.999, g=0.89, r_eff=100.0,
+          rho_snw=300.0, rho_ice=917.0, depth_m=1.0, theta0=60.0, lb=True, R=0.5):
+    """The same case walked by the C restatement of monte_carlo3D (oracle/mcoracle.c,
+    FP64, numpy's MT19937 stream per rank): ``threads`` ranks of n_photon / threads
+    photons each, one host thread per rank (the GIL is released inside C).  A
+    compiled single-core-per-rank CPU line beside the reference-structured one."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import coracle
+    T = threads or min(16, len(os.sched_getaffinity(0)))
+    ext_cff_mss = walk.sphere_ext_cff_mss(r_eff, rho_ice)
+    tau_tot = depth_m * ext_cff_mss * rho_snw
+    per = -(-int(n_photon) // T)
+
+    def par(n):
+        return dict(wvl=np.full(n, wvl), g=np.full(n, g), ssa_ice=np.full(n, ssa), ssa_imp=np.zeros(n),
+                    ext_cff_mss=np.full(n, ext_cff_mss), p_ext_imp=np.zeros(n),
+                    snow_depth=np.full(n, tau_tot / (ext_cff_mss * rho_snw)))
+
+    def one(r):
+        mt = coracle.mt_from_numpy(np.random.RandomState(seed + r))
+        t0 = time.perf_counter()
+        out = coracle.walk(par(per), tau_tot, rho_snw, theta0, lb, R, mt=mt)
+        return time.perf_counter() - t0, int(out['steps'])
+
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        parts = list(ex.map(one, range(T)))
+    walk_s = max(t for t, _ in parts)
+    steps = sum(k for _, k in parts)
+    return dict(threads=T, photons=per * T, steps=steps, walk_s=walk_s,
+                photons_per_s=per * T / walk_s, steps_per_s=steps / walk_s)
+
+
 def cpu_model():
     try:
         with open('/proc/cpuinfo') as fh:
