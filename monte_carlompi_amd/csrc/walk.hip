@@ -59,7 +59,10 @@ struct DevCase {
 
 struct DevBand {
     double wvn, g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, cdf;
-    float hg_a, hg_b, hg_c, hg_d, hg_e;  // 1/(2g), 1+g^2, 1-g^2, 1-g, 2g
+    // HG inverse CDF from the 32 Philox bits x: t = hg_c / (hg_e x + hg_d),
+    // cos = hg_a (hg_b - t^2), with hg_e = 2g 2^-32 and hg_d = 1 - g + 2g 2^-33
+    // (u = (x + 1/2) 2^-32 folded in); isotropic bands: hg_e = 0, hg_d = 1
+    float hg_a, hg_b, hg_c, hg_d, hg_e;
     float inv_ext;                       // 1/ext_cff [m]
     uint32_t t_abs5;                     // bits < P_imp (1 - ssa_imp) 2^32: absorbed by impurity
     uint32_t t_abs_all;                  // bits < [P_imp (1 - ssa_imp) + (1 - P_imp)(1 - ssa_ice)] 2^32: absorbed
@@ -354,30 +357,33 @@ struct NativeWalker {
         const u32x4 cur = r;
         r = philox_step(pre, i + 1u, P.key0, P.key1);
         const float dtau = free_path(cur);
-        // polar cosine: HG inverse CDF (mc3D:790-800), isotropic for g = 0
-        const float u0 = u01f(cur.x);
-        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, u0, d));
-        const float cost = iso ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        // polar cosine: HG inverse CDF (mc3D:790-800) with the uniform's
+        // scale and offset folded into d and e; isotropic for g = 0
+        const float xf = (float)cur.x;
+        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
+        const float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));   // |cost| <= 1
         // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
         const float rev = u01f(cur.y);
         const float sinp = __builtin_amdgcn_sinf(rev);
         const float cosp = __builtin_amdgcn_cosf(rev);
         // rotation (mc3D:1255-1281); near-vertical incoming directions take the
-        // axis form (rare: a branch, not a select)
+        // axis form, behind a wave-uniform branch so that the common path
+        // issues none of its instructions
         const float s2 = fmaf(-muz, muz, 1.f);
-        if (s2 < 1e-7f) {
-            const float sg = muz > 0.f ? 1.f : -1.f;
-            mux = sint * cosp;
-            muy = sg * sint * sinp;
-            muz = sg * cost;
-        } else {
-            const float st = sint * __builtin_amdgcn_rsqf(s2);
-            const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-            const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-            const float gz = fmaf(-s2 * st, cosp, muz * cost);
-            mux = gx; muy = gy; muz = gz;
+        const float st = sint * __builtin_amdgcn_rsqf(s2);
+        float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        float gz = fmaf(-s2 * st, cosp, muz * cost);
+        if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
+            if (s2 < 1e-7f) {
+                const float sg = muz > 0.f ? 1.f : -1.f;
+                gx = sint * cosp;
+                gy = sg * sint * sinp;
+                gz = sg * cost;
+            }
         }
+        mux = gx; muy = gy; muz = gz;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
         const bool top = z > 0.f;
@@ -788,11 +794,12 @@ __device__ __forceinline__ void coop_local(const u32x4 &cur, float ca, float cb,
                                            float &dtau, Quat &L)
 {
     dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
-    const float u0 = u01f(cur.x);
-    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, u0, cd));
-    const float cost = (ce == 0.f) ? fmaf(-2.f, u0, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
-    const float hc = __builtin_amdgcn_sqrtf(fmaxf(fmaf(0.5f, cost, 0.5f), 0.f));   // cos(theta/2)
-    const float hs = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-0.5f, cost, 0.5f), 0.f));  // sin(theta/2)
+    const float xf = (float)cur.x;
+    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, xf, cd));
+    // isotropic bands carry e = 0
+    const float cost = (ce == 0.f) ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
+    const float hc = __builtin_amdgcn_sqrtf(fmaf(0.5f, cost, 0.5f));    // cos(theta/2); |cost| <= 1
+    const float hs = __builtin_amdgcn_sqrtf(fmaf(-0.5f, cost, 0.5f));   // sin(theta/2)
     const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
     const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
     L = Quat{ha * hc, -hb * hs, ha * hs, hb * hc};
@@ -1672,7 +1679,8 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.iso = (B.g == 0.0);
         const double g = D.iso ? 0.5 : B.g;
         D.hg_a = (float)(1.0 / (2.0 * g)); D.hg_b = (float)(1.0 + g * g); D.hg_c = (float)(1.0 - g * g);
-        D.hg_d = (float)(1.0 - g); D.hg_e = (float)(2.0 * g);
+        D.hg_d = D.iso ? 1.f : (float)(1.0 - g + 2.0 * g * 0x1p-33);
+        D.hg_e = D.iso ? 0.f : (float)(2.0 * g * 0x1p-32);
         D.inv_ext = (float)(1.0 / B.ext_cff);
         const double q5 = B.p_ext_imp * (1.0 - B.ssa_imp);
         D.t_abs5 = prob_bits(q5);

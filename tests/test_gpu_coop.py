@@ -196,3 +196,31 @@ def test_hand_over_with_lambertian_bottom_and_step_cap(walker):
         assert set(np.unique(r['condition']).tolist()) <= {0, 1, 2, 3}   # no absorption at omega 1
     finally:
         walker.set_launch()
+
+
+def test_coop_isotropic_scattering(walker):
+    """g = 0 bands (isotropic, mc3D:790-800's g == 0 branch) in the cooperative
+    mode: exit fractions, BRF and step counts agree with the one-lane walk, and
+    the one-lane walk with the C oracle.  (A forward-peaked phase function in
+    place of the isotropic one moves the exit fractions by tens of sigma.)"""
+    tau = 20.0
+    case = lambda n: gpu_case(n, tau, 30.0, False, 0.0, g=0.0, ssa=0.999)
+    a, ca = coop_runs(walker, case, range(6), 32)
+    b, cb = coop_runs(walker, case, range(6, 12), 0)
+    assert ca[U_COOP] > 0.3 * ca[U_STEPS] and cb[U_COOP] == 0
+    n_a, n_b = a['condition'].size, b['condition'].size
+    for cond in (1, 2, 4):
+        z = two_sample_z(int(np.sum(a['condition'] == cond)), n_a, int(np.sum(b['condition'] == cond)), n_b)
+        assert abs(z) < 3.0, (cond, z)
+    ha = brf_counts(a['theta_n'][a['condition'] == 1], a['phi_n'][a['condition'] == 1])
+    hb = brf_counts(b['theta_n'][b['condition'] == 1], b['phi_n'][b['condition'] == 1])
+    keep = (ha + hb) >= 10
+    chi2 = np.sum((np.sqrt(n_b / n_a) * ha[keep] - np.sqrt(n_a / n_b) * hb[keep]) ** 2 / (ha[keep] + hb[keep]))
+    assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, chi2
+    assert stats.ks_2samp(a['n_scat'], b['n_scat']).pvalue > 1e-3
+    cpu = coracle.run_ranks(77, 8, 5_000, lambda n: flat_params(n, 1.3, 0.0, 0.999, EXT100, tau, 300.0),
+                            tau, 300.0, 30.0, False, 0.0)
+    n_c = cpu['condition'].size
+    for cond in (1, 2, 4):
+        z = two_sample_z(int(np.sum(b['condition'] == cond)), n_b, int(np.sum(cpu['condition'] == cond)), n_c)
+        assert abs(z) < 3.0, (cond, z)
