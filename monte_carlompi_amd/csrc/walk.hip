@@ -372,30 +372,32 @@ struct NativeWalker {
         // issues none of its instructions
         const float s2 = fmaf(-muz, muz, 1.f);
         const float st = sint * __builtin_amdgcn_rsqf(s2);
-        float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-        float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-        float gz = fmaf(-s2 * st, cosp, muz * cost);
+        const float mz0 = muz;
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        muz = fmaf(-s2 * st, cosp, muz * cost);
+        mux = gx; muy = gy;
         if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
             if (s2 < 1e-7f) {
-                const float sg = muz > 0.f ? 1.f : -1.f;
-                gx = sint * cosp;
-                gy = sg * sint * sinp;
-                gz = sg * cost;
+                const float sg = mz0 > 0.f ? 1.f : -1.f;
+                mux = sint * cosp;
+                muy = sg * sint * sinp;
+                muz = sg * cost;
             }
         }
-        mux = gx; muy = gy; muz = gz;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
-        const bool top = z > 0.f;
-        const bool bot = z < -tau_tot;
-        if (!(top || bot || cur.w < t_abs_all)) return false;
-        // ---- rare: exits and absorption
-        if (top) {                                   // mc3D:1390-1397
+        if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all)) return false;
+        // ---- rare: exits and absorption.  The depth passes an empty asm so
+        // that the tests below are issued here, not hoisted into every step.
+        float ze = z;
+        asm volatile("" : "+v"(ze));
+        if (ze > 0.f) {                              // mc3D:1390-1397
             tau_acc -= fminf(z / muz, dtau);
             cond = 1;
             return true;
         }
-        if (bot) {                                   // mc3D:1399-1459
+        if (ze < -tau_tot) {                         // mc3D:1399-1459
             tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
             if (lamb && reflects(P)) return reemit(P);
