@@ -224,3 +224,51 @@ def test_coop_isotropic_scattering(walker):
     for cond in (1, 2, 4):
         z = two_sample_z(int(np.sum(b['condition'] == cond)), n_b, int(np.sum(cpu['condition'] == cond)), n_c)
         assert abs(z) < 3.0, (cond, z)
+
+
+def test_hand_over_across_cases(walker):
+    """Four cases in one launch (thick and thin layers, an isotropic band,
+    impurity absorption, a two-band Lambertian-bottom case), 16-wave blocks
+    with about one chunk each: photons handed between waves find their own
+    case's chunk among slots of other cases.  Per case every photon is
+    recorded once, and the condition fractions agree with the one-lane walk
+    (3.5 sigma: 15 comparisons)."""
+    from monte_carlompi_amd.gpu import Band, Case
+    n = 256 * walker.cus // 2
+    cases = [
+        Case(tau_tot=30.0, theta0_deg=0.0, n_photons=n,
+             bands=[Band(wvl=1.0, g=0.85, ssa_ice=0.9995, ext_cff=5000.0)]),
+        Case(tau_tot=15.0, theta0_deg=45.0, n_photons=n,
+             bands=[Band(wvl=1.1, g=0.0, ssa_ice=0.999, ext_cff=5000.0)]),
+        Case(tau_tot=40.0, theta0_deg=60.0, n_photons=n,
+             bands=[Band(wvl=1.2, g=0.89, ssa_ice=0.9999, ext_cff=5000.0, ssa_imp=0.5, p_ext_imp=0.002)]),
+        Case(tau_tot=20.0, theta0_deg=30.0, n_photons=n, lambertian_bottom=True, r_lambertian=0.7,
+             bands=[Band(wvl=1.3, g=0.8, ssa_ice=0.999, ext_cff=5000.0, weight=0.5),
+                    Band(wvl=1.31, g=0.75, ssa_ice=0.998, ext_cff=5000.0, weight=0.5)]),
+    ]
+    walker.set_cases(cases)
+    runs = {}
+    try:
+        for coop in (32, 0):
+            walker.set_launch(1024, 1, 2048, 16, coop)
+            recs, cnt = [], 0
+            for s in range(3):
+                walker.walk(mode='native', seed=300 + 10 * coop + s, records=True)
+                recs.append(walker.records())
+                cnt = cnt + walker.counts()
+            runs[coop] = ({k: np.concatenate([r[k] for r in recs]) for k in recs[0]}, cnt)
+    finally:
+        walker.set_launch()
+    (a, ca), (b, cb) = runs[32], runs[0]
+    print('handed over per case:', ca[:, U_MOVED], flush=True)
+    assert int(ca[:, U_MOVED].sum()) > 0 and int(cb[:, U_MOVED].sum()) == 0
+    assert np.all(ca[:, 0] == 3 * n) and np.all(cb[:, 0] == 3 * n)
+    per_run = 4 * n
+    for k in range(4):
+        sel = np.concatenate([np.arange(r * per_run + k * n, r * per_run + (k + 1) * n) for r in range(3)])
+        ka, kb = a['condition'][sel], b['condition'][sel]
+        assert np.all(ka >= 1)
+        for cond in (1, 2, 4, 5) if k == 2 else (1, 2, 4):
+            z = two_sample_z(int(np.sum(ka == cond)), ka.size, int(np.sum(kb == cond)), kb.size)
+            assert abs(z) < 3.5, (k, cond, z)
+        assert int(ca[k, U_STEPS]) == int(np.sum(a['n_scat'][sel] + 1))
