@@ -67,6 +67,10 @@ struct DevBand {
     uint32_t t_abs5;                     // bits < P_imp (1 - ssa_imp) 2^32: absorbed by impurity
     uint32_t t_abs_all;                  // bits < [P_imp (1 - ssa_imp) + (1 - P_imp)(1 - ssa_ice)] 2^32: absorbed
     int32_t iso;                         // g == 0
+    // alias table of the case's band distribution (Walker): column j of the
+    // case keeps band j iff the fraction bits < alias_t, else takes alias_k
+    uint32_t alias_t;
+    int32_t alias_k;
 };
 
 // probability -> 32-bit threshold (bits < t has probability p)
@@ -181,14 +185,14 @@ struct NativeWalker {
     __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
     {
         const DevCase &C = P.cases[case_id];
-        const int bnd = select_band(P, C, case_id, photon, gid);
+        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
+        r = philox_step(pre, 1u, P.key0, P.key1);
+        const int bnd = select_band(P, C, gid, r.x);
         band = bnd;
         load_params(P, C, bnd);
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0;
         cond = 0;
-        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
-        r = philox_step(pre, 1u, P.key0, P.key1);
         if (C.surf) return surface_walk(P, C);
         // first step: along the incident direction (mc3D:1344-1347)
         ++i;
@@ -219,18 +223,20 @@ struct NativeWalker {
     }
 
     // the photon's wavelength bin: given per photon, or drawn from the case's
-    // band CDF with Philox block (photon, case, 0, 2)
-    __device__ static int select_band(const KParams &P, const DevCase &C, int case_id, uint32_t photon, uint64_t gid)
+    // band distribution with the first step's polar word x (the first step
+    // goes along the incident direction and draws no angle): the high bits of
+    // x n_bands pick an alias-table column, the low bits accept it or take its
+    // alias -- one table read instead of a search
+    __device__ static int select_band(const KParams &P, const DevCase &C, uint64_t gid, uint32_t x)
     {
         int bnd = 0;
         if (P.photon_band) {
             bnd = P.photon_band[gid];
         } else if (C.n_bands > 1) {
-            const u32x4 q = philox4x32_10(u32x4{photon, (uint32_t)case_id, 0u, 2u}, P.key0, P.key1);
-            const double u = ((double)q.x + 0.5) * 2.3283064365386963e-10;
-            bnd = C.n_bands - 1;
-            for (int k = 0; k < C.n_bands - 1; ++k)
-                if (u < P.bands[C.band_begin + k].cdf) { bnd = k; break; }
+            const uint64_t m = (uint64_t)x * (uint32_t)C.n_bands;
+            const int j = (int)(m >> 32);
+            const DevBand &B = P.bands[C.band_begin + j];
+            bnd = (uint32_t)m < B.alias_t ? j : B.alias_k;
         }
         return bnd;
     }
@@ -243,7 +249,7 @@ struct NativeWalker {
         pre = p;
         const uint32_t cs = case_id(P);
         const DevCase &C = P.cases[cs];
-        band = select_band(P, C, (int)cs, photon_id(), gid);
+        band = select_band(P, C, gid, philox_step(pre, 1u, P.key0, P.key1).x);
         load_params(P, C, band);
         z = z_; tau_acc = tau_; mux = dx; muy = dy; muz = dz; i = i_;
         cond = 0;
@@ -1592,6 +1598,33 @@ static int ensure(mcs_ctx *c, DevBuf &b, size_t bytes)
     return MCS_OK;
 }
 
+// Walker/Vose alias table over a case's bands.  Band k's probability is what
+// "first k with u < cdf_k" gives (cdf clamped to [0, 1], mass beyond the
+// running maximum; the last band takes the rest), so any cdf column keeps its
+// meaning.
+static void build_alias(DevBand *db, const mcs_band *b, int n)
+{
+    std::vector<double> q(n);
+    double m = 0.0;
+    for (int k = 0; k < n - 1; ++k) {
+        const double ck = std::min(std::max(b[k].cdf, 0.0), 1.0);
+        q[k] = std::max(ck - m, 0.0) * n;
+        m = std::max(m, ck);
+    }
+    q[n - 1] = (1.0 - m) * n;
+    std::vector<int> small, large;
+    for (int k = 0; k < n; ++k) (q[k] < 1.0 ? small : large).push_back(k);
+    for (int k = 0; k < n; ++k) { db[k].alias_k = k; db[k].alias_t = 0xFFFFFFFFu; }
+    while (!small.empty() && !large.empty()) {
+        const int s = small.back(), l = large.back();
+        small.pop_back();
+        db[s].alias_t = (uint32_t)std::min(q[s] * 4294967296.0, 4294967295.0);
+        db[s].alias_k = l;
+        q[l] -= 1.0 - q[s];
+        if (q[l] < 1.0) { large.pop_back(); small.push_back(l); }
+    }
+}
+
 static uint32_t prob_bits(double p)
 {
     double t = p * 4294967296.0;
@@ -1689,6 +1722,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.t_abs_all = prob_bits(q5 + (1.0 - B.p_ext_imp) * (1.0 - B.ssa_ice));
     }
     int64_t total = 0;
+    std::vector<int> row_case(n_bands, -1);   // first case using each band row
     for (int k = 0; k < n_cases; ++k) {
         const mcs_case &C = cases[k];
         if (!(C.tau_tot > 0) || !std::isfinite(C.tau_tot)) return set_err(c, MCS_ERR_ARG, "case %d: tau_tot must be finite and > 0", k);
@@ -1699,6 +1733,13 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         if (C.n_bands < 1 || C.n_bands > MCS_MAX_BANDS || C.band_begin < 0 || C.band_begin + C.n_bands > n_bands)
             return set_err(c, MCS_ERR_ARG, "case %d: band range [%d, %d) invalid", k, C.band_begin, C.band_begin + C.n_bands);
         if (C.n_photons < 0 || C.n_photons > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "case %d: n_photons outside [0, 2^32)", k);
+        // a band row's cdf (and alias entry) belongs to one band range
+        for (int j = C.band_begin; j < C.band_begin + C.n_bands; ++j) {
+            const int o = row_case[j];
+            if (o < 0) { row_case[j] = k; continue; }
+            if (cases[o].band_begin != C.band_begin || cases[o].n_bands != C.n_bands)
+                return set_err(c, MCS_ERR_ARG, "cases %d and %d share band %d with different band ranges", o, k, j);
+        }
         DevCase &D = dc[k];
         D.tau_tot = C.tau_tot; D.mux0 = C.mux0; D.muz0 = C.muz0; D.r_lamb = C.r_lambertian;
         D.t_refl = (unsigned long long)llround(C.r_lambertian * 4294967296.0);
@@ -1707,6 +1748,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.surf = C.lambertian_surface ? 1 : 0;
         D.n_photons = C.n_photons; D.gid_base = total;
         total += C.n_photons;
+        build_alias(&db[C.band_begin], &bands[C.band_begin], C.n_bands);
     }
     (void)hipSetDevice(c->device);
     if (c->pending) (void)hipStreamSynchronize(c->stream), c->pending = false;

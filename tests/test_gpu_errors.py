@@ -113,3 +113,16 @@ def test_step_cap_with_lambertian_reemission(walker, coop, cap):
         assert int(c[8]) == int(np.sum(r['n_scat'] + 1))
     finally:
         walker.set_launch()
+
+
+def test_cases_sharing_band_rows(walker):
+    """Cases may share band rows only as a whole range (each row's cdf and
+    alias entry belong to one band distribution)."""
+    carr, barr = _lib.McsCase * 2, _lib.McsBand * 3
+    bands = barr(*[_lib.McsBand(1.0, 0.85, 0.99, 0.0, 5000.0, 0.0, 0.001, c) for c in (0.5, 1.0, 1.0)])
+    ok = carr(_lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 0, 2, 0, 100),
+              _lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 0, 2, 0, 100))
+    assert _lib.lib().mcs_set_cases(walker._ctx, ok, 2, bands, 3) == 0
+    bad = carr(_lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 0, 2, 0, 100),
+               _lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 1, 2, 0, 100))
+    assert _lib.lib().mcs_set_cases(walker._ctx, bad, 2, bands, 3) == -1   # MCS_ERR_ARG

@@ -233,3 +233,29 @@ def test_reference_known_answer_through_drop_in(tmp_path, monkeypatch):
     assert abs(mc.calculate_albedo(rec) - R) < 1e-12
     mc.close()
 
+
+
+def test_native_band_sampling_follows_the_weights(walker):
+    """Native mode draws each photon's wavelength bin from its case's band
+    weights (an alias table over the cdf column): band frequencies in the
+    records match the weights (chi^2), a zero-weight band is never drawn, and
+    two cases with different band sets keep their own distributions."""
+    from monte_carlompi_amd.gpu import Band, Case
+    wa = np.array([3.0, 0.0, 1.0, 7.0, 2.0, 0.5, 4.0, 9.0, 1.5, 2.0])
+    wb = np.array([0.2, 0.8])
+    mk = lambda w, wvl0: [Band(wvl=wvl0 + 0.01 * j, g=0.85, ssa_ice=0.99, ext_cff=5000.0, weight=float(x))
+                          for j, x in enumerate(w)]
+    n = 400_000
+    walker.set_cases([Case(tau_tot=2.0, theta0_deg=30.0, n_photons=n, bands=mk(wa, 1.0)),
+                      Case(tau_tot=2.0, theta0_deg=30.0, n_photons=n, bands=mk(wb, 1.5))])
+    walker.walk(mode='native', seed=5, records=True)
+    r = walker.records()
+    for k, w in enumerate((wa, wb)):
+        b = r['band'][k * n:(k + 1) * n].astype(int)
+        cnt = np.bincount(b, minlength=w.size)
+        assert cnt.size == w.size
+        assert np.all(cnt[w == 0] == 0)
+        p = w / w.sum()
+        keep = p > 0
+        chi2 = np.sum((cnt[keep] - n * p[keep]) ** 2 / (n * p[keep]))
+        assert stats.chi2.sf(chi2, keep.sum() - 1) > 1e-3, (k, chi2, cnt)
