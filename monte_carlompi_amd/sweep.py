@@ -21,8 +21,8 @@ import numpy as np
 from .gpu import GpuWalker, brf_from_tallies, fractions_from_tallies
 
 DEFAULT_CANDIDATES = [
-    (256, 4, 2048, 8), (256, 4, 8192, 8), (256, 6, 4096, 16), (512, 2, 4096, 8),
-    (512, 3, 8192, 16), (1024, 1, 8192, 16), (256, 4, 4096, 32), (256, 5, 16384, 8),
+    (256, 6, 8192, 24), (256, 6, 4096, 24), (256, 6, 16384, 24), (256, 6, 8192, 16),
+    (256, 6, 4096, 16), (512, 3, 8192, 24), (256, 5, 8192, 24), (512, 3, 16384, 24),
 ]
 
 
