@@ -35,7 +35,7 @@ def copy_kernel(mib: int = 256):
     print('copied %d bytes x3' % (n * 4))
 
 
-def run_sweep(repeats=2, launch=(256, 6, 4096, 16)):
+def run_sweep(repeats=2, launch=(256, 6, 8192, 24)):
     """The batched sweep (16,384 cases) `repeats` times with a fixed launch shape
     (a workload to profile on its own)."""
     from .gpu import GpuWalker
