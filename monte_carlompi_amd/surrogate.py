@@ -1,4 +1,4 @@
-"""Learned BRF surrogate with the plain Monte Carlo as fallback and judge.
+"""Learned BRF surrogate with the plain Monte Carlo as fallback and auditor.
 
 Inputs per case: (r_eff [um], omega, g, theta0 [deg], tau) -- the snowpack
 optical properties the walk consumes.  The network sees eight standardised
@@ -15,7 +15,7 @@ kernels with FP32 Adam.  Serving evaluates the network for every query,
 re-runs a seeded fraction of the queries with the plain walk at high photon
 counts, and falls back to the plain walk for every query in an input region
 where the audited errors exceed the bars (albedo > 1 %, or BRF bins beyond
-3 % once the judge's own Monte Carlo noise is removed).
+3 % once the audit walk's own Monte Carlo noise is removed).
 
     python -m monte_carlompi_amd.surrogate --help
 """
@@ -243,7 +243,7 @@ def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=400):
     """Per-case albedo relative error and BRF error of the prediction against
     a Monte Carlo run.  BRF: over bins holding >= min_bin_photons reflected
     photons (after symmetrisation), the rms relative difference with the
-    judge's own binomial noise removed in quadrature, and the raw median."""
+    audit walk's own binomial noise removed in quadrature, and the raw median."""
     alb_err = np.abs(pred[:, BRF_BINS] - mc[:, BRF_BINS]) / np.maximum(mc[:, BRF_BINS], 1e-12)
     H = symmetrise_brf(mc_tallies[:, :BRF_BINS]) / mc_tallies[:, gpumod.T_QDOWN][:, None, None]
     nbin = (H * photons * 2).reshape(-1, BRF_BINS)          # two mirrored bins pooled
@@ -300,7 +300,7 @@ def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_pho
     fb = np.isin(reg_q, bad_regions)
     fb[aud] = False
     answers = pred.copy()
-    answers[aud] = mc                               # audited queries keep the judge's answer
+    answers[aud] = mc                               # audited queries keep the walk's answer
     fb_dev_s = 0.0
     if run_fallback and fb.any():
         fmc, _, fb_dev_s = run_mc(walker, queries[fb], fallback_photons, seed=seed + 99)

@@ -27,7 +27,7 @@ def test_surrogate_learns_and_beats_plain_mc_rate(walker):
     sur.fit(inp, tg, epochs=60, batch=1024, lr=3e-3)
     assert sur.history[-1] < 0.5 * sur.history[0]
     test_inp = S.sample_inputs(128, seed=2)
-    print('surrogate: judge', flush=True)
+    print('surrogate: audit walk', flush=True)
     mc, mct, mc_s = S.run_mc(walker, test_inp, 200_000, seed=13)
     pred = sur.predict(test_inp)
     err = S.case_errors(pred, mc, mct, 200_000)

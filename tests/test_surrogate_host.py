@@ -1,5 +1,5 @@
 """Surrogate host logic (no GPU): feature map, log-target bias correction,
-target variances, regions, symmetrisation, judge arithmetic."""
+target variances, regions, symmetrisation, audit arithmetic."""
 import numpy as np
 
 from monte_carlompi_amd import gpu, surrogate as S
