@@ -104,3 +104,33 @@ def test_sweep_cases_statistics_against_oracle(walker):
             pg, pc = cnt[k, cond] / c.n_photons, np.mean(cpu['condition'] == cond)
             s = np.sqrt(pg * (1 - pg) / c.n_photons + pc * (1 - pc) / m)
             assert abs(pg - pc) <= 3 * s + 1e-9, (k, cond, pg, pc, meta['tau'][pick[k]])
+
+
+def test_empty_and_tiny_cases_in_a_batch(walker):
+    """Cases with no photons, one photon and a partial last chunk, in one launch
+    with records: the chunk-to-case search of the walk and of the record
+    ordering skips the empty cases, every photon is recorded once in its own
+    case's id range, and per-case counts match the records."""
+    from monte_carlompi_amd.gpu import Band, Case
+    band = lambda: [Band(wvl=1.0, g=0.85, ssa_ice=0.99, ext_cff=5000.0)]
+    sizes = [0, 1, 0, 5000, 0, 0, 2049, 64, 0]
+    walker.set_cases([Case(tau_tot=3.0 + k, theta0_deg=20.0, n_photons=n, bands=band())
+                      for k, n in enumerate(sizes)])
+    try:
+        for shape in [(256, 4, 2048, 8, 16), (1024, 1, 2048, 16, 32), (64, 2, 64, 1, 0)]:
+            walker.set_launch(*shape)
+            walker.walk(mode='native', seed=3, records=True)
+            r = walker.records()
+            c = walker.counts()
+            assert r['condition'].size == sum(sizes)
+            assert np.all(r['condition'] >= 1)
+            off = 0
+            for k, n in enumerate(sizes):
+                assert int(c[k, 0]) == n, (shape, k)
+                seg = r['condition'][off:off + n]
+                for cond in range(1, 6):
+                    assert int(c[k, cond]) == int(np.sum(seg == cond)), (shape, k, cond)
+                assert int(c[k, 8]) == int(np.sum(r['n_scat'][off:off + n] + 1)), (shape, k)
+                off += n
+    finally:
+        walker.set_launch()
