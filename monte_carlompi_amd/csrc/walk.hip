@@ -8,13 +8,14 @@
 // that finishes a chunk's last photon flushes the slot to the case's global
 // tallies.  Finished photons are compacted per wave (ballot/mbcnt + one LDS
 // atomic) into the chunk's contiguous id range, so record stores of lanes that
-// finish together are contiguous; a second kernel scatters them to photon
-// order.
+// finish together are contiguous; a second kernel puts each chunk's records in
+// photon order through LDS.
 //
 // Two walkers share that skeleton:
 //  * NativeWalker: FP32, Philox4x32-10 keyed by the seed, counter
 //    (photon, case, step, stream); once the queue is empty, sparse waves
-//    finish their photons in the wave-cooperative mode below;
+//    finish their photons in the wave-cooperative mode below and hand them
+//    to waves of the block that ran out of work;
 //  * ReplayWalker: FP64 with the reference's exact step semantics
 //    (monte_carloMPI/monte_carlo3D.py:1212-1490), consuming the reference's
 //    uniform deviates from CSR slices.
@@ -158,11 +159,11 @@ __device__ __forceinline__ T lds_load(const T *p)
 
 // ----------------------------------------------------------- native walker
 // FP32 walk with one Philox4x32-10 call per step (4 uniforms: HG / Lambertian
-// cosine, azimuth, free path, absorption).  The common path is branch-free
-// (selects instead of branches) and the next step's Philox call is issued in
-// the same basic block as the current step's arithmetic: the two chains are
-// independent, so their latencies overlap -- what bounds the last, longest
-// walks of a launch.
+// cosine, azimuth, free path, absorption).  The common path is straight-line
+// code -- exits, absorption and near-vertical rotations sit behind rarely
+// taken branches (the last one wave-uniform) -- and the next step's Philox call
+// is issued in the same basic block as the current step's arithmetic: the two
+// chains are independent, so their latencies overlap.
 struct NativeWalker {
     using Real = float;
     static constexpr bool COOP = true;        // has the wave-cooperative tail mode below
