@@ -35,6 +35,7 @@
 namespace mcs {
 
 constexpr int SLOTS = 4;
+constexpr int MAX_WAVES = 16;     // waves per block (1024 threads)
 static_assert((SLOTS & (SLOTS - 1)) == 0, "SLOTS is a power of two");
 constexpr int TF = MCS_TALLY_F64;
 constexpr int TU = MCS_TALLY_U64;
@@ -127,12 +128,12 @@ struct BlockState {
     int lock;
     int exhausted;
     // hand-over of cooperative-tail photons between the block's waves
-    int holders;             // waves not waiting for photons
-    int idle;                // waves waiting for photons
-    int mb_state;            // mailbox: 0 free, 3 being filled, 1 posted, 2 being read
-    int mb_wave;             // posting wave (its coop table holds the photons)
-    unsigned int mb_mask;    // their table entries
-    int mb_pad;
+    int holders;                  // waves not waiting for photons
+    unsigned int idle_mask;       // waves waiting for photons (bit per wave)
+    int ib_state[MAX_WAVES];      // inbox of each wave: 1 = photons posted
+    int ib_src[MAX_WAVES];        //   posting wave (its coop table holds them)
+    unsigned int ib_mask[MAX_WAVES];  //   their table entries
+    int out_posts[MAX_WAVES];     // posts of each wave not yet taken
     SlotHdr hdr[SLOTS];
     SlotTally tal[SLOTS];
 };
@@ -955,14 +956,16 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     }
 }
 
-// Hand-over within the block.  A wave that has finished its photons waits
-// (`idle`) for a mailbox post; a wave in the cooperative phase with L >= 2
-// photons posts the upper half of them while a sibling waits: their walk state
-// goes into their table entries, the entry mask and the wave into the
-// mailbox, and their lanes are emptied.  The photons keep their chunk slot
-// (found again from case and photon id), so tallies, records and chunk
-// completion are unchanged; each photon still walks its own Philox stream.
-// Spread over the block's four SIMDs, the last long walks get more lanes each.
+// Hand-over within the block.  A wave that has finished its photons sets its
+// bit in `idle_mask` and polls its own inbox; a wave in the cooperative phase
+// with L >= 2 photons claims a waiting wave (clears its bit), hands its
+// carried group state back to the photon lanes, writes the upper half of its
+// photons' walk state into their table entries and posts entry mask and wave
+// to the claimed wave's inbox; those lanes are emptied.  Each wave has its own
+// inbox, so hand-overs to different waves proceed at once.  The photons keep
+// their chunk slot (found again from case and photon id), so tallies, records
+// and chunk completion are unchanged; each photon still walks its own Philox
+// stream.  Spread over the block's SIMDs, the last long walks get more lanes.
 __device__ __forceinline__ void coop_donate(BlockState &bs, NativeWalker &w, int &state, CoopEntry *tab, int me,
                                             CoopGroups &cg, unsigned long long *coop_steps, unsigned long long *moved,
                                             int wave)
@@ -970,24 +973,40 @@ __device__ __forceinline__ void coop_donate(BlockState &bs, NativeWalker &w, int
     const int lane = lane_id();
     const unsigned long long live = __ballot(state == LIVE);
     const int L = __popcll(live);
-    if (L < 2 || lds_load(&bs.idle) == 0 || lds_load(&bs.mb_state) != 0) return;
-    int ok = 0;
-    if (lane == 0) ok = atomicCAS(&bs.mb_state, 0, 3) == 0;
-    if (!bcast(ok, 0)) return;
+    if (L < 2) return;
+    unsigned int im = lds_load(&bs.idle_mask);
+    if (im == 0u) return;
+    int v = -1;
+    if (lane == 0) {
+        while (im) {
+            const int c = __ffs(im) - 1;
+            const unsigned int old = atomicAnd(&bs.idle_mask, ~(1u << c));
+            if ((old >> c) & 1u) {
+                v = c;
+                break;
+            }
+            im = old & ~(1u << c);
+        }
+    }
+    v = bcast(v, 0);
+    if (v < 0) return;
     if (cg.em) coop_handback(w, cg, coop_steps);
     cg.em = 0ull;   // regroup next round
     const int nd = L >> 1;
-    if (lane == 0) bs.mb_mask = 0u;
+    if (lane == 0) {
+        bs.ib_mask[v] = 0u;
+        bs.ib_src[v] = wave;
+        atomicAdd(&bs.out_posts[wave], 1);
+    }
     if (state == LIVE && lane_rank(live) >= L - nd) {
         CoopEntry &E = tab[me];
         E.z = w.z; E.tau = w.tau_acc; E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
-        atomicOr(&bs.mb_mask, 1u << me);
+        atomicOr(&bs.ib_mask[v], 1u << me);
         atomicAdd(moved, 1ull);
         state = EMPTY;
     }
-    if (lane == 0) bs.mb_wave = wave;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) __atomic_store_n(&bs.mb_state, 1, __ATOMIC_RELAXED);
+    if (lane == 0) __atomic_store_n(&bs.ib_state[v], 1, __ATOMIC_RELAXED);
 }
 
 // The cooperative phase of a wave: every photon the wave still holds is
@@ -1013,35 +1032,30 @@ __device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int 
     for (uint32_t round = 1; __ballot(state == LIVE); ++round) {
         coop_round(w, state, P, tab, map, me, cg, coop_steps);
         if constexpr (HAND) {
-            if ((round & 3u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
+            if ((round & 1u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
         }
     }
 }
 
-// Wait for photons posted by a sibling wave (all lanes EMPTY on entry).  True
-// with the posted photons LIVE on the first lanes; false once no sibling holds
-// photons and nothing is posted.
+// Wait for photons posted to this wave's inbox (all lanes EMPTY on entry).
+// True with the posted photons LIVE on the first lanes; false once no sibling
+// holds photons (then none can post) and the wave has left the idle set before
+// a donor claimed it.
 __device__ bool coop_receive(BlockState &bs, const KParams &P, NativeWalker &w, int &state, int &my_slot,
-                             uint64_t &my_gid, const CoopEntry *coop_tab)
+                             uint64_t &my_gid, const CoopEntry *coop_tab, int wave)
 {
     const int lane = lane_id();
     if (lane == 0) {
-        atomicAdd(&bs.idle, 1);
         atomicSub(&bs.holders, 1);
+        atomicOr(&bs.idle_mask, 1u << wave);
     }
     __builtin_amdgcn_s_setprio(0);
     for (int spin = 0; spin < (1 << 24); ++spin) {
-        if (lds_load(&bs.mb_state) == 1) {
-            int ok = 0;
-            if (lane == 0) ok = atomicCAS(&bs.mb_state, 1, 2) == 1;
-            if (!bcast(ok, 0)) continue;
-            if (lane == 0) {
-                atomicAdd(&bs.holders, 1);
-                atomicSub(&bs.idle, 1);
-            }
+        if (lds_load(&bs.ib_state[wave]) == 1) {
+            if (lane == 0) atomicAdd(&bs.holders, 1);
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const int dw = lds_load(&bs.mb_wave);
-            const unsigned int mask = lds_load(&bs.mb_mask);
+            const int dw = lds_load(&bs.ib_src[wave]);
+            const unsigned int mask = lds_load(&bs.ib_mask[wave]);
             if (lane < __popc(mask)) {
                 unsigned int m = mask;
                 for (int k = 0; k < lane; ++k) m &= m - 1u;
@@ -1064,19 +1078,38 @@ __device__ bool coop_receive(BlockState &bs, const KParams &P, NativeWalker &w, 
                     atomicAdd(P.ctl + 1, 1ull);   // cannot happen: the chunk waits for the photon
                 }
             }
-            // the entries are read before the mailbox is free again
+            // the donor's entries are read before it may rewrite its table
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (lane == 0) __atomic_store_n(&bs.mb_state, 0, __ATOMIC_RELAXED);
+            if (lane == 0) {
+                bs.ib_state[wave] = 0;
+                atomicSub(&bs.out_posts[dw], 1);
+            }
             return true;
         }
-        if (lds_load(&bs.holders) == 0 && lds_load(&bs.mb_state) == 0) {
-            if (lane == 0) atomicSub(&bs.idle, 1);
-            return false;
+        if (lds_load(&bs.holders) == 0) {
+            // nobody can post any more -- unless a donor claimed this wave just
+            // before: then its post is on the way
+            int left = 0;
+            if (lane == 0) left = (atomicAnd(&bs.idle_mask, ~(1u << wave)) >> wave) & 1u;
+            if (bcast(left, 0)) return false;
+            continue;
         }
         __builtin_amdgcn_s_sleep(4);
     }
     if (lane == 0) atomicAdd(P.ctl + 1, 1ull);   // cannot happen: siblings finish
     return false;
+}
+
+// Before a wave rewrites its coop table for photons it received: wait until
+// the waves it posted photons to have read them (they are polling their
+// inboxes, so this is short).
+__device__ __forceinline__ void await_own_posts(BlockState &bs, const KParams &P, int wave)
+{
+    for (int spin = 0; spin < (1 << 24); ++spin) {
+        if (lds_load(&bs.out_posts[wave]) == 0) return;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    if (lane_id() == 0) atomicAdd(P.ctl + 1, 1ull);   // cannot happen
 }
 
 // ------------------------------------------------------------ chunk queue
@@ -1392,12 +1425,13 @@ k_walk(const KParams P)
             for (;;) {
                 if (to_coop) {
                     __builtin_amdgcn_s_setprio(3);
+                    await_own_posts(bs, P, wave);
                     coop_phase<true>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
                                      &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
                     // every photon the wave held is finished or handed over
                     retire(bs, P, w, state, my_slot, my_gid);
                 }
-                if (!coop_receive(bs, P, w, state, my_slot, my_gid, coop_tab)) break;
+                if (!coop_receive(bs, P, w, state, my_slot, my_gid, coop_tab, wave)) break;
                 to_coop = true;
             }
         }
