@@ -8,6 +8,8 @@ phi0 = 0, Lambertian bottom R = 0.5 (monte_carlo3D-run.py:9-12).
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from .gpu import Band, Case
@@ -83,3 +85,73 @@ def spectral_bands(wvl0: float, fwhm: float, max_bands: int = 64, tail: float = 
     if len(out) > max_bands:
         raise ValueError('distribution spans more than %d bins' % max_bands)
     return out
+
+
+# ---------------------------------------------------------------- aspherical
+# The Yang et al. (2013) ice tables are not distributed with the reference;
+# this writes synthetic tables in their layout so that the aspherical
+# (phase-matrix + Stokes) walk can run: an ice-like P11 (forward peak, 22 deg
+# halo, weak backscatter) and smooth, bounded ratios P12/P11 .. P44/P11.
+YANG_WVLS = np.round(np.arange(0.30, 2.0001, 0.05), 2)
+YANG_REFF = (50.0, 100.0, 200.0)
+
+
+def yang_theta_grid():
+    """207 angles [deg]: 0.1 deg steps to 2, 0.5 to 10, 1 deg to 180."""
+    return np.concatenate([np.arange(0.0, 2.0, 0.1), np.arange(2.0, 10.0, 0.5), np.arange(10.0, 180.5, 1.0)])
+
+
+def _hg(mu, g):
+    return (1 - g * g) / (1 + g * g - 2 * g * mu) ** 1.5
+
+
+def synthetic_phase_matrix(theta_deg, wvl, reff):
+    """The six elements (P11 normalised to (1/2) int P11 dmu = 1; the others
+    as P_ij itself, not divided by P11) of one synthetic particle."""
+    th = np.deg2rad(theta_deg)
+    mu = np.cos(th)
+    g1 = 0.86 + 0.04 * np.sin(1.7 * wvl) + 0.01 * np.log(reff / 100.0)
+    halo = np.exp(-((theta_deg - 22.0) / 1.5) ** 2)
+    p11 = 0.82 * _hg(mu, g1) + 0.14 * _hg(mu, -0.25) + 0.6 * halo
+    norm = 0.5 * np.trapezoid(p11[::-1], mu[::-1])
+    p11 = p11 / norm
+    s2 = np.sin(th) ** 2
+    dol = 0.25 + 0.1 * np.cos(wvl)
+    el = {'P11': p11,
+          'P12': -dol * s2 / (1 + mu * mu) * p11,
+          'P22': (0.92 + 0.08 * mu) * p11,
+          'P33': (0.85 * mu + 0.1) * p11,
+          'P43': 0.12 * np.sin(th) ** 3 * p11,
+          'P44': (0.8 * mu + 0.15) * p11}
+    return el
+
+
+def write_synthetic_yang(optics_dir, shape='droxtal', roughness='smooth', wvls=YANG_WVLS, reffs=YANG_REFF):
+    """Write isca.dat and P11..P44.dat under
+    ``optics_dir/ice_optics/0.2-15.25/<shape>/<roughness>``: rows ordered by
+    wavelength then size; the projected area and volume give r_eff = 3/4 V / A
+    exactly ``reffs``.  Returns the directory."""
+    from . import optics
+    d = os.path.join(optics_dir, 'ice_optics', '0.2-15.25', optics.SHAPE_DIRS[shape],
+                     optics.ROUGHNESS_DIRS[roughness])
+    theta = yang_theta_grid()
+    rows = dict(wvl=[], max_dim=[], volume=[], area=[], q_ext=[], ssa=[], g=[])
+    els = {e: [] for e in optics.PM_ELEMENTS}
+    for w in wvls:
+        for r in reffs:
+            area = 2.0 * r * r
+            rows['wvl'].append(w)
+            rows['max_dim'].append(2.6 * r)
+            rows['volume'].append(area * r / 0.75)
+            rows['area'].append(area)
+            rows['q_ext'].append(2.0 + 0.02 * np.sin(3.0 * w))
+            rows['ssa'].append(1.0 - 2e-4 * (1.0 + 8.0 * w * w) * (r / 100.0))
+            el = synthetic_phase_matrix(theta, w, r)
+            mu = np.cos(np.deg2rad(theta))
+            rows['g'].append(float(0.5 * np.trapezoid((mu * el['P11'])[::-1], mu[::-1])))
+            for e in optics.PM_ELEMENTS:
+                els[e].append(el[e])
+    optics.write_yang_tables(d, *[np.array(rows[k]) for k in ('wvl', 'max_dim', 'volume', 'area', 'q_ext',
+                                                                'ssa', 'g')],
+                             theta_deg=theta, elements={e: np.array(v) for e, v in els.items()})
+    return d

@@ -145,7 +145,8 @@ class PhotonUnboundPhi(UnboundLocalError):
 
 def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
                 p_ext_imp_k, snow_depth_k, tau_tot, rho_snw, theta_0,
-                lambertian_bottom, r_lambertian, trajectory=True, lambertian_surface=False):
+                lambertian_bottom, r_lambertian, trajectory=True, lambertian_surface=False,
+                polar=None):
     """Walk one photon packet (mc3D:1111-1490, sphere/HG; with
     ``lambertian_surface`` the layer is a Lambertian reflector, mc3D:1228-1238,
     1385-1387).
@@ -153,7 +154,15 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
     ``first`` is the photon's initial triple (tau0, ssa0, ext0) from
     ``initial_triples``.  Returns the 7 CSV fields
     ``(condition, wvn, theta_n, phi_n, n_scat, path_length, snow_depth)``.
+
+    ``polar`` (an aspherical particle with its phase matrix, from
+    ``oracle.polar``): the angles come from the phase-matrix rejection
+    sampler and the Stokes vector is carried (mc3D:898-1004, 1283-1342); the
+    record then ends with its final Q, U, V (mc3D:1637-1646).
     """
+    if polar is not None:
+        from . import polar as _polar
+        stokes = polar['stokes0']
     # initial state: mc3D:1114-1132
     x_tau = np.array([0]); y_tau = np.array([0]); z_tau = np.array([0])
     mux_0 = np.sin(theta_0)
@@ -172,7 +181,10 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
     while condition == 0:                                   # mc3D:1212
         i += 1
         if i >= 2:                                          # i_rand > i_max: mc3D:1215-1224
-            p_r, tau_r, phi_r, ssa_r, ext_r = _draw_step(src, g_k)
+            if polar is not None:
+                p_r, tau_r, phi_r, ssa_r, ext_r = _polar.draw_step(src, polar, stokes)
+            else:
+                p_r, tau_r, phi_r, ssa_r, ext_r = _draw_step(src, g_k)
         dtau_current = tau_r                                # mc3D:1227
         if lambertian_surface and i == 1:                   # mc3D:1228-1229
             dtau_current = 0
@@ -209,8 +221,12 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
                 muy_n = ((sintheta * (muy_0 * muz_0 * cosphi + mux_0 * sinphi)) /
                          (np.sqrt(1 - muz2_0)) + muy_0 * costheta)
                 muz_n = (-np.sqrt(1 - muz2_0) * sintheta * cosphi + muz_0 * costheta)
-            if bottom_reflection:
+            if bottom_reflection:                           # mc3D:1283-1285
+                if polar is not None:
+                    stokes = np.array([1, 0, 0, 0])
                 bottom_reflection = False
+            elif polar is not None:                         # mc3D:1287-1342
+                stokes = _polar.stokes_update(polar, stokes, costheta, cosphi, muz_n, muz_0)
         else:                                               # mc3D:1344-1347
             mux_n = mux_0; muy_n = muy_0; muz_n = muz_0
 
@@ -288,6 +304,9 @@ def walk_photon(src, first, wvl, g_k, ssa_ice_k, ssa_imp_k, ext_cff_mss_k,
     else:
         raise PhotonUnboundPhi("local variable 'phi_n' referenced before assignment")
     n_scat = i - 1
+    if polar is not None:
+        return (condition, wvn, theta_n, phi_n, n_scat, path_length, snow_depth_k,
+                stokes[1], stokes[2], stokes[3])
     return (condition, wvn, theta_n, phi_n, n_scat, path_length, snow_depth_k)
 
 

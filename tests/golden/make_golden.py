@@ -19,7 +19,7 @@ How the reference is made to run here (nothing of it is copied):
 * the global numpy stream is seeded with ``np.random.seed(seed)`` before
   each ``run()``; the reference never seeds it itself.
 
-Usage:  python tests/golden/make_golden.py   (needs /root/reference)
+Usage:  python tests/golden/make_golden.py [--polar]   (needs /root/reference)
 """
 import json
 import os
@@ -207,5 +207,99 @@ def main():
         json.dump(dict(rho_snw=300.0, rho_ice=917.0, cases=meta), fh, indent=1)
 
 
+# ---------------------------------------------------------------- aspherical
+# Phase-matrix + Stokes walk (mc3D:145-496, 819-1004, 1287-1342) and the
+# aspherical SSPs with the HG phase function (HG=True).  The Yang et al.
+# tables are absent: synthetic tables in their layout are written by
+# monte_carlompi_amd.workloads.write_synthetic_yang (wavelengths 0.30..2.00 um
+# every 0.05 um, r_eff 50 / 100 / 200 um).
+POLAR_CASES = [
+    dict(name='pm_free', seed=2001, n=300, wvl0=1.3, fwhm=1e-12, rds=100, theta0=30.0, lb=False,
+         R=0.0, tau=3.0, imp_cnc=0.0, shape='droxtal', rough='smooth', HG=False, stokes=[1, 0, 0, 0]),
+    dict(name='pm_linear_q', seed=2002, n=250, wvl0=1.0, fwhm=1e-12, rds=200, theta0=0.0, lb=True,
+         R=0.5, tau=2.0, imp_cnc=0.0, shape='droxtal', rough='smooth', HG=False, stokes=[1, 1, 0, 0]),
+    dict(name='pm_spectral_u', seed=2003, n=200, wvl0=1.3, fwhm=0.2, rds=60, theta0=60.0, lb=True,
+         R=0.8, tau=4.0, imp_cnc=1e-4, shape='droxtal', rough='smooth', HG=False, stokes=[1, 0, 1, 0]),
+    dict(name='pm_circular', seed=2004, n=200, wvl0=0.5, fwhm=1e-12, rds=100, theta0=45.0, lb=False,
+         R=0.0, tau=5.0, imp_cnc=0.0, shape='droxtal', rough='smooth', HG=False, stokes=[1, 0, 0, 1]),
+    dict(name='pm_surface', seed=2005, n=200, wvl0=1.3, fwhm=1e-12, rds=100, theta0=50.0, lb=False,
+         R=0.7, tau=5.0, imp_cnc=0.0, shape='droxtal', rough='smooth', HG=False, stokes=[1, 1, 0, 0],
+         ls=True),
+    dict(name='pm_one_photon', seed=2006, n=1, wvl0=1.3, fwhm=0.085, rds=100, theta0=20.0, lb=True,
+         R=0.5, tau=3.0, imp_cnc=0.0, shape='droxtal', rough='smooth', HG=False, stokes=[1, 0, 0, 0]),
+    dict(name='asph_hg', seed=2007, n=400, wvl0=1.3, fwhm=0.085, rds=100, theta0=30.0, lb=True,
+         R=0.5, tau=6.0, imp_cnc=0.0, shape='hexagonal plate', rough='severely rough', HG=True,
+         stokes=[1, 0, 0, 0]),
+]
+_INT = re.compile(r'np\.int64\(([^)]*)\)')
+
+
+def polar_main():
+    """Run POLAR_CASES through the reference; write golden_polar.npz/json
+    (records plus Q_n, U_n, V_n, the table directory is regenerated from
+    the same synthetic writer at test time)."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+    from monte_carlompi_amd import workloads
+    tmp = tempfile.mkdtemp(prefix='mc_golden_pm_')
+    os.makedirs(os.path.join(tmp, 'mpi4py'))
+    with open(os.path.join(tmp, 'mpi4py', '__init__.py'), 'w') as fh:
+        fh.write(MPI_STUB)
+    with open(os.path.join(tmp, 'config.ini'), 'w') as fh:
+        fh.write(CONFIG_INI)
+    os.chdir(tmp)
+    sys.argv = [sys.argv[0]]
+    sys.path[:0] = [tmp, REF]
+    import matplotlib
+    matplotlib.use('Agg')
+    from monte_carloMPI import monte_carlo3D
+    store = {}
+    meta = []
+    for c in POLAR_CASES:
+        optics = os.path.join(tmp, 'optics_' + c['name'])
+        os.makedirs(os.path.join(optics, 'mie', 'snicar'))
+        write_nc(os.path.join(optics, 'mie', 'snicar', 'imp.nc'), imp_table(0.3, 12000.0))
+        workloads.write_synthetic_yang(optics, shape=c['shape'], roughness=c['rough'])
+        outdir = os.path.join(tmp, 'out_' + c['name'])
+        np.random.seed(c['seed'])
+        mc = monte_carlo3D.MonteCarlo(tau_tot=c['tau'], imp_cnc=c['imp_cnc'], output_dir=outdir,
+                                      optics_dir=optics, fi_imp='imp.nc', HG=c['HG'])
+        mc.run(c['n'], c['wvl0'], c['fwhm'], c['rds'], theta_0=c['theta0'],
+               stokes_params=np.array(c['stokes']), shape=c['shape'], roughness=c['rough'],
+               Lambertian_bottom=c['lb'], Lambertian_reflectance=c['R'],
+               Lambertian_surface=c.get('ls', False))
+        sub = os.path.join(outdir, mc.shape_dir, mc.roughness_dir)
+        files = os.listdir(sub)
+        assert len(files) == 1, files
+        header, rows, lowprec = parse_output(os.path.join(sub, files[0]))
+        rows = [[_INT.sub(r'\1', f) for f in r] for r in rows]
+        arr = np.array(rows, dtype=object)
+        p = c['name'] + '/'
+        store[p + 'condition'] = arr[:, 0].astype(np.int64)
+        for j, key in enumerate(['wvn', 'theta_n', 'phi_n'], start=1):
+            store[p + key] = np.array([float(x) for x in arr[:, j]])
+        store[p + 'n_scat'] = arr[:, 4].astype(np.int64)
+        store[p + 'path_length'] = np.array([float(x) for x in arr[:, 5]])
+        store[p + 'snow_depth'] = np.array([float(x) for x in arr[:, 6]])
+        if not c['HG']:
+            for j, key in enumerate(['Q_n', 'U_n', 'V_n'], start=7):
+                store[p + key] = np.array([float(x) for x in arr[:, j]])
+        m = dict(c)
+        m.setdefault('ls', False)
+        m['file_name'] = files[0]
+        m['subdir'] = os.path.join(mc.shape_dir, mc.roughness_dir)
+        m['header'] = header
+        m['lowprec_columns'] = lowprec
+        m['snow_effective_radius'] = float(mc.snow_effective_radius)
+        meta.append(m)
+        print('%-14s n=%5d  conditions=%s  file=%s' % (
+            c['name'], c['n'], np.bincount(store[p + 'condition'], minlength=6).tolist(), files[0]))
+    np.savez_compressed(os.path.join(HERE, 'golden_polar.npz'), **store)
+    with open(os.path.join(HERE, 'golden_polar.json'), 'w') as fh:
+        json.dump(dict(rho_snw=300.0, rho_ice=917.0, cases=meta), fh, indent=1)
+
+
 if __name__ == '__main__':
-    main()
+    if '--polar' in sys.argv:
+        polar_main()
+    else:
+        main()
