@@ -180,6 +180,40 @@ int mcs_sync(mcs_ctx *ctx);
 /* replaces: comm.gather + flatten_list (parallelize.py:17-26, mc3D:1618) */
 int mcs_read_records(mcs_ctx *ctx, const mcs_records *out, int64_t n_photons);
 
+/* Aspherical ice grains (Yang et al. 2013 tables, shape != 'sphere' and not
+ * HG): the scattering phase matrix replaces Henyey-Greenstein and every
+ * photon carries a Stokes vector.  One entry per case set by mcs_set_cases. */
+typedef struct {
+    int32_t table;             /* phase-matrix table of the case's band 0
+                                  (band j uses table + j); -1: HG (spheres,
+                                  or HG forced for aspherical grains)         */
+    int32_t cutoff;            /* cutoff_idx of interpolate_phase_matrix
+                                  (mc3D:858-883): index into the angle grid
+                                  in file order (theta ascending)             */
+    double stokes0[4];         /* initial Stokes vector (I, Q, U, V), I != 0
+                                  (run(stokes_params=...), mc3D:1132)         */
+} mcs_polar_case;
+
+/* replaces: interpolate_phase_matrix (mc3D:819-883) and the per-wavelength
+ * P11..P44 of get_aspherical_SSPs (mc3D:267-496).  `tables` holds n_tables
+ * tables of 7 x n_angles doubles: row 0 the ascending cos(theta) grid
+ * (cos(theta_deg * pi / 180) of the files, reversed), rows 1..6 P11, P12,
+ * P22, P33, P43, P44 on that grid (P12..P44 already multiplied by P11), i.e.
+ * the arrays of the reference's interp1d(cos_theta[::-1], P[::-1]).  The
+ * tables of one case share one grid.  Call after mcs_set_cases; pc = NULL
+ * clears the polar state (mcs_set_cases also does).  Walks then sample the
+ * angles by the reference's rejection method (mc3D:905-1004; native mode:
+ * Philox blocks (photon, case, step, 2 + attempt)) and update the Stokes
+ * vector at every scattering (mc3D:1283-1342). */
+int mcs_set_polar(mcs_ctx *ctx, const mcs_polar_case *pc, int32_t n_cases,
+                  const double *tables, int32_t n_tables, int32_t n_angles);
+
+/* replaces: the Q_n, U_n, V_n columns of the aspherical record
+ * (mc3D:1637-1646): final Stokes vector (normalised to I = 1) of every
+ * photon of the last walk, photon order (needs want_records and polar cases;
+ * HG cases of the same walk read (0, 0, 0)). */
+int mcs_read_stokes(mcs_ctx *ctx, double *q, double *u, double *v, int64_t n_photons);
+
 /* Per-case float64 tallies (n_cases x MCS_TALLY_F64), the sums that
  * post_processing.py computes from the CSV (pp:46-100, 125-128, 269-275):
  *   [0, 648)   sum of wvn over reflected photons (condition 1) binned by

@@ -26,6 +26,7 @@ MAX_BANDS = 256
 EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs_set_cases',
            'mcs_set_launch', 'mcs_get_launch', 'mcs_walk', 'mcs_sync', 'mcs_read_records',
            'mcs_read_tallies', 'mcs_read_counts', 'mcs_last_stats', 'mcs_autotune', 'mcs_philox',
+           'mcs_set_polar', 'mcs_read_stokes',
            'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_last_error', 'mcs_mlp_param_count',
            'mcs_mlp_set_params', 'mcs_mlp_get_params', 'mcs_mlp_forward', 'mcs_mlp_last_ms',
            'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps']
@@ -69,6 +70,11 @@ class McsLaunch(ctypes.Structure):
 
     def as_tuple(self):
         return (self.block, self.blocks_per_cu, self.chunk, self.refill_min, self.coop_live)
+
+
+class McsPolarCase(ctypes.Structure):
+    _fields_ = [('table', ctypes.c_int32), ('cutoff', ctypes.c_int32),
+                ('stokes0', ctypes.c_double * 4)]
 
 
 class McsRecords(ctypes.Structure):
@@ -130,6 +136,8 @@ def lib():
         'mcs_autotune': ([P, P, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32, ctypes.c_uint64,
                           P, ctypes.POINTER(McsLaunch)], I),
         'mcs_philox': ([P, P, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_int64], I),
+        'mcs_set_polar': ([P, P, ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32], I),
+        'mcs_read_stokes': ([P, P, P, P, ctypes.c_int64], I),
         'mcs_mlp_create': ([I, ctypes.POINTER(McsMlpDims), ctypes.POINTER(P)], I),
         'mcs_mlp_destroy': ([P], I),
         'mcs_mlp_last_error': ([P], ctypes.c_char_p),

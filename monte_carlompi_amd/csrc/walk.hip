@@ -50,6 +50,9 @@ constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
 constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
 // counts (uint64)
 constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED = 10;
+// rejection attempts per scattering angle before the walk reports an error
+// (ctl[2]); valid tables accept within a few
+constexpr uint32_t POLAR_MAX_TRIES = 1u << 20;
 
 struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
@@ -57,6 +60,11 @@ struct DevCase {
     float tau_tot_f, mux0_f, muz0_f;
     int32_t lamb, band_begin, n_bands, surf;
     int64_t n_photons, gid_base;
+    // aspherical particles: initial Stokes vector and the rejection sampler's
+    // cos(theta) range and split (mc3D:858-883, 1132)
+    int32_t polar;
+    double s0[4], cmax, ccut, cmin;
+    float s0f[4], fcmax, fccut, fcmin;
 };
 
 struct DevBand {
@@ -73,6 +81,11 @@ struct DevBand {
     // case keeps band j iff the fraction bits < alias_t, else takes alias_k
     uint32_t alias_t;
     int32_t alias_k;
+    // phase-matrix table of the band (-1: Henyey-Greenstein) and the maxima
+    // of |P11| and |P12| on either side of the cutoff (mc3D:926-941)
+    int32_t pm;
+    double pA1, pB1, pA2, pB2;
+    float fA1, fB1, fA2, fB2;
 };
 
 // probability -> 32-bit threshold (bits < t has probability p)
@@ -104,6 +117,15 @@ struct KParams {
     unsigned long long *ctl;     // [0] replay slice errors, [1] failed hand-overs
     uint32_t *rec_gid, *rec_nscat, *rec_cb;
     void *rec_theta, *rec_phi, *rec_path;
+    // phase-matrix tables (7 rows of pm_n: ascending cos theta, P11, P12, P22,
+    // P33, P43, P44), FP64 and FP32 copies; final Stokes (Q, U, V) per photon
+    // in photon order (3 x n_total, nullable)
+    const double *pm;
+    const float *pmf;
+    int32_t pm_n;
+    int32_t pad0;
+    void *rec_stokes;
+    uint64_t n_total;
 };
 
 // ---------------------------------------------------------------- LDS state
@@ -165,9 +187,41 @@ __device__ __forceinline__ T lds_load(const T *p)
 // taken branches (the last one wave-uniform) -- and the next step's Philox call
 // is issued in the same basic block as the current step's arithmetic: the two
 // chains are independent, so their latencies overlap.
-struct NativeWalker {
+// np.interp(v, x, y) on an ascending grid of n points (what the reference's
+// interp1d(cos_theta[::-1], P[::-1]) evaluates, mc3D:842-857): y at the ends
+// and at grid points, else slope (v - x_j) + y_j with x_j <= v < x_j+1.
+template <class T>
+__device__ __forceinline__ int grid_index(const T *x, int n, T v)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v)
+{
+#pragma clang fp contract(off)
+    if (v <= x[0]) return y[0];
+    if (v >= x[n - 1]) return y[n - 1];
+    if (x[j] == v) return y[j];
+    const T slope = (y[j + 1] - y[j]) / (x[j + 1] - x[j]);
+    T r = slope * (v - x[j]) + y[j];
+    if (r != r) {
+        r = slope * (v - x[j + 1]) + y[j + 1];
+        if (r != r && y[j] == y[j + 1]) r = y[j];
+    }
+    return r;
+}
+
+template <bool POLAR>
+struct NativeWalkerT {
     using Real = float;
-    static constexpr bool COOP = true;        // has the wave-cooperative tail mode below
+    static constexpr bool COOP = !POLAR;      // has the wave-cooperative tail mode below
+    static constexpr bool STOKES = POLAR;     // carries a Stokes vector (aspherical particles)
     static constexpr int MAX_BLOCK = 1024;
     // k_walk is built for 5 (96 VGPRs) and 6 (80 VGPRs) waves per SIMD
     float z, mux, muy, muz, tau_acc;
@@ -181,6 +235,10 @@ struct NativeWalker {
     bool iso, lamb;
     int cond;
     PhiloxPre pre;
+    // aspherical particles (POLAR): Stokes vector normalised to I = 1 and the
+    // band's phase-matrix table (-1: Henyey-Greenstein)
+    float sI, sQ, sU, sV;
+    int pm;
 
     // Set up photon `photon` of case `case_id` and take its first step (or
     // its whole walk in the Lambertian surface mode); true when it is done.
@@ -195,6 +253,10 @@ struct NativeWalker {
         z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
         i = 0;
         cond = 0;
+        if constexpr (POLAR) {
+            pm = C.polar ? P.bands[C.band_begin + bnd].pm : -1;
+            sI = C.s0f[0]; sQ = C.s0f[1]; sU = C.s0f[2]; sV = C.s0f[3];
+        }
         if (C.surf) return surface_walk(P, C);
         // first step: along the incident direction (mc3D:1344-1347)
         ++i;
@@ -307,6 +369,13 @@ struct NativeWalker {
         const u32x4 cur = r;
         r = philox_step(pre, i + 1u, P.key0, P.key1);
         lambertian_direction(cur);
+        if constexpr (POLAR) {
+            // the step after a bottom reflection still draws its angles from
+            // the phase matrix (the azimuth is used), then resets the Stokes
+            // vector (mc3D:1238-1285)
+            if (pm >= 0) polar_lambertian(P, cur, false);
+            sI = 1.f; sQ = 0.f; sU = 0.f; sV = 0.f;
+        }
         const float dtau = free_path(cur);
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
@@ -342,6 +411,12 @@ struct NativeWalker {
             ++i;
             cur = philox_step(pre, i, P.key0, P.key1);
             lambertian_direction(cur);
+            if constexpr (POLAR) {
+                // Lambertian surface of aspherical grains: azimuth from the
+                // phase matrix, Stokes vector scattered at the Lambertian
+                // angle (mc3D:1238-1250, 1287-1342)
+                if (pm >= 0) polar_lambertian(P, cur, true);
+            }
             const float dtau = free_path(cur);
             z = fmaf(dtau, muz, z);
             tau_acc += dtau;
@@ -369,10 +444,17 @@ struct NativeWalker {
         // scale and offset folded into d and e; isotropic for g = 0
         const float xf = (float)cur.x;
         const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
-        const float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));   // |cost| <= 1
+        float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
         // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
-        const float rev = u01f(cur.y);
+        float rev = u01f(cur.y);
+        float rev2 = 0.f;
+        if constexpr (POLAR) {
+            if (pm >= 0) {   // aspherical: phase-matrix rejection sampler
+                polar_angles(P, cost, rev2);
+                rev = 0.5f * rev2;
+            }
+        }
+        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));   // |cost| <= 1
         const float sinp = __builtin_amdgcn_sinf(rev);
         const float cosp = __builtin_amdgcn_cosf(rev);
         // rotation (mc3D:1255-1281); near-vertical incoming directions take the
@@ -392,6 +474,9 @@ struct NativeWalker {
                 muy = sg * sint * sinp;
                 muz = sg * cost;
             }
+        }
+        if constexpr (POLAR) {
+            if (pm >= 0) stokes_update(P, cost, rev2, muz, mz0);
         }
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
@@ -433,6 +518,121 @@ struct NativeWalker {
         return (unsigned long long)q.y < P.cases[cs].t_refl;
     }
 
+    // ---- aspherical particles (POLAR only)
+    __device__ __forceinline__ const float *pm_table(const KParams &P) const
+    {
+        return P.pmf + (size_t)pm * 7 * (size_t)P.pm_n;
+    }
+
+    // Scattering angles of mc3D:905-1004: rejection sampling of
+    // I S11(cos) + S12(cos) (Q cos 2phi + U sin 2phi) over cos theta in two
+    // areas split at the cutoff (each bounded by |I| max|P11| + max|P12|
+    // sqrt(Q^2 + U^2), the reference's A + B C) and 2 phi uniform in
+    // [0, 4 pi).  Attempt k of step i draws Philox block (photon, case, i,
+    // 2 + k): area, cos theta, 2 phi, acceptance.  Returns cos theta and
+    // 2 phi in revolutions.
+    __device__ void polar_angles(const KParams &P, float &cost, float &rev2) const
+    {
+        const uint32_t cs = case_id(P);
+        const DevCase &C = P.cases[cs];
+        const DevBand &B = P.bands[C.band_begin + band];
+        const float *T = pm_table(P);
+        const int n = P.pm_n;
+        const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        const float ai = fabsf(sI);
+        const float mv1 = fmaf(ai, B.fA1, B.fB1 * cq), mv2 = fmaf(ai, B.fA2, B.fB2 * cq);
+        const float w1 = C.fcmax - C.fccut, w2 = C.fccut - C.fcmin;
+        const float a1 = w1 * mv1, a2 = w2 * mv2;
+        const float pct = a1 / (a1 + a2);
+        const uint32_t ph = photon_id();
+        for (uint32_t k = 0;; ++k) {
+            const u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u + k}, P.key0, P.key1);
+            const bool in1 = u01f(q.x) < pct;
+            const float u = u01f(q.y);
+            const float ct = in1 ? fmaf(u, w1, C.fccut) : fmaf(-u, w2, C.fccut);
+            const float r2 = 2.f * u01f(q.z);
+            const float r3 = 1.000001f * u01f(q.w) * (in1 ? mv1 : mv2);
+            const int j = grid_index(T, n, ct);
+            const float s11 = interp_at(T, T + n, n, j, ct);
+            const float s12 = interp_at(T, T + 2 * n, n, j, ct);
+            const float val = fmaf(sI, s11, s12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
+            if (r3 <= val) {
+                cost = ct;
+                rev2 = r2;
+                return;
+            }
+            if (k + 1 >= POLAR_MAX_TRIES) {
+                atomicAdd(P.ctl + 2, 1ull);
+                cost = ct;
+                rev2 = r2;
+                return;
+            }
+        }
+    }
+
+    // Stokes vector after scattering by (theta, phi) from direction z-cosine
+    // mz0 to muz_n (mc3D:1287-1342): rotate into the scattering plane by
+    // phi_sca = arccos(cos phi) (phi folded into [0, pi]: the double angle's
+    // sine changes sign for phi > pi), apply the phase matrix at cos theta,
+    // rotate back by -i_2 with cos i_2 = (muz_n cos - mz0) / (sign den)
+    // (sign -1 unless phi_sca = pi, as in the reference; cos i_2 = 0 where
+    // den = 0), normalise to I = 1.  The doubled angles of the return
+    // rotation come from cos i_2 directly: cos 2i = 2c^2 - 1, sin 2i = 2c s.
+    __device__ void stokes_update(const KParams &P, float cost, float rev2, float muz_n, float mz0)
+    {
+        const float *T = pm_table(P);
+        const int n = P.pm_n;
+        const float c2 = __builtin_amdgcn_cosf(rev2);
+        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * __builtin_amdgcn_sinf(rev2);
+        const float Qs = fmaf(sQ, c2, sU * s2);
+        const float Us = fmaf(-sQ, s2, sU * c2);
+        const int j = grid_index(T, n, cost);
+        const float p11 = interp_at(T, T + n, n, j, cost);
+        const float p12 = interp_at(T, T + 2 * n, n, j, cost);
+        const float p22 = interp_at(T, T + 3 * n, n, j, cost);
+        const float p33 = interp_at(T, T + 4 * n, n, j, cost);
+        const float p43 = interp_at(T, T + 5 * n, n, j, cost);
+        const float p44 = interp_at(T, T + 6 * n, n, j, cost);
+        const float Ic = fmaf(sI, p11, Qs * p12);
+        const float Qc = fmaf(sI, p12, Qs * p22);
+        const float Uc = fmaf(Us, p33, -sV * p43);
+        const float Vc = fmaf(Us, p43, sV * p44);
+        const float den = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f) *
+                                                 fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
+        float ci = 0.f;
+        if (den != 0.f) {
+            const float sg = (rev2 == 1.f) ? 1.f : -1.f;
+            ci = fminf(fmaxf(fmaf(muz_n, cost, -mz0) / (sg * den), -1.f), 1.f);
+        }
+        const float si = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-ci, ci, 1.f), 0.f));
+        const float cr = fmaf(2.f * ci, ci, -1.f);      // cos(-2 i_2)
+        const float sr = -2.f * ci * si;               // sin(-2 i_2)
+        const float inv = 1.f / Ic;
+        const float Qm = fmaf(Qc, cr, Uc * sr);
+        const float Um = fmaf(-Qc, sr, Uc * cr);
+        sI = 1.f;
+        sQ = Qm * inv;
+        sU = Um * inv;
+        sV = Vc * inv;
+    }
+
+    // Lambertian re-emission of an aspherical-grain photon (direction set by
+    // lambertian_direction from cur): the azimuth is the phase-matrix
+    // sampler's (the reference keeps phi_rand there); with `update` the
+    // Stokes vector scatters at the Lambertian angle from mz0 = 1.
+    __device__ void polar_lambertian(const KParams &P, const u32x4 &cur, bool update)
+    {
+        float ct, rev2;
+        polar_angles(P, ct, rev2);
+        (void)cur;
+        const float cost = muz;
+        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        const float rv = 0.5f * rev2;
+        mux = sint * __builtin_amdgcn_cosf(rv);
+        muy = sint * __builtin_amdgcn_sinf(rv);
+        if (update) stokes_update(P, cost, rev2, muz, 1.f);
+    }
+
     __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const
     {
         theta = acosf(fminf(fmaxf(muz, -1.f), 1.f));
@@ -449,12 +649,16 @@ struct NativeWalker {
     }
 };
 
+using NativeWalker = NativeWalkerT<false>;
+using PolarWalker = NativeWalkerT<true>;
+
 // ----------------------------------------------------------- replay walker
 // FP64, the reference's operation order; contraction off so that every
 // product and sum rounds as numpy's float64 scalars do.
 struct ReplayWalker {
     using Real = double;
     static constexpr bool COOP = false;
+    static constexpr bool STOKES = true;
     static constexpr int MAX_BLOCK = 256;
     static constexpr int WAVES_PER_EU = 2;
     double z, mux, muy, muz, muz2, path;
@@ -465,6 +669,10 @@ struct ReplayWalker {
     int band;
     bool bounce, lamb, err, surf;
     int cond;
+    // aspherical particles: Stokes vector and the band's phase-matrix table
+    double sI, sQ, sU, sV;
+    int pm;
+    int case_id;
 
     __device__ double draw(const KParams &P)
     {
@@ -489,6 +697,9 @@ struct ReplayWalker {
         pos = P.roff[gid];
         end = P.roff[gid + 1];
         err = false;
+        this->case_id = case_id;
+        pm = C.polar ? B.pm : -1;
+        sI = C.s0[0]; sQ = C.s0[1]; sU = C.s0[2]; sV = C.s0[3];
         dtau0 = -log(draw(P));
         u_ssa = draw(P);
         u_ext = draw(P);
@@ -505,10 +716,14 @@ struct ReplayWalker {
     {
 #pragma clang fp contract(off)
         ++i;
-        double dtau, u_hg = 0.0, u_phi = 0.0;
+        double dtau, u_hg = 0.0, u_phi = 0.0, p_cos = 0.0, two_phi = 0.0;
         if (i >= 2) {
-            u_hg = draw(P);
-            u_phi = draw(P);
+            if (pm >= 0) {
+                polar_draws(P, p_cos, two_phi);
+            } else {
+                u_hg = draw(P);
+                u_phi = draw(P);
+            }
             dtau = -log(draw(P));
             u_ssa = draw(P);
             u_ext = draw(P);
@@ -530,6 +745,9 @@ struct ReplayWalker {
                         break;
                     }
                 }
+            } else if (pm >= 0) {
+                cost = p_cos;
+                sint = sqrt(1.0 - cost * cost);
             } else if (g == 0.0) {
                 cost = 1.0 - 2.0 * u_hg;
                 sint = sqrt(1.0 - cost * cost);
@@ -538,7 +756,7 @@ struct ReplayWalker {
                 cost = (1.0 / (2.0 * g)) * ((1.0 + g * g) - t * t);
                 sint = sqrt(1.0 - cost * cost);
             }
-            const double ph = u_phi * (2.0 * PI_D);
+            const double ph = (pm >= 0) ? two_phi / 2.0 : u_phi * (2.0 * PI_D);
             const double cosp = cos(ph), sinp = sin(ph);
             if (muz == 1.0) {
                 mxn = sint * cosp; myn = sint * sinp; mzn = cost;
@@ -549,6 +767,11 @@ struct ReplayWalker {
                 mxn = (sint * (mux * muz * cosp - muy * sinp)) / sq + mux * cost;
                 myn = (sint * (muy * muz * cosp + mux * sinp)) / sq + muy * cost;
                 mzn = -sq * sint * cosp + muz * cost;
+            }
+            if (bounce) {                        // mc3D:1283-1285
+                sI = 1.0; sQ = 0.0; sU = 0.0; sV = 0.0;
+            } else if (pm >= 0) {                // mc3D:1287-1342
+                stokes_update(P, cost, cosp, mzn, muz);
             }
             bounce = false;
         }
@@ -586,6 +809,108 @@ struct ReplayWalker {
             return true;
         }
         return err;
+    }
+
+    // populate_pdfs for an aspherical particle (mc3D:898-1004): area, cos
+    // theta and phi, then the rejection loop (r3; after a rejection: area,
+    // cos theta, 2 phi, r3) against I S11 + S12 (Q cos 2phi + U sin 2phi).
+    // The reference's operation order and float64 functions.
+    __device__ void polar_draws(const KParams &P, double &ct, double &two_phi)
+    {
+#pragma clang fp contract(off)
+        const DevCase &C = P.cases[case_id];
+        const DevBand &B = P.bands[C.band_begin + band];
+        const int n = P.pm_n;
+        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
+        const double I = sI, Q = sQ, U = sU;
+        const double two_beta = (Q == 0.0) ? atan(U / 1e-15) : atan(U / Q);
+        double A = fabs(I) * B.pA1;
+        double Bm = B.pB1;
+        const double Cc = fabs(Q * cos(two_beta) + U * sin(two_beta));
+        const double mv1 = A + Bm * Cc;
+        A = fabs(I) * B.pA2;
+        Bm = B.pB2;
+        const double mv2 = A + Bm * Cc;
+        const double a1 = (C.cmax - C.ccut) * mv1;
+        const double a2 = (C.ccut - C.cmin) * mv2;
+        const double pct = a1 / (a1 + a2);
+        double ar = draw(P);
+        ct = draw(P);
+        two_phi = 2.0 * (draw(P) * (2.0 * PI_D));
+        int area;
+        if (ar < pct) {
+            area = 1;
+            ct = ct * (C.cmax - C.ccut) + C.ccut;
+        } else {
+            area = 2;
+            ct = -ct * (C.ccut - C.cmin) + C.ccut;
+        }
+        double r3 = 1.0, val = 0.0;
+        for (int k = 0; r3 > val; ++k) {
+            if (k > 0) {
+                ar = draw(P);
+                if (ar < pct) {
+                    area = 1;
+                    ct = draw(P) * (C.cmax - C.ccut) + C.ccut;
+                } else {
+                    area = 2;
+                    ct = -draw(P) * (C.ccut - C.cmin) + C.ccut;
+                }
+                two_phi = draw(P) * (4.0 * PI_D);
+            }
+            r3 = 1.000001 * draw(P);
+            r3 = (area == 1) ? r3 * mv1 : r3 * mv2;
+            const int j = grid_index(x, n, ct);
+            const double S11 = interp_at(x, x + n, n, j, ct);
+            const double S12 = interp_at(x, x + 2 * n, n, j, ct);
+            val = I * S11 + S12 * (Q * cos(two_phi) + U * sin(two_phi));
+            if (err) break;
+        }
+    }
+
+    // mc3D:1287-1342 and rotate_stokes_vector (mc3D:1090-1109), the
+    // reference's order and float64 functions.
+    __device__ void stokes_update(const KParams &P, double costheta, double cosphi, double muz_n, double muz_0)
+    {
+#pragma clang fp contract(off)
+        const int n = P.pm_n;
+        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
+        const double cos2theta = costheta * costheta;
+        const double phi_sca = acos(cosphi);
+        double da = 2.0 * phi_sca;
+        double c = cos(da), s = sin(da);
+        const double Isp = sI, Qsp = sQ * c + sU * s, Usp = -sQ * s + sU * c, Vsp = sV;
+        const int j = grid_index(x, n, costheta);
+        const double P11 = interp_at(x, x + n, n, j, costheta);
+        const double P12 = interp_at(x, x + 2 * n, n, j, costheta);
+        const double P22 = interp_at(x, x + 3 * n, n, j, costheta);
+        const double P33 = interp_at(x, x + 4 * n, n, j, costheta);
+        const double P43 = interp_at(x, x + 5 * n, n, j, costheta);
+        const double P44 = interp_at(x, x + 6 * n, n, j, costheta);
+        const double Ic = Isp * P11 + Qsp * P12;
+        const double Qc = Isp * P12 + Qsp * P22;
+        const double Uc = Usp * P33 - Vsp * P43;
+        const double Vc = Usp * P43 + Vsp * P44;
+        const double den = sqrt((1.0 - cos2theta) * (1.0 - muz_n * muz_n));
+        double ci;
+        if (den == 0.0) {
+            ci = 0.0;
+        } else {
+            const double num = muz_n * costheta - muz_0;
+            const double sg = (phi_sca >= PI_D) ? 1.0 : -1.0;
+            ci = num / (sg * den);
+            if (ci < -1.0) ci = -1.0;
+            else if (ci > 1.0) ci = 1.0;
+        }
+        const double i2 = acos(ci);
+        da = 2.0 * (-i2);
+        c = cos(da);
+        s = sin(da);
+        const double Qm = Qc * c + Uc * s, Um = -Qc * s + Uc * c;
+        sI = Ic / Ic;
+        sQ = Qm / Ic;
+        sU = Um / Ic;
+        sV = Vc / Ic;
     }
 
     __device__ void result(const KParams &, int, double &theta, double &phi, double &pth, bool &axis) const
@@ -1218,6 +1543,14 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
             bool axis;
             w.result(P, H.case_id, th, ph, pa, axis);
             if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
+            if constexpr (W::STOKES) {
+                if (P.rec_stokes) {   // photon order: Q, U, V planes of n_total
+                    typename W::Real *st = (typename W::Real *)P.rec_stokes;
+                    st[my_gid] = w.sQ;
+                    st[P.n_total + my_gid] = w.sU;
+                    st[2 * P.n_total + my_gid] = w.sV;
+                }
+            }
             const DevBand &B = P.bands[P.cases[H.case_id].band_begin + w.band];
             const double wvn = B.wvn;
             unsafeAtomicAdd(&T.f[T_QDOWN], wvn);
@@ -1540,7 +1873,7 @@ __global__ void k_summary(const unsigned long long *tu, int nc, const unsigned l
     }
     if (t == 0) {
         out[0] = red[0][0]; out[1] = red[1][0]; out[2] = red[2][0]; out[3] = red[3][0];
-        out[4] = ctl[0]; out[5] = ctl[1];
+        out[4] = ctl[0]; out[5] = ctl[1]; out[6] = ctl[2];
         out[7] = 1ull;
     }
 }
@@ -1580,6 +1913,11 @@ struct mcs_ctx {
     // device buffers
     DevBuf d_cases, d_bands, d_chunk0, d_tally_f, d_tally_u, d_ctl, d_pband, d_rdev, d_roff;
     DevBuf d_rec[6], d_out[5];
+    // aspherical particles: phase-matrix tables (FP64, FP32), final Stokes
+    bool polar = false;
+    int32_t pm_n = 0;
+    DevBuf d_pm, d_pmf, d_stokes;
+    bool last_stokes = false;
     int last_mode = MCS_NATIVE;
     bool last_records = false;
     int64_t last_n = 0;
@@ -1703,7 +2041,7 @@ int mcs_destroy(mcs_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     DevBuf *all[] = {&c->d_cases, &c->d_bands, &c->d_chunk0, &c->d_tally_f, &c->d_tally_u,
-                     &c->d_ctl, &c->d_pband, &c->d_rdev, &c->d_roff};
+                     &c->d_ctl, &c->d_pband, &c->d_rdev, &c->d_roff, &c->d_pm, &c->d_pmf, &c->d_stokes};
     for (DevBuf *b : all) if (b->p) (void)hipFree(b->p);
     for (auto &b : c->d_rec) if (b.p) (void)hipFree(b.p);
     for (auto &b : c->d_out) if (b.p) (void)hipFree(b.p);
@@ -1755,6 +2093,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         const double q5 = B.p_ext_imp * (1.0 - B.ssa_imp);
         D.t_abs5 = prob_bits(q5);
         D.t_abs_all = prob_bits(q5 + (1.0 - B.p_ext_imp) * (1.0 - B.ssa_ice));
+        D.pm = -1;   // Henyey-Greenstein until mcs_set_polar
     }
     int64_t total = 0;
     std::vector<int> row_case(n_bands, -1);   // first case using each band row
@@ -1781,6 +2120,8 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.tau_tot_f = (float)C.tau_tot; D.mux0_f = (float)C.mux0; D.muz0_f = (float)C.muz0;
         D.lamb = C.lambertian_bottom ? 1 : 0; D.band_begin = C.band_begin; D.n_bands = C.n_bands;
         D.surf = C.lambertian_surface ? 1 : 0;
+        D.polar = 0;
+        D.s0[0] = 1.0; D.s0f[0] = 1.f;   // (1, 0, 0, 0): unpolarised until mcs_set_polar
         D.n_photons = C.n_photons; D.gid_base = total;
         total += C.n_photons;
         build_alias(&db[C.band_begin], &bands[C.band_begin], C.n_bands);
@@ -1804,6 +2145,100 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
     c->dbands.swap(db);
     c->total_photons = total;
     c->cases_set = true;
+    c->polar = false;
+    c->gkey = 0;
+    return MCS_OK;
+}
+
+int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const double *tables,
+                  int32_t n_tables, int32_t n_angles)
+{
+    if (!c) return MCS_ERR_ARG;
+    if (!c->cases_set) return set_err(c, MCS_ERR_STATE, "mcs_set_cases must come first");
+    const int nc = (int)c->cases.size();
+    std::vector<DevCase> dc = c->dcases;
+    std::vector<DevBand> db = c->dbands;
+    for (auto &D : dc) {
+        D.polar = 0;
+        D.s0[0] = 1.0; D.s0[1] = D.s0[2] = D.s0[3] = 0.0;
+        D.s0f[0] = 1.f; D.s0f[1] = D.s0f[2] = D.s0f[3] = 0.f;
+    }
+    for (auto &D : db) D.pm = -1;
+    bool any = false;
+    const int n = n_angles;
+    if (pc) {
+        if (n_cases != nc) return set_err(c, MCS_ERR_ARG, "n_cases = %d, %d cases are set", n_cases, nc);
+        if (n_tables < 0 || (n_tables > 0 && (!tables || n < 2)))
+            return set_err(c, MCS_ERR_ARG, "tables: need n_tables >= 0 and n_angles >= 2");
+        if ((int64_t)n_tables * 7 * n > ((int64_t)1 << 31))
+            return set_err(c, MCS_ERR_ARG, "phase-matrix tables too large");
+        for (int t = 0; t < n_tables; ++t) {
+            const double *x = tables + (size_t)t * 7 * n;
+            for (int k = 0; k < 7 * n; ++k)
+                if (!std::isfinite(x[k])) return set_err(c, MCS_ERR_ARG, "table %d: non-finite value", t);
+            for (int k = 0; k < n; ++k) {
+                if (x[k] < -1.0 || x[k] > 1.0) return set_err(c, MCS_ERR_ARG, "table %d: cos theta outside [-1, 1]", t);
+                if (k && !(x[k] > x[k - 1])) return set_err(c, MCS_ERR_ARG, "table %d: cos theta not ascending", t);
+            }
+        }
+        for (int k = 0; k < nc; ++k) {
+            const mcs_polar_case &Q = pc[k];
+            if (Q.table < 0) continue;
+            DevCase &D = dc[k];
+            if (Q.table + D.n_bands > n_tables)
+                return set_err(c, MCS_ERR_ARG, "case %d: tables [%d, %d) beyond %d", k, Q.table, Q.table + D.n_bands, n_tables);
+            if (Q.cutoff < 0 || Q.cutoff >= n) return set_err(c, MCS_ERR_ARG, "case %d: cutoff outside the grid", k);
+            for (int q = 0; q < 4; ++q)
+                if (!std::isfinite(Q.stokes0[q])) return set_err(c, MCS_ERR_ARG, "case %d: non-finite Stokes vector", k);
+            if (!(Q.stokes0[0] != 0.0)) return set_err(c, MCS_ERR_ARG, "case %d: Stokes I must be non-zero", k);
+            const double *x0 = tables + (size_t)Q.table * 7 * n;
+            const int jc = n - 1 - Q.cutoff;   // the cutoff in ascending order
+            D.polar = 1;
+            for (int q = 0; q < 4; ++q) { D.s0[q] = Q.stokes0[q]; D.s0f[q] = (float)Q.stokes0[q]; }
+            D.cmax = x0[n - 1]; D.cmin = x0[0]; D.ccut = x0[jc];
+            D.fcmax = (float)D.cmax; D.fcmin = (float)D.cmin; D.fccut = (float)D.ccut;
+            for (int b = 0; b < D.n_bands; ++b) {
+                const int t = Q.table + b;
+                const double *x = tables + (size_t)t * 7 * n;
+                for (int k2 = 0; k2 < n; ++k2)
+                    if (x[k2] != x0[k2]) return set_err(c, MCS_ERR_ARG, "case %d: its tables have different grids", k);
+                DevBand &B = db[D.band_begin + b];
+                if (B.pm >= 0 && B.pm != t)
+                    return set_err(c, MCS_ERR_ARG, "band row %d: two phase-matrix tables", D.band_begin + b);
+                B.pm = t;
+                // |P11|, |P12| maxima over theta <= theta_cut (area 1) and >= (area 2)
+                double a1 = 0, b1 = 0, a2 = 0, b2 = 0;
+                for (int k2 = 0; k2 < n; ++k2) {
+                    const double p11 = std::fabs(x[n + k2]), p12 = std::fabs(x[2 * n + k2]);
+                    if (k2 >= jc) { a1 = std::max(a1, p11); b1 = std::max(b1, p12); }
+                    if (k2 <= jc) { a2 = std::max(a2, p11); b2 = std::max(b2, p12); }
+                }
+                if (!(a1 > 0) || !(a2 > 0)) return set_err(c, MCS_ERR_ARG, "table %d: P11 must be positive", t);
+                B.pA1 = a1; B.pB1 = b1; B.pA2 = a2; B.pB2 = b2;
+                B.fA1 = (float)a1; B.fB1 = (float)b1; B.fA2 = (float)a2; B.fB2 = (float)b2;
+            }
+            any = true;
+        }
+    }
+    (void)hipSetDevice(c->device);
+    if (c->pending) (void)hipStreamSynchronize(c->stream), c->pending = false;
+    int r;
+    if (any) {
+        const size_t nv = (size_t)n_tables * 7 * n;
+        std::vector<float> tf(nv);
+        for (size_t k = 0; k < nv; ++k) tf[k] = (float)tables[k];
+        if ((r = ensure(c, c->d_pm, sizeof(double) * nv))) return r;
+        if ((r = ensure(c, c->d_pmf, sizeof(float) * nv))) return r;
+        HIPCHK(c, hipMemcpyAsync(c->d_pm.p, tables, sizeof(double) * nv, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_pmf.p, tf.data(), sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_cases.p, dc.data(), sizeof(DevCase) * nc, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_bands.p, db.data(), sizeof(DevBand) * db.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->dcases.swap(dc);
+    c->dbands.swap(db);
+    c->polar = any;
+    c->pm_n = any ? n : 0;
     c->gkey = 0;
     return MCS_OK;
 }
@@ -1852,7 +2287,10 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     // events bracket the walk kernel alone: kernel_ms is its device time
     if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
-    if (mode == MCS_NATIVE) {
+    if (mode == MCS_NATIVE && c->polar) {
+        // aspherical particles: one photon per lane (no cooperative tail)
+        hipLaunchKernelGGL((k_walk<PolarWalker, 4>), dim3(grid), dim3(block), 0, c->stream, P);
+    } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         // shapes asking for more than 5 waves per SIMD get the 80-VGPR instance
         // (6 waves per SIMD; its spills sit outside the step loop)
@@ -2000,6 +2438,14 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     P.tally_u = (unsigned long long *)c->d_tally_u.p;
     P.ctl = (unsigned long long *)c->d_ctl.p;
     P.chunk_counter = (unsigned int *)((unsigned long long *)c->d_ctl.p + 8);
+    P.pm = (const double *)c->d_pm.p;
+    P.pmf = (const float *)c->d_pmf.p;
+    P.pm_n = c->pm_n;
+    P.n_total = (uint64_t)n;
+    if (records && c->polar) {
+        if ((r = ensure(c, c->d_stokes, 3 * rsz * (size_t)std::max<int64_t>(n, 1)))) return r;
+        P.rec_stokes = c->d_stokes.p;
+    }
     if (records) {
         P.rec_gid = (uint32_t *)c->d_rec[0].p;
         P.rec_nscat = (uint32_t *)c->d_rec[1].p;
@@ -2039,6 +2485,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     }
     c->last_mode = a->mode;
     c->last_records = records;
+    c->last_stokes = records && c->polar;
     c->last_n = n;
     c->pending = true;
     if (a->async) return MCS_OK;
@@ -2054,7 +2501,7 @@ static int finish_walk(mcs_ctx *c)
     // k_summary wrote the sums into pinned host memory before the stream drained
     const volatile unsigned long long *h = c->h_sum;
     if (h[7] != 1ull) return set_err(c, MCS_ERR_HIP, "internal: walk summary not written");
-    unsigned long long ctl[2] = {h[4], h[5]};
+    unsigned long long ctl[3] = {h[4], h[5], h[6]};
     mcs_stats s{};
     s.photons = h[0];
     s.steps = h[1];
@@ -2065,6 +2512,9 @@ static int finish_walk(mcs_ctx *c)
     if (ctl[1])
         return set_err(c, MCS_ERR_HIP, "internal: %llu failed photon hand-overs between waves",
                        (unsigned long long)ctl[1]);
+    if (ctl[2])
+        return set_err(c, MCS_ERR_ARG, "%llu scattering angles not accepted in %u phase-matrix rejection tries "
+                       "(P11 + P12 terms not positive?)", (unsigned long long)ctl[2], POLAR_MAX_TRIES);
     if (s.photons != (uint64_t)c->last_n)
         return set_err(c, MCS_ERR_HIP, "walk finished %llu of %lld photons",
                        (unsigned long long)s.photons, (long long)c->last_n);
@@ -2111,6 +2561,28 @@ int mcs_read_records(mcs_ctx *c, const mcs_records *o, int64_t n)
         for (int k = 0; k < 3; ++k) {
             if (!dst[k]) continue;
             HIPCHK(c, hipMemcpy(tmp.data(), c->d_out[2 + k].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
+            for (int64_t j = 0; j < n; ++j) dst[k][j] = tmp[j];
+        }
+    }
+    return MCS_OK;
+}
+
+int mcs_read_stokes(mcs_ctx *c, double *q, double *u, double *v, int64_t n)
+{
+    if (!c) return MCS_ERR_ARG;
+    if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (!c->last_stokes) return set_err(c, MCS_ERR_STATE, "last walk kept no Stokes vectors (records off or no polar cases)");
+    if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
+    (void)hipSetDevice(c->device);
+    double *dst[3] = {q, u, v};
+    if (c->last_mode == MCS_REPLAY) {
+        for (int k = 0; k < 3; ++k)
+            if (dst[k] && n) HIPCHK(c, hipMemcpy(dst[k], (double *)c->d_stokes.p + (size_t)k * n, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    } else {
+        std::vector<float> tmp(n);
+        for (int k = 0; k < 3; ++k) {
+            if (!dst[k] || !n) continue;
+            HIPCHK(c, hipMemcpy(tmp.data(), (float *)c->d_stokes.p + (size_t)k * n, 4 * (size_t)n, hipMemcpyDeviceToHost));
             for (int64_t j = 0; j < n; ++j) dst[k][j] = tmp[j];
         }
     }

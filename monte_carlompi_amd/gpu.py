@@ -14,8 +14,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (BRF_BINS, MCS_NATIVE, MCS_REPLAY, PHI_BINS, TALLY_F64, TALLY_U64,
-                   THETA_BINS, McsBand, McsCase, McsLaunch, McsRecords, McsStats,
-                   McsWalkArgs, check, lib)
+                   THETA_BINS, McsBand, McsCase, McsLaunch, McsPolarCase, McsRecords,
+                   McsStats, McsWalkArgs, check, lib)
 
 # tally columns (see include/mcsnow.h)
 T_QDOWN = BRF_BINS
@@ -38,8 +38,22 @@ class Band:
 
 
 @dataclass
+class PolarTables:
+    """Scattering phase matrices of an aspherical-grain case (one per band, in
+    band order): ``x`` the ascending cos(theta) grid, ``Y[j]`` the (6, n)
+    array P11, P12, P22, P33, P43, P44 of band j on it (P12.. multiplied by
+    P11), ``cutoff`` the rejection split as a file-order grid index
+    (mc3D:858-883), ``stokes0`` the initial Stokes vector."""
+    x: np.ndarray
+    Y: list
+    cutoff: int
+    stokes0: tuple = (1.0, 0.0, 0.0, 0.0)
+
+
+@dataclass
 class Case:
-    """A snowpack/illumination configuration walked by ``n_photons`` photons."""
+    """A snowpack/illumination configuration walked by ``n_photons`` photons.
+    ``polar``: phase matrices for aspherical grains (None: Henyey-Greenstein)."""
     tau_tot: float
     theta0_deg: float
     n_photons: int
@@ -47,6 +61,7 @@ class Case:
     lambertian_bottom: bool = False
     r_lambertian: float = 0.0
     lambertian_surface: bool = False
+    polar: PolarTables = None
 
     def direction(self):
         th = (np.pi * self.theta0_deg) / 180.           # mc3D:1508
@@ -74,6 +89,32 @@ def pack_cases(cases):
                               c.tau_tot / x.ext_cff, float(cdf[j]))
             b += 1
     return carr, barr
+
+
+def pack_polar(cases):
+    """Case list -> (mcs_polar_case array, tables (n_tables, 7, n), n) or None
+    when no case is polar."""
+    pol = [c.polar for c in cases if c.polar is not None]
+    if not pol:
+        return None
+    n = int(np.asarray(pol[0].x).size)
+    parr = (McsPolarCase * len(cases))()
+    tabs = []
+    for k, c in enumerate(cases):
+        if c.polar is None:
+            parr[k] = McsPolarCase(-1, 0, (ctypes.c_double * 4)(1.0, 0.0, 0.0, 0.0))
+            continue
+        x = np.asarray(c.polar.x, dtype=np.float64)
+        if x.size != n:
+            raise ValueError('case %d: polar grids must all have %d points' % (k, n))
+        if len(c.polar.Y) != len(c.bands):
+            raise ValueError('case %d: %d phase matrices for %d bands' % (k, len(c.polar.Y), len(c.bands)))
+        parr[k] = McsPolarCase(len(tabs), int(c.polar.cutoff),
+                               (ctypes.c_double * 4)(*[float(v) for v in c.polar.stokes0]))
+        for Y in c.polar.Y:
+            Y = np.asarray(Y, dtype=np.float64).reshape(6, n)
+            tabs.append(np.concatenate([x[None, :], Y], axis=0))
+    return parr, np.ascontiguousarray(np.stack(tabs)), n
 
 
 class GpuWalker:
@@ -117,6 +158,12 @@ class GpuWalker:
     def set_cases(self, cases):
         carr, barr = pack_cases(cases)
         self._check(lib().mcs_set_cases(self._ctx, carr, len(cases), barr, len(barr)))
+        pp = pack_polar(cases)
+        self.polar = pp is not None
+        if pp is not None:
+            parr, tabs, n = pp
+            self._check(lib().mcs_set_polar(self._ctx, parr, len(cases), tabs.ctypes.data,
+                                            tabs.shape[0], n))
         self.cases = list(cases)
         self.n_photons = int(sum(c.n_photons for c in cases))
         self._band_wvn = [np.array([1.0 / b.wvl for b in c.bands]) for c in cases]
@@ -187,6 +234,14 @@ class GpuWalker:
         out['wvn'] = wvn
         out['snow_depth'] = depth
         return out
+
+    def stokes(self):
+        """Final Stokes vector (Q_n, U_n, V_n; I = 1) of every photon of the
+        last walk, photon order (polar cases; HG cases read zeros)."""
+        n = self.n_photons
+        q, u, v = np.empty(n), np.empty(n), np.empty(n)
+        self._check(lib().mcs_read_stokes(self._ctx, q.ctypes.data, u.ctypes.data, v.ctypes.data, n))
+        return dict(Q_n=q, U_n=u, V_n=v)
 
     def tallies(self):
         t = np.empty((len(self.cases), TALLY_F64))

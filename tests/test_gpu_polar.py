@@ -1,0 +1,159 @@
+"""Aspherical grains on the GPU (phase-matrix sampling + Stokes vectors,
+mc3D:898-1004, 1283-1342).
+
+* replay (FP64, the reference's deviate stream): photon by photon the
+  reference's records -- condition and n_scat exactly, angles, path and the
+  final Q, U, V within 1e-5 relative / 1e-9 absolute -- on every golden case
+  the reference itself wrote (tests/golden/golden_polar.*);
+* native (FP32, Philox): the same law as the oracle -- exit fractions within
+  4 sigma, mean final Q, U, V of reflected photons within 4 sigma, reflected
+  zenith-angle histogram chi^2 -- on cases with and without a Lambertian
+  bottom and polarised incidence.
+"""
+import numpy as np
+import pytest
+
+from oracle import polar
+from tests import golden_polar
+from tests.gpu_cases import case_from_params
+from tests.test_oracle_polar import IMP, META, run_case
+
+pytestmark = pytest.mark.gpu
+_, CASES = golden_polar.load()
+ELEMENTS = polar.ELEMENTS
+
+
+@pytest.fixture(scope='module')
+def walker():
+    from monte_carlompi_amd.gpu import GpuWalker
+    w = GpuWalker(0)
+    yield w
+    w.close()
+
+
+def gpu_case(case, par, tabs, bands):
+    from monte_carlompi_amd.gpu import PolarTables
+    g, pband = case_from_params(par, case['tau'], META['rho_snw'], case['theta0'], case['lb'], case['R'],
+                                case.get('ls', False))
+    if tabs is not None:
+        keys = [b.wvl for b in g.bands]
+        g.polar = PolarTables(x=tabs['x'], Y=[np.stack([bands[k]['Y'][e] for e in ELEMENTS]) for k in keys],
+                              cutoff=tabs['cut'], stokes0=tuple(float(v) for v in case['stokes']))
+    return g, pband
+
+
+@pytest.mark.parametrize('case', CASES, ids=[c['name'] for c in CASES])
+def test_polar_replay_matches_reference_golden(walker, case, tmp_path):
+    print('polar replay', case['name'], flush=True)
+    records, par, values, offsets, reff, tabs, bands = run_case(case, str(tmp_path))
+    g, pband = gpu_case(case, par, tabs, bands)
+    walker.set_cases([g])
+    walker.walk(mode='replay', records=True, photon_band=pband, replay=(values, offsets))
+    got = walker.records()
+    ref = case['records']
+    np.testing.assert_array_equal(got['condition'], ref['condition'])
+    np.testing.assert_array_equal(got['n_scat'], ref['n_scat'])
+    rtol = 2e-5 if case['lowprec_columns'] else 1e-5
+    for f in ('theta_n', 'phi_n', 'path_length', 'wvn', 'snow_depth'):
+        np.testing.assert_allclose(got[f], ref[f], rtol=rtol, atol=1e-12, err_msg=f)
+    if not case['HG']:
+        st = walker.stokes()
+        for f in ('Q_n', 'U_n', 'V_n'):
+            np.testing.assert_allclose(st[f], ref[f], rtol=1e-5, atol=1e-9, err_msg=f)
+
+
+def native_vs_oracle(walker, case, root, n_oracle, n_gpu, seed):
+    """Oracle records of n_oracle photons and GPU native records of n_gpu
+    photons of the same case (one wavelength)."""
+    c = dict(case, n=n_oracle, seed=seed)
+    records, par, values, offsets, reff, tabs, bands = run_case(c, root)
+    g, pband = gpu_case(c, par, tabs, bands)
+    g.n_photons = n_gpu
+    walker.set_cases([g])
+    walker.walk(mode='native', seed=seed, records=True)
+    rec = walker.records()
+    st = walker.stokes() if tabs is not None else None
+    cols = list(zip(*records))
+    o = dict(condition=np.asarray(cols[0]), theta_n=np.asarray(cols[2], float), n_scat=np.asarray(cols[4]))
+    if tabs is not None:
+        o.update(Q_n=np.asarray(cols[7], float), U_n=np.asarray(cols[8], float), V_n=np.asarray(cols[9], float))
+    return o, rec, st
+
+
+def assert_same_law(o, rec, st, polarised):
+    no, ng = o['condition'].size, rec['condition'].size
+    for cond in range(1, 6):
+        # two-proportion z test with the pooled fraction (rare conditions
+        # have a few oracle counts: the pooled variance is the right scale)
+        po = np.mean(o['condition'] == cond)
+        pg = np.mean(rec['condition'] == cond)
+        pp = (po * no + pg * ng) / (no + ng)
+        sd = np.sqrt(pp * (1 - pp) * (1 / no + 1 / ng)) + 1e-12
+        assert abs(po - pg) <= 4 * sd + 1e-9, (cond, po, pg, sd)
+    # reflected zenith angles: 9 bins of 10 deg, chi^2 of two samples
+    ro = o['theta_n'][o['condition'] == 1]
+    rg = rec['theta_n'][rec['condition'] == 1]
+    edges = np.linspace(0, np.pi / 2, 10)
+    ho = np.histogram(ro, edges)[0].astype(float)
+    hg = np.histogram(rg, edges)[0].astype(float)
+    k1, k2 = np.sqrt(hg.sum() / ho.sum()), np.sqrt(ho.sum() / hg.sum())
+    m = (ho + hg) > 0
+    chi2 = np.sum((k1 * ho[m] - k2 * hg[m]) ** 2 / (ho[m] + hg[m]))
+    from scipy import stats
+    assert stats.chi2.sf(chi2, m.sum() - 1) > 1e-4, (chi2, ho, hg)
+    if polarised:
+        for f in ('Q_n', 'U_n', 'V_n'):
+            a = o[f][o['condition'] == 1]
+            b = st[f][rec['condition'] == 1]
+            sd = np.sqrt(a.var() / a.size + b.var() / b.size) + 1e-12
+            assert abs(a.mean() - b.mean()) <= 4 * sd, (f, a.mean(), b.mean(), sd)
+
+
+@pytest.mark.parametrize('name,n_oracle', [('pm_free', 6000), ('pm_linear_q', 6000), ('pm_circular', 12000),
+                                           ('pm_surface', 6000), ('asph_hg', 6000)])
+def test_polar_native_follows_the_oracle_law(walker, name, n_oracle, tmp_path):
+    print('polar native', name, flush=True)
+    case = [c for c in CASES if c['name'] == name][0]
+    case = dict(case, fwhm=1e-12)
+    o, rec, st = native_vs_oracle(walker, case, str(tmp_path), n_oracle, 400_000, seed=77)
+    assert_same_law(o, rec, st, polarised=not case['HG'])
+
+
+def test_polar_native_records_and_tallies_agree(walker, tmp_path):
+    """Every photon is recorded once with a finite, normalised-frame Stokes
+    vector (|Q|, |U|, |V| <= 1 for these tables) and the tallies equal sums
+    over the records."""
+    from monte_carlompi_amd.gpu import fractions_from_tallies
+    case = [c for c in CASES if c['name'] == 'pm_spectral_u'][0]
+    records, par, values, offsets, reff, tabs, bands = run_case(case, str(tmp_path))
+    g, pband = gpu_case(case, par, tabs, bands)
+    g.n_photons = 200_000
+    walker.set_cases([g])
+    walker.walk(mode='native', seed=5, records=True)
+    rec = walker.records()
+    st = walker.stokes()
+    assert walker.stats()['photons'] == 200_000
+    for f in ('Q_n', 'U_n', 'V_n'):
+        assert np.all(np.isfinite(st[f])) and np.all(np.abs(st[f]) <= 1.0 + 1e-5), f
+    fr = fractions_from_tallies(walker.tallies())
+    w = rec['wvn']
+    np.testing.assert_allclose(fr['albedo'][0], w[rec['condition'] == 1].sum() / w.sum(), rtol=1e-12)
+
+
+def test_polar_tables_are_validated(walker):
+    from monte_carlompi_amd._lib import McsError
+    from monte_carlompi_amd.gpu import Band, Case, PolarTables
+    x = np.linspace(-1, 1, 5)
+    Y = np.ones((6, 5))
+    base = dict(tau_tot=2.0, theta0_deg=0.0, n_photons=10, bands=[Band(wvl=1.0, g=0.8, ssa_ice=0.99, ext_cff=1e4)])
+    bad = [PolarTables(x=x[::-1], Y=[Y], cutoff=2),                   # grid not ascending
+           PolarTables(x=x, Y=[np.zeros((6, 5))], cutoff=2),          # P11 = 0
+           PolarTables(x=x, Y=[Y], cutoff=9),                         # cutoff off the grid
+           PolarTables(x=x, Y=[Y], cutoff=2, stokes0=(0, 1, 0, 0))]   # I = 0
+    for p in bad:
+        with pytest.raises(McsError) as e:
+            walker.set_cases([Case(polar=p, **base)])
+        assert 'MCS_ERR_ARG' in str(e.value)
+    walker.set_cases([Case(polar=PolarTables(x=x, Y=[Y], cutoff=2), **base)])
+    walker.walk(mode='native', seed=1, records=True)
+    assert walker.stats()['photons'] == 10

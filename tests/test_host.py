@@ -130,3 +130,22 @@ def test_bench_cli_parses_and_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode != 0 and 'one GPU' in (r.stdout + r.stderr)
+
+
+def test_pack_polar_layout():
+    """Polar cases -> mcs_polar_case rows and 7 x n tables (grid, P11..P44);
+    HG cases get table -1."""
+    import numpy as np
+    from monte_carlompi_amd.gpu import Band, Case, PolarTables, pack_polar
+    x = np.linspace(-1, 1, 4)
+    b = [Band(wvl=1.0, g=0.8, ssa_ice=0.99, ext_cff=1e4), Band(wvl=1.1, g=0.8, ssa_ice=0.99, ext_cff=1e4)]
+    Y0, Y1 = np.arange(24.).reshape(6, 4) + 1, np.arange(24.).reshape(6, 4) + 100
+    cases = [Case(tau_tot=1, theta0_deg=0, n_photons=3, bands=b[:1]),
+             Case(tau_tot=1, theta0_deg=0, n_photons=3, bands=b, polar=PolarTables(x, [Y0, Y1], 1, (1, 0.5, 0, 0)))]
+    parr, tabs, n = pack_polar(cases)
+    assert n == 4 and tabs.shape == (2, 7, 4)
+    assert parr[0].table == -1 and parr[1].table == 0 and parr[1].cutoff == 1
+    assert list(parr[1].stokes0) == [1.0, 0.5, 0.0, 0.0]
+    np.testing.assert_array_equal(tabs[1, 0], x)
+    np.testing.assert_array_equal(tabs[1, 1:], Y1)
+    assert pack_polar(cases[:1]) is None
