@@ -47,6 +47,7 @@ def main():
     ap.add_argument('--cpu-photons', type=int, default=10_000)
     ap.add_argument('--cpu-procs', type=int, default=0)
     ap.add_argument('--no-sweep', action='store_true', help='skip the batched sweep block')
+    ap.add_argument('--no-polar', action='store_true', help='skip the aspherical (phase-matrix) block')
     ap.add_argument('--launch', type=str, default='',
                     help='fixed launch shape block,blocks_per_cu,chunk,refill_min[,coop_live] (skips the '
                          'autotuner; for profiling runs)')
@@ -137,6 +138,8 @@ def main():
     }
     if not args.no_sweep:
         out['sweep'] = sweep_block(w)
+    if not args.no_polar:
+        out['polar'] = polar_block(w, args.no_cpu)
     w.close()
     if not args.no_cpu:
         from oracle import cpu_baseline
@@ -170,6 +173,57 @@ def profiled(name, key):
         return None
     with open(path) as fh:
         return json.load(fh).get(key)
+
+
+def polar_block(w, no_cpu):
+    """Aspherical grains: the snow_1m geometry with droxtal phase matrices
+    and polarised light, 1e6 photons, native FP32 walk (rejection sampling of
+    the phase matrix, Stokes vector per photon); the oracle restatement of the
+    reference's walk on a bounded sample beside it."""
+    import tempfile
+    import numpy as np
+    from monte_carlompi_amd.workloads import SNOW_POLAR, snow_polar_case
+    d = tempfile.mkdtemp(prefix='mc_polar_')
+    n = 1_000_000
+    case = snow_polar_case(n, d)
+    w.set_cases([case])
+    w.set_launch(1024, 1, 2048, 16, 0)
+    for k in range(2):
+        w.walk(mode='native', seed=900 + k, records=True)
+    times, steps = [], []
+    for k in range(5):
+        w.walk(mode='native', seed=k, records=True)
+        st = w.stats()
+        times.append(st['kernel_ms'])
+        steps.append(st['steps'])
+    rec = w.records()
+    sto = w.stokes()
+    ms = float(np.median(times))
+    refl = rec['condition'] == 1
+    out = {'workload': '1e6 photons, 1 m of snow, droxtal phase matrices (synthetic Yang-layout tables, '
+                       'r_eff 100 um, 1.30 um), theta0 60 deg, Lambertian bottom R 0.5, Stokes (1, 1, 0, 0)',
+           'photons': n, 'kernel_ms': round(ms, 3), 'photons_per_s': round(n / (ms * 1e-3), 1),
+           'photon_steps_per_s': round(float(np.mean(steps)) / (ms * 1e-3), 1),
+           'mean_steps': round(float(np.mean(steps)) / n, 2),
+           'albedo': round(float(np.mean(refl)), 5),
+           'reflected_mean_QUV': [round(float(np.mean(sto[k][refl])), 5) for k in ('Q_n', 'U_n', 'V_n')]}
+    if not no_cpu:
+        import time
+        from oracle import polar as opolar
+        from monte_carlompi_amd.workloads import RHO_ICE, RHO_SNW
+        from monte_carlompi_amd import optics
+        p = SNOW_POLAR
+        m = 1000
+        t0 = time.perf_counter()
+        opolar.polar_run(1, m, p['wvl'], 1e-12, p['r_eff'], optics.aspherical_dir(d, p['shape'], p['roughness'], p['wvl']),
+                         False, p['stokes0'], lambda w_: dict(ssa=np.zeros(w_.shape), ext=np.zeros(w_.shape)),
+                         0.0, case.tau_tot, RHO_SNW, RHO_ICE, p['theta0'], True, p['r_lambertian'])
+        dt = time.perf_counter() - t0
+        out['cpu_baseline'] = {'value': round(m / dt, 2), 'unit': 'photons/s', 'cores': 1, 'kind': 'port',
+                               'sample': '%d photons of the same case walked by oracle/polar.py (the reference\'s '
+                                         'per-step numpy structure, one core): %.1f s' % (m, dt)}
+        out['gpu_over_cpu'] = round(out['photons_per_s'] / out['cpu_baseline']['value'], 1)
+    return out
 
 
 def sweep_block(w):

@@ -86,6 +86,11 @@ struct DevBand {
     int32_t pm;
     double pA1, pB1, pA2, pB2;
     float fA1, fB1, fA2, fB2;
+    // native mode: |P12| <= P11 on the whole grid, so I S11 + S12 (Q cos 2phi
+    // + U sin 2phi) >= 0 for |(Q, U)| <= I and its cos(theta) marginal is S11:
+    // the walk samples it exactly by inversion (fM = integral of S11)
+    int32_t pm_exact;
+    float fM;
 };
 
 // probability -> 32-bit threshold (bits < t has probability p)
@@ -117,9 +122,10 @@ struct KParams {
     unsigned long long *ctl;     // [0] replay slice errors, [1] failed hand-overs
     uint32_t *rec_gid, *rec_nscat, *rec_cb;
     void *rec_theta, *rec_phi, *rec_path;
-    // phase-matrix tables (7 rows of pm_n: ascending cos theta, P11, P12, P22,
-    // P33, P43, P44), FP64 and FP32 copies; final Stokes (Q, U, V) per photon
-    // in photon order (3 x n_total, nullable)
+    // phase-matrix tables (rows of pm_n: ascending cos theta, P11, P12, P22,
+    // P33, P43, P44; FP64: 7 rows; FP32: 8 rows, the 8th the integral of the
+    // linear P11 from the first grid point); final Stokes (Q, U, V) per
+    // photon in photon order (3 x n_total, nullable)
     const double *pm;
     const float *pmf;
     int32_t pm_n;
@@ -448,9 +454,10 @@ struct NativeWalkerT {
         // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
         float rev = u01f(cur.y);
         float rev2 = 0.f;
+        int seg = -1;
         if constexpr (POLAR) {
-            if (pm >= 0) {   // aspherical: phase-matrix rejection sampler
-                polar_angles(P, cost, rev2);
+            if (pm >= 0) {   // aspherical: phase-matrix sampler
+                polar_angles(P, cost, rev2, seg);
                 rev = 0.5f * rev2;
             }
         }
@@ -476,7 +483,7 @@ struct NativeWalkerT {
             }
         }
         if constexpr (POLAR) {
-            if (pm >= 0) stokes_update(P, cost, rev2, muz, mz0);
+            if (pm >= 0) stokes_update(P, cost, rev2, muz, mz0, seg);
         }
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
@@ -521,7 +528,7 @@ struct NativeWalkerT {
     // ---- aspherical particles (POLAR only)
     __device__ __forceinline__ const float *pm_table(const KParams &P) const
     {
-        return P.pmf + (size_t)pm * 7 * (size_t)P.pm_n;
+        return P.pmf + (size_t)pm * 8 * (size_t)P.pm_n;
     }
 
     // Scattering angles of mc3D:905-1004: rejection sampling of
@@ -531,13 +538,55 @@ struct NativeWalkerT {
     // [0, 4 pi).  Attempt k of step i draws Philox block (photon, case, i,
     // 2 + k): area, cos theta, 2 phi, acceptance.  Returns cos theta and
     // 2 phi in revolutions.
-    __device__ void polar_angles(const KParams &P, float &cost, float &rev2) const
+    __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg) const
     {
         const uint32_t cs = case_id(P);
         const DevCase &C = P.cases[cs];
         const DevBand &B = P.bands[C.band_begin + band];
         const float *T = pm_table(P);
         const int n = P.pm_n;
+        seg = -1;
+        if (B.pm_exact) {
+            // The same law without the two-area rejection loop (whose
+            // acceptance is ~10 % on a forward-peaked P11): cos theta from
+            // the marginal S11 by inverting its piecewise-quadratic integral
+            // (one search over the 8th row), then 2 phi from
+            // 1 + (S12 / S11) (Q cos 2phi + U sin 2phi) by rejection against
+            // 1 + |S12 / S11| sqrt(Q^2 + U^2) (acceptance >= 1/2, no table
+            // reads).  Philox block (photon, case, i, 2 + k), k = 0, 1, ...
+            const float *cum = T + 7 * n;
+            const uint32_t ph = photon_id();
+            u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u}, P.key0, P.key1);
+            const float t = u01f(q.x) * B.fM;
+            const int j = grid_index(cum, n, t);
+            const float x0 = T[j], h = T[j + 1] - T[j];
+            const float y0 = T[n + j], y1 = T[n + j + 1];
+            const float slope = (y1 - y0) / h;
+            const float dm = fmaxf(t - cum[j], 0.f);
+            // y0 s + slope s^2 / 2 = dm, in the form stable for either sign of slope
+            const float s = 2.f * dm / (y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
+            const float ct = fminf(fmaxf(x0 + fminf(fmaxf(s, 0.f), h), T[0]), T[n - 1]);
+            const float w = (ct - x0) / h;
+            const float s11 = fmaf(w, y1 - y0, y0);
+            const float s12 = fmaf(w, T[2 * n + j + 1] - T[2 * n + j], T[2 * n + j]);
+            const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+            float r2 = 2.f * u01f(q.y), acc = u01f(q.z);
+            for (uint32_t k = 1;; ++k) {
+                const float val = fmaf(sI, s11, s12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
+                if (acc * bound <= val) break;
+                if (k >= POLAR_MAX_TRIES) {
+                    atomicAdd(P.ctl + 2, 1ull);
+                    break;
+                }
+                q = philox4x32_10(u32x4{ph, cs, i, 2u + k}, P.key0, P.key1);
+                r2 = 2.f * u01f(q.x);
+                acc = u01f(q.y);
+            }
+            cost = ct;
+            rev2 = r2;
+            seg = j;
+            return;
+        }
         const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
         const float ai = fabsf(sI);
         const float mv1 = fmaf(ai, B.fA1, B.fB1 * cq), mv2 = fmaf(ai, B.fA2, B.fB2 * cq);
@@ -578,7 +627,7 @@ struct NativeWalkerT {
     // (sign -1 unless phi_sca = pi, as in the reference; cos i_2 = 0 where
     // den = 0), normalise to I = 1.  The doubled angles of the return
     // rotation come from cos i_2 directly: cos 2i = 2c^2 - 1, sin 2i = 2c s.
-    __device__ void stokes_update(const KParams &P, float cost, float rev2, float muz_n, float mz0)
+    __device__ void stokes_update(const KParams &P, float cost, float rev2, float muz_n, float mz0, int seg)
     {
         const float *T = pm_table(P);
         const int n = P.pm_n;
@@ -586,7 +635,7 @@ struct NativeWalkerT {
         const float s2 = (rev2 > 1.f ? -1.f : 1.f) * __builtin_amdgcn_sinf(rev2);
         const float Qs = fmaf(sQ, c2, sU * s2);
         const float Us = fmaf(-sQ, s2, sU * c2);
-        const int j = grid_index(T, n, cost);
+        const int j = seg >= 0 ? seg : grid_index(T, n, cost);
         const float p11 = interp_at(T, T + n, n, j, cost);
         const float p12 = interp_at(T, T + 2 * n, n, j, cost);
         const float p22 = interp_at(T, T + 3 * n, n, j, cost);
@@ -623,14 +672,15 @@ struct NativeWalkerT {
     __device__ void polar_lambertian(const KParams &P, const u32x4 &cur, bool update)
     {
         float ct, rev2;
-        polar_angles(P, ct, rev2);
+        int seg;
+        polar_angles(P, ct, rev2, seg);
         (void)cur;
         const float cost = muz;
         const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
         const float rv = 0.5f * rev2;
         mux = sint * __builtin_amdgcn_cosf(rv);
         muy = sint * __builtin_amdgcn_sinf(rv);
-        if (update) stokes_update(P, cost, rev2, muz, 1.f);
+        if (update) stokes_update(P, cost, rev2, muz, 1.f, -1);
     }
 
     __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const
@@ -2216,6 +2266,14 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
                 if (!(a1 > 0) || !(a2 > 0)) return set_err(c, MCS_ERR_ARG, "table %d: P11 must be positive", t);
                 B.pA1 = a1; B.pB1 = b1; B.pA2 = a2; B.pB2 = b2;
                 B.fA1 = (float)a1; B.fB1 = (float)b1; B.fA2 = (float)a2; B.fB2 = (float)b2;
+                bool exact = true;
+                double mass = 0.0;
+                for (int k2 = 0; k2 < n; ++k2) {
+                    if (!(x[n + k2] >= 0.0) || std::fabs(x[2 * n + k2]) > x[n + k2]) exact = false;
+                    if (k2 + 1 < n) mass += 0.5 * (x[n + k2] + x[n + k2 + 1]) * (x[k2 + 1] - x[k2]);
+                }
+                B.pm_exact = (exact && mass > 0.0) ? 1 : 0;
+                B.fM = (float)mass;
             }
             any = true;
         }
@@ -2225,12 +2283,22 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     int r;
     if (any) {
         const size_t nv = (size_t)n_tables * 7 * n;
-        std::vector<float> tf(nv);
-        for (size_t k = 0; k < nv; ++k) tf[k] = (float)tables[k];
+        std::vector<float> tf((size_t)n_tables * 8 * n);
+        for (int t = 0; t < n_tables; ++t) {
+            const double *x = tables + (size_t)t * 7 * n;
+            float *f = tf.data() + (size_t)t * 8 * n;
+            for (int k = 0; k < 7 * n; ++k) f[k] = (float)x[k];
+            double cum = 0.0;   // integral of the linear P11 from x[0]
+            f[7 * n] = 0.f;
+            for (int k = 0; k + 1 < n; ++k) {
+                cum += 0.5 * (x[n + k] + x[n + k + 1]) * (x[k + 1] - x[k]);
+                f[7 * n + k + 1] = (float)cum;
+            }
+        }
         if ((r = ensure(c, c->d_pm, sizeof(double) * nv))) return r;
-        if ((r = ensure(c, c->d_pmf, sizeof(float) * nv))) return r;
+        if ((r = ensure(c, c->d_pmf, sizeof(float) * tf.size()))) return r;
         HIPCHK(c, hipMemcpyAsync(c->d_pm.p, tables, sizeof(double) * nv, hipMemcpyHostToDevice, c->stream));
-        HIPCHK(c, hipMemcpyAsync(c->d_pmf.p, tf.data(), sizeof(float) * nv, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->d_pmf.p, tf.data(), sizeof(float) * tf.size(), hipMemcpyHostToDevice, c->stream));
     }
     HIPCHK(c, hipMemcpyAsync(c->d_cases.p, dc.data(), sizeof(DevCase) * nc, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->d_bands.p, db.data(), sizeof(DevBand) * db.size(), hipMemcpyHostToDevice, c->stream));

@@ -14,10 +14,11 @@ Modes (keyword ``mode`` of ``run``, default from ``--mode`` / config):
   reference's draw order (``replay=(values, offsets)``), which reproduces the
   reference's records photon for photon.
 
-Scope: spheres / Henyey-Greenstein (with the Lambertian bottom and the
-Lambertian surface modes).  The aspherical phase-matrix path, Stokes vectors,
-debug trajectory plots and phase-function plots of the reference raise
-NotImplementedError.
+Scope: spheres / Henyey-Greenstein and aspherical grains (Yang et al. 2013
+tables: the HG phase function with HG=True, else the scattering phase matrix
+with Stokes vectors), with the Lambertian bottom and the Lambertian surface
+modes.  Debug trajectory plots and phase-function plots of the reference
+raise NotImplementedError.
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ import sys
 import numpy as np
 
 from . import optics
-from .gpu import Band, Case, GpuWalker
+from .gpu import Band, Case, GpuWalker, PolarTables
 from .parallelize import Parallel
 
 TWO_PIE = 2 * np.pi
@@ -53,6 +54,9 @@ class MonteCarlo(object):
                   'flg_3D', 'output_dir', 'optics_dir', 'fi_imp', 'HG', 'phase_functions', 'mode',
                   'seed', 'device', 'depth_m', 'ssa_ice', 'g', 'q_ext'):
             setattr(self, k if k not in ('ssa_ice', 'g') else '_param_' + k, d.get(k))
+        self.shape = 'sphere'                      # set by run() (mc3D:1509-1510)
+        self.roughness = 'smooth'
+        self.initial_stokes_params = np.array([1, 0, 0, 0])
         self._walker = None
 
     # ------------------------------------------------------------------ config
@@ -102,17 +106,26 @@ class MonteCarlo(object):
 
     # ------------------------------------------------------------------ output
     def setup_output(self, n_photon, wvl0, half_width):
-        """Output path ``output_dir/sphere/WVL_FWHM_REFF_N_THETA_HG[_i].txt``,
-        never overwriting (mc3D:96-143)."""
+        """Output path ``output_dir/sphere/WVL_FWHM_REFF_N_THETA_HG[_i].txt``
+        (aspherical grains: ``output_dir/<shape>/<roughness>/..._HG`` or
+        ``..._I<IQUV>``), never overwriting (mc3D:96-143)."""
         output_dir = self.output_dir
         if not os.path.isdir(output_dir):
             os.mkdir(output_dir)
-        save_dir = os.path.join(output_dir, 'sphere')
-        if not os.path.isdir(save_dir):
-            os.mkdir(save_dir)
+        if self.shape == 'sphere':
+            save_dir = os.path.join(output_dir, 'sphere')
+        else:
+            save_dir = os.path.join(output_dir, optics.SHAPE_DIRS[self.shape],
+                                    optics.ROUGHNESS_DIRS[self.roughness])
+        os.makedirs(save_dir, exist_ok=True)
         theta0_deg = np.rad2deg(self.theta_0)
+        if self.shape == 'sphere' or self.HG:
+            appendix = 'HG'
+        else:
+            I, Q, U, V = self.initial_stokes_params[:4]
+            appendix = 'I%d%d%d%d' % (I, Q, U, V)
         base = '%s_%s_%s_%s_%s_%s' % (wvl0, half_width, self.snow_effective_radius, n_photon,
-                                      theta0_deg, 'HG')
+                                      theta0_deg, appendix)
         output_file = os.path.join(save_dir, base + '.txt')
         i = 0
         while os.path.isfile(output_file):
@@ -130,9 +143,16 @@ class MonteCarlo(object):
         nsc = rec['n_scat'].tolist()
         cols = [rec[k].tolist() for k in ('wvn', 'theta_n', 'phi_n', 'path_length', 'snow_depth')]
         with open(path, 'w') as fh:
-            fh.write('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m]\n')
-            fh.writelines('%d %r %r %r %d %r %r\n' % (c, w, t, p, n, pl, sd) for c, w, t, p, n, pl, sd in
-                          zip(cond, cols[0], cols[1], cols[2], nsc, cols[3], cols[4]))
+            if 'Q_n' not in rec:
+                fh.write('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m]\n')
+                fh.writelines('%d %r %r %r %d %r %r\n' % (c, w, t, p, n, pl, sd) for c, w, t, p, n, pl, sd in
+                              zip(cond, cols[0], cols[1], cols[2], nsc, cols[3], cols[4]))
+                return
+            st = [rec[k].tolist() for k in ('Q_n', 'U_n', 'V_n')]     # mc3D:1637-1646
+            fh.write('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m], Q_n, '
+                     'U_n, V_n\n')
+            fh.writelines('%d %r %r %r %d %r %r %r %r %r\n' % r for r in
+                          zip(cond, cols[0], cols[1], cols[2], nsc, cols[3], cols[4], st[0], st[1], st[2]))
 
     # ------------------------------------------------------------------ optics
     def get_optical_properties(self, wvls, rds_snw):
@@ -170,9 +190,6 @@ class MonteCarlo(object):
         Mirrors mc3D:1492-1648; returns the output path (None if write=False).
         ``replay=(values, offsets)``: CSR uniform deviates of every photon in
         the reference's order (mode 'replay')."""
-        if shape != 'sphere' and not self.HG:
-            raise NotImplementedError('aspherical phase matrices / Stokes vectors are not built '
-                                      '(use shape="sphere" or HG=True)')
         if debug or self.phase_functions:
             raise NotImplementedError('debug trajectory and phase-function plots are not built')
         self.Lambertian_bottom = Lambertian_bottom
@@ -188,7 +205,17 @@ class MonteCarlo(object):
         par_wvls = Parallel(wvls)
         wvls = par_wvls.working_set
 
-        ssa_ice, ext_ice, g = self.get_optical_properties(wvls, rds_snw)
+        pm = None
+        if shape == 'sphere':
+            ssa_ice, ext_ice, g = self.get_optical_properties(wvls, rds_snw)
+        else:                                              # mc3D:1525-1545
+            a = optics.aspherical_ssps(wvls, rds_snw, wvl0, self.optics_dir, shape, roughness,
+                                       bool(self.HG), self.rho_ice)
+            wvls = a['wvls']                               # nearest table wavelengths
+            par_wvls.working_set = wvls
+            ssa_ice, ext_ice, g = a['ssa'], a['ext_cff_mss'], a['g']
+            self.snow_effective_radius = a['snow_effective_radius']
+            pm = a['pm']
         ssa_imp, ext_imp = self.get_impurity_optics(wvls)
         if test:                                           # mc3D:1553-1573
             for name in ('ssa_ice', 'ext_cff_mss_ice', 'g', 'ssa_imp', 'ext_cff_mss_imp'):
@@ -218,9 +245,14 @@ class MonteCarlo(object):
                       ext_cff=float(ext_cff_mss[first[j]] * self.rho_snw),
                       ssa_imp=float(ssa_imp[first[j]]), p_ext_imp=float(p_ext_imp[first[j]]),
                       weight=float(np.sum(band == j))) for j in range(uniq.size)]
+        polar = None
+        if pm is not None:                                 # mc3D:819-883, 1132
+            x = pm.interp_arrays(float(uniq[0]))[0]
+            polar = PolarTables(x=x, Y=[pm.interp_arrays(float(w))[1] for w in uniq],
+                                cutoff=pm.cutoff(wvl0, uniq), stokes0=tuple(float(v) for v in stokes_params))
         case = Case(tau_tot=tau_tot, theta0_deg=theta_0, n_photons=n_photon, bands=bands,
                     lambertian_bottom=Lambertian_bottom, r_lambertian=Lambertian_reflectance,
-                    lambertian_surface=Lambertian_surface)
+                    lambertian_surface=Lambertian_surface, polar=polar)
         w = self.walker()
         w.set_cases([case])
         mode = 'replay' if replay is not None else (self.mode or 'native')
@@ -229,6 +261,8 @@ class MonteCarlo(object):
         seed = self.seed if self.seed is not None else int(np.random.randint(0, 2**31 - 1))
         w.walk(mode=mode, seed=seed, records=True, photon_band=band.astype(np.uint8), replay=replay)
         rec = w.records()
+        if polar is not None:
+            rec.update(w.stokes())
         self.records = rec
         self.tallies = w.tallies()[0]
         self.stats = w.stats()

@@ -155,3 +155,28 @@ def write_synthetic_yang(optics_dir, shape='droxtal', roughness='smooth', wvls=Y
                                                                 'ssa', 'g')],
                              theta_deg=theta, elements={e: np.array(v) for e, v in els.items()})
     return d
+
+
+SNOW_POLAR = dict(wvl=1.30, r_eff=100.0, depth_m=1.0, theta0=60.0, lambertian_bottom=True,
+                  r_lambertian=0.5, shape='droxtal', roughness='smooth', stokes0=(1.0, 1.0, 0.0, 0.0))
+
+
+def snow_polar_case(n_photons: int, optics_dir: str) -> Case:
+    """The snow_1m geometry (1 m of snow at 300 kg/m^3, theta0 60 deg,
+    Lambertian bottom R 0.5) of aspherical droxtals (synthetic Yang-layout
+    tables written under ``optics_dir``, r_eff 100 um, 1.30 um) lit by
+    linearly polarised light (Stokes (1, 1, 0, 0)): the phase-matrix walk."""
+    from . import optics
+    from .gpu import PolarTables
+    p = SNOW_POLAR
+    write_synthetic_yang(optics_dir, shape=p['shape'], roughness=p['roughness'])
+    a = optics.aspherical_ssps(np.array([p['wvl']]), p['r_eff'], p['wvl'], optics_dir, p['shape'],
+                               p['roughness'], False, RHO_ICE)
+    key = float(a['wvls'][0])
+    ext_cff = float(a['ext_cff_mss'][0]) * RHO_SNW
+    x, Y = a['pm'].interp_arrays(key)
+    return Case(tau_tot=p['depth_m'] * ext_cff, theta0_deg=p['theta0'], n_photons=int(n_photons),
+                bands=[Band(wvl=key, g=float(a['g'][0]), ssa_ice=float(a['ssa'][0]), ext_cff=ext_cff)],
+                lambertian_bottom=p['lambertian_bottom'], r_lambertian=p['r_lambertian'],
+                polar=PolarTables(x=x, Y=[Y], cutoff=a['pm'].cutoff(p['wvl'], a['wvls']),
+                                  stokes0=p['stokes0']))

@@ -103,12 +103,38 @@ def test_brf_and_albedo_estimators():
         list(zip(g['condition'], g['wvn'])))
 
 
-def test_unsupported_reference_modes_raise():
-    mc = MonteCarlo()
-    with pytest.raises(NotImplementedError):
-        mc.run(10, 1.3, 0.085, 100, shape='droxtal')
+def test_unsupported_reference_modes_raise(tmp_path):
+    mc = MonteCarlo(optics_dir=str(tmp_path))
     with pytest.raises(NotImplementedError):
         mc.run(10, 1.3, 0.085, 100, debug=True)
+    # aspherical grains need the Yang-layout tables (the reference reads them too)
+    with pytest.raises(FileNotFoundError):
+        mc.run(10, 1.3, 0.085, 100, shape='droxtal')
+    with pytest.raises(ValueError):
+        mc.run(10, 15.5, 0.085, 100, shape='droxtal')     # between the two table bands
+
+
+def test_aspherical_output_naming(tmp_path):
+    """output_dir/<shape dir>/<roughness dir>/..._I<IQUV>.txt or ..._HG.txt
+    (mc3D:105-134) and the 10-column header with Q_n, U_n, V_n (mc3D:1637-1646)."""
+    import numpy as np
+    mc = MonteCarlo(output_dir=str(tmp_path / 'out'))
+    mc.theta_0 = np.pi / 6
+    mc.snow_effective_radius = 100.0000125
+    mc.shape, mc.roughness = 'hollow bullet rosette', 'moderately rough'
+    mc.initial_stokes_params = np.array([1, 0, 1, 0])
+    p = mc.setup_output(300, 1.3, 1e-12)
+    assert p.endswith('out/hollow_bullet_rosette/Rough003/1.3_1e-12_100.0000125_300_29.999999999999996_I1010.txt')
+    mc.HG = True
+    assert mc.setup_output(300, 1.3, 1e-12).endswith('_HG.txt')
+    rec = dict(condition=np.array([1, 4]), wvn=np.array([1 / 1.3, 1 / 1.3]), theta_n=np.array([0.5, 1.0]),
+               phi_n=np.array([1.0, 2.0]), n_scat=np.array([3, 0]), path_length=np.array([0.1, 0.2]),
+               snow_depth=np.array([1.0, 1.0]), Q_n=np.array([0.25, 0.0]), U_n=np.array([-0.5, 0.0]),
+               V_n=np.array([0.125, 0.0]))
+    MonteCarlo.write_records(p, rec)
+    lines = open(p).read().splitlines()
+    assert lines[0].endswith('snow_depth[m], Q_n, U_n, V_n')
+    assert lines[1].split()[-3:] == ['0.25', '-0.5', '0.125']
 
 
 def test_balanced_chunk_gives_every_block_the_same_number_of_chunks():
