@@ -129,7 +129,7 @@ struct KParams {
     const double *pm;
     const float *pmf;
     int32_t pm_n;
-    int32_t pad0;
+    int32_t pm_lds_floats;       // > 0: the FP32 tables are copied into dynamic LDS at block start
     void *rec_stokes;
     uint64_t n_total;
 };
@@ -228,7 +228,9 @@ struct NativeWalkerT {
     using Real = float;
     static constexpr bool COOP = !POLAR;      // has the wave-cooperative tail mode below
     static constexpr bool STOKES = POLAR;     // carries a Stokes vector (aspherical particles)
-    static constexpr int MAX_BLOCK = 1024;
+    // the polarised walk is built for 3 waves per SIMD (168 VGPRs), which
+    // caps its blocks at 512 threads
+    static constexpr int MAX_BLOCK = POLAR ? 512 : 1024;
     // k_walk is built for 5 (96 VGPRs) and 6 (80 VGPRs) waves per SIMD
     float z, mux, muy, muz, tau_acc;
     uint32_t i;
@@ -245,6 +247,8 @@ struct NativeWalkerT {
     // band's phase-matrix table (-1: Henyey-Greenstein)
     float sI, sQ, sU, sV;
     int pm;
+    bool pm_exact;               // the band's exact sampler applies (see DevBand)
+    float pm_mass;               // integral of S11 over the grid
 
     // Set up photon `photon` of case `case_id` and take its first step (or
     // its whole walk in the Lambertian surface mode); true when it is done.
@@ -260,7 +264,10 @@ struct NativeWalkerT {
         i = 0;
         cond = 0;
         if constexpr (POLAR) {
-            pm = C.polar ? P.bands[C.band_begin + bnd].pm : -1;
+            const DevBand &PB = P.bands[C.band_begin + bnd];
+            pm = C.polar ? PB.pm : -1;
+            pm_exact = PB.pm_exact != 0;   // per-photon constants stay in registers:
+            pm_mass = PB.fM;               // a step reads only the tables
             sI = C.s0f[0]; sQ = C.s0f[1]; sU = C.s0f[2]; sV = C.s0f[3];
         }
         if (C.surf) return surface_walk(P, C);
@@ -528,7 +535,9 @@ struct NativeWalkerT {
     // ---- aspherical particles (POLAR only)
     __device__ __forceinline__ const float *pm_table(const KParams &P) const
     {
-        return P.pmf + (size_t)pm * 8 * (size_t)P.pm_n;
+        extern __shared__ float pm_lds_tab[];
+        const size_t off = (size_t)pm * 8 * (size_t)P.pm_n;
+        return (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + off;
     }
 
     // Scattering angles of mc3D:905-1004: rejection sampling of
@@ -541,12 +550,10 @@ struct NativeWalkerT {
     __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg) const
     {
         const uint32_t cs = case_id(P);
-        const DevCase &C = P.cases[cs];
-        const DevBand &B = P.bands[C.band_begin + band];
         const float *T = pm_table(P);
         const int n = P.pm_n;
         seg = -1;
-        if (B.pm_exact) {
+        if (pm_exact) {
             // The same law without the two-area rejection loop (whose
             // acceptance is ~10 % on a forward-peaked P11): cos theta from
             // the marginal S11 by inverting its piecewise-quadratic integral
@@ -557,7 +564,7 @@ struct NativeWalkerT {
             const float *cum = T + 7 * n;
             const uint32_t ph = photon_id();
             u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u}, P.key0, P.key1);
-            const float t = u01f(q.x) * B.fM;
+            const float t = u01f(q.x) * pm_mass;
             const int j = grid_index(cum, n, t);
             const float x0 = T[j], h = T[j + 1] - T[j];
             const float y0 = T[n + j], y1 = T[n + j + 1];
@@ -587,6 +594,8 @@ struct NativeWalkerT {
             seg = j;
             return;
         }
+        const DevCase &C = P.cases[cs];
+        const DevBand &B = P.bands[C.band_begin + band];
         const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
         const float ai = fabsf(sI);
         const float mv1 = fmaf(ai, B.fA1, B.fB1 * cq), mv2 = fmaf(ai, B.fA2, B.fB2 * cq);
@@ -1737,6 +1746,13 @@ k_walk(const KParams P)
         unsigned long long *w64 = reinterpret_cast<unsigned long long *>(&bs);
         for (int k = threadIdx.x; k < (int)(sizeof(BlockState) / 8); k += blockDim.x) w64[k] = 0ull;
     }
+    if constexpr (W::STOKES && W::COOP == false && sizeof(typename W::Real) == 4) {
+        // polarised native walk: phase-matrix tables into dynamic LDS
+        if (P.pm_lds_floats > 0) {
+            extern __shared__ float pm_lds_tab[];
+            for (int k = threadIdx.x; k < P.pm_lds_floats; k += blockDim.x) pm_lds_tab[k] = P.pmf[k];
+        }
+    }
     __syncthreads();
     if (threadIdx.x == 0) bs.holders = blockDim.x >> 6;
     if (wave == 0) {
@@ -1966,6 +1982,7 @@ struct mcs_ctx {
     // aspherical particles: phase-matrix tables (FP64, FP32), final Stokes
     bool polar = false;
     int32_t pm_n = 0;
+    size_t pm_floats = 0;        // FP32 table floats (n_tables x 8 x pm_n)
     DevBuf d_pm, d_pmf, d_stokes;
     bool last_stokes = false;
     int last_mode = MCS_NATIVE;
@@ -2307,6 +2324,7 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     c->dbands.swap(db);
     c->polar = any;
     c->pm_n = any ? n : 0;
+    c->pm_floats = any ? (size_t)n_tables * 8 * n : 0;
     c->gkey = 0;
     return MCS_OK;
 }
@@ -2348,16 +2366,22 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                        (unsigned long long *)c->d_tally_f.p, (uint64_t)TF * nc,
                        (unsigned long long *)c->d_tally_u.p, (uint64_t)TU * nc,
                        (unsigned long long *)c->d_ctl.p, (uint64_t)16);
-    const int block = (mode == MCS_REPLAY) ? std::min(c->launch.block, ReplayWalker::MAX_BLOCK)
-                                           : c->launch.block;
+    int block = (mode == MCS_REPLAY) ? std::min(c->launch.block, ReplayWalker::MAX_BLOCK)
+                                     : c->launch.block;
     int grid = c->launch.blocks_per_cu * c->cus;
+    if (mode == MCS_NATIVE && c->polar && block > PolarWalker::MAX_BLOCK) {
+        grid *= block / PolarWalker::MAX_BLOCK;   // same threads per CU in smaller blocks
+        block = PolarWalker::MAX_BLOCK;
+    }
     if ((uint32_t)grid > n_chunks) grid = (int)std::max<uint32_t>(n_chunks, 1u);
     // events bracket the walk kernel alone: kernel_ms is its device time
     if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (mode == MCS_NATIVE && c->polar) {
-        // aspherical particles: one photon per lane (no cooperative tail)
-        hipLaunchKernelGGL((k_walk<PolarWalker, 4>), dim3(grid), dim3(block), 0, c->stream, P);
+        // aspherical particles: one photon per lane (no cooperative tail);
+        // tables up to 48 KB ride in LDS
+        const size_t lds = (size_t)P.pm_lds_floats * sizeof(float);
+        hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
     } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         // shapes asking for more than 5 waves per SIMD get the 80-VGPR instance
@@ -2509,6 +2533,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     P.pm = (const double *)c->d_pm.p;
     P.pmf = (const float *)c->d_pmf.p;
     P.pm_n = c->pm_n;
+    P.pm_lds_floats = (c->polar && c->pm_floats * sizeof(float) <= 48 * 1024) ? (int32_t)c->pm_floats : 0;
     P.n_total = (uint64_t)n;
     if (records && c->polar) {
         if ((r = ensure(c, c->d_stokes, 3 * rsz * (size_t)std::max<int64_t>(n, 1)))) return r;

@@ -51,6 +51,24 @@ def run_sweep(repeats=2, launch=(256, 6, 8192, 24)):
                                                                       st['steps']), flush=True)
 
 
+def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 0)):
+    """The aspherical-grain workload of bench.py (1e6 photons, droxtal phase
+    matrices, polarised light) `repeats` times (a workload to profile on its
+    own)."""
+    import tempfile
+    from .gpu import GpuWalker
+    from .workloads import snow_polar_case
+    case = snow_polar_case(1_000_000, tempfile.mkdtemp(prefix='mc_polar_'))
+    with GpuWalker(0) as w:
+        w.set_cases([case])
+        w.set_launch(*launch)
+        for k in range(repeats):
+            w.walk(mode='native', seed=1 + k, records=True)
+            st = w.stats()
+            print('polar %d: %.2f ms, %d photons, %d photon-steps' % (k, st['kernel_ms'], st['photons'],
+                                                                      st['steps']), flush=True)
+
+
 def valu_metrics(path, kernel, simds=1024, n_xcd=8, clock_hz=2.4e9):
     """VALU metrics of `kernel` per dispatch from one --pmc pass.  rocprofv3
     writes one row per counter per dispatch, summed over all XCDs, so the
@@ -125,6 +143,7 @@ def main():
     sub = ap.add_subparsers(dest='cmd', required=True)
     c = sub.add_parser('copy'); c.add_argument('--mib', type=int, default=256)
     sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
+    po = sub.add_parser('polar'); po.add_argument('--repeats', type=int, default=3)
     vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
     vm.add_argument('--json', help='write the per-dispatch medians here')
     s = sub.add_parser('stats'); s.add_argument('csv')
@@ -138,6 +157,8 @@ def main():
         copy_kernel(a.mib)
     elif a.cmd == 'sweep':
         run_sweep(a.repeats)
+    elif a.cmd == 'polar':
+        run_polar(a.repeats)
     elif a.cmd == 'valu':
         rows = valu_metrics(a.csv, a.kernel)
         for r in rows:
