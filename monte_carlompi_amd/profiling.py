@@ -51,14 +51,14 @@ def run_sweep(repeats=2, launch=(256, 6, 8192, 24)):
                                                                       st['steps']), flush=True)
 
 
-def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 0)):
+def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 0), photons=1_000_000):
     """The aspherical-grain workload of bench.py (1e6 photons, droxtal phase
     matrices, polarised light) `repeats` times (a workload to profile on its
     own)."""
     import tempfile
     from .gpu import GpuWalker
     from .workloads import snow_polar_case
-    case = snow_polar_case(1_000_000, tempfile.mkdtemp(prefix='mc_polar_'))
+    case = snow_polar_case(photons, tempfile.mkdtemp(prefix='mc_polar_'))
     with GpuWalker(0) as w:
         w.set_cases([case])
         w.set_launch(*launch)
@@ -144,6 +144,7 @@ def main():
     c = sub.add_parser('copy'); c.add_argument('--mib', type=int, default=256)
     sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
     po = sub.add_parser('polar'); po.add_argument('--repeats', type=int, default=3)
+    po.add_argument('--photons', type=int, default=1_000_000)
     vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
     vm.add_argument('--json', help='write the per-dispatch medians here')
     s = sub.add_parser('stats'); s.add_argument('csv')
@@ -158,7 +159,7 @@ def main():
     elif a.cmd == 'sweep':
         run_sweep(a.repeats)
     elif a.cmd == 'polar':
-        run_polar(a.repeats)
+        run_polar(a.repeats, photons=a.photons)
     elif a.cmd == 'valu':
         rows = valu_metrics(a.csv, a.kernel)
         for r in rows:
