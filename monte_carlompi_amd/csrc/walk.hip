@@ -53,6 +53,7 @@ constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED =
 // rejection attempts per scattering angle before the walk reports an error
 // (ctl[2]); valid tables accept within a few
 constexpr uint32_t POLAR_MAX_TRIES = 1u << 20;
+constexpr int PM_GUIDE = 256;   // guide cells of the FP32 phase-matrix tables
 
 struct DevCase {
     double tau_tot, mux0, muz0, r_lamb;
@@ -122,14 +123,19 @@ struct KParams {
     unsigned long long *ctl;     // [0] replay slice errors, [1] failed hand-overs
     uint32_t *rec_gid, *rec_nscat, *rec_cb;
     void *rec_theta, *rec_phi, *rec_path;
-    // phase-matrix tables (rows of pm_n: ascending cos theta, P11, P12, P22,
-    // P33, P43, P44; FP64: 7 rows; FP32: 8 rows, the 8th the integral of the
-    // linear P11 from the first grid point); final Stokes (Q, U, V) per
-    // photon in photon order (3 x n_total, nullable)
+    // phase-matrix tables.  FP64 (replay): 7 rows of pm_n -- ascending cos
+    // theta, P11, P12, P22, P33, P43, P44.  FP32 (native), pm_stride floats
+    // per table: those 7 rows, the integral of the linear P11 from the first
+    // grid point, the six elements' slopes per grid segment, and a guide of
+    // PM_GUIDE + 1 segment indices (int bits) where that integral crosses
+    // k / PM_GUIDE of its total.  Final Stokes (Q, U, V) per photon in photon
+    // order (3 x n_total, nullable).
     const double *pm;
     const float *pmf;
     int32_t pm_n;
     int32_t pm_lds_floats;       // > 0: the FP32 tables are copied into dynamic LDS at block start
+    int32_t pm_stride;
+    int32_t pad1;
     void *rec_stokes;
     uint64_t n_total;
 };
@@ -536,7 +542,7 @@ struct NativeWalkerT {
     __device__ __forceinline__ const float *pm_table(const KParams &P) const
     {
         extern __shared__ float pm_lds_tab[];
-        const size_t off = (size_t)pm * 8 * (size_t)P.pm_n;
+        const size_t off = (size_t)pm * (size_t)P.pm_stride;
         return (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + off;
     }
 
@@ -564,18 +570,28 @@ struct NativeWalkerT {
             const float *cum = T + 7 * n;
             const uint32_t ph = photon_id();
             u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u}, P.key0, P.key1);
-            const float t = u01f(q.x) * pm_mass;
-            const int j = grid_index(cum, n, t);
+            const float u = u01f(q.x);
+            const float t = u * pm_mass;
+            // the guide brackets the segment: search only between two entries
+            const int *guide = reinterpret_cast<const int *>(T + 14 * n);
+            const int gk = min((int)(u * (float)PM_GUIDE), PM_GUIDE - 1);
+            int lo = guide[gk], hi = guide[gk + 1] + 1;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (cum[mid] <= t) lo = mid; else hi = mid;
+            }
+            const int j = lo;
             const float x0 = T[j], h = T[j + 1] - T[j];
-            const float y0 = T[n + j], y1 = T[n + j + 1];
-            const float slope = (y1 - y0) / h;
+            const float y0 = T[n + j];
+            const float slope = T[8 * n + j];
             const float dm = fmaxf(t - cum[j], 0.f);
             // y0 s + slope s^2 / 2 = dm, in the form stable for either sign of slope
-            const float s = 2.f * dm / (y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
-            const float ct = fminf(fmaxf(x0 + fminf(fmaxf(s, 0.f), h), T[0]), T[n - 1]);
-            const float w = (ct - x0) / h;
-            const float s11 = fmaf(w, y1 - y0, y0);
-            const float s12 = fmaf(w, T[2 * n + j + 1] - T[2 * n + j], T[2 * n + j]);
+            const float s = 2.f * dm *
+                            __builtin_amdgcn_rcpf(y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
+            const float ds = fminf(fmaxf(s, 0.f), h);
+            const float ct = fminf(x0 + ds, T[n - 1]);
+            const float s11 = fmaf(slope, ct - x0, y0);
+            const float s12 = fmaf(T[9 * n + j], ct - x0, T[2 * n + j]);
             const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
             float r2 = 2.f * u01f(q.y), acc = u01f(q.z);
             for (uint32_t k = 1;; ++k) {
@@ -645,12 +661,14 @@ struct NativeWalkerT {
         const float Qs = fmaf(sQ, c2, sU * s2);
         const float Us = fmaf(-sQ, s2, sU * c2);
         const int j = seg >= 0 ? seg : grid_index(T, n, cost);
-        const float p11 = interp_at(T, T + n, n, j, cost);
-        const float p12 = interp_at(T, T + 2 * n, n, j, cost);
-        const float p22 = interp_at(T, T + 3 * n, n, j, cost);
-        const float p33 = interp_at(T, T + 4 * n, n, j, cost);
-        const float p43 = interp_at(T, T + 5 * n, n, j, cost);
-        const float p44 = interp_at(T, T + 6 * n, n, j, cost);
+        // linear in the segment (slopes precomputed), clamped to the grid
+        const float dx = fminf(fmaxf(cost, T[0]), T[n - 1]) - T[j];
+        const float p11 = fmaf(T[8 * n + j], dx, T[n + j]);
+        const float p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
+        const float p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
+        const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
+        const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
+        const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
         const float Ic = fmaf(sI, p11, Qs * p12);
         const float Qc = fmaf(sI, p12, Qs * p22);
         const float Uc = fmaf(Us, p33, -sV * p43);
@@ -660,12 +678,12 @@ struct NativeWalkerT {
         float ci = 0.f;
         if (den != 0.f) {
             const float sg = (rev2 == 1.f) ? 1.f : -1.f;
-            ci = fminf(fmaxf(fmaf(muz_n, cost, -mz0) / (sg * den), -1.f), 1.f);
+            ci = fminf(fmaxf(fmaf(muz_n, cost, -mz0) * __builtin_amdgcn_rcpf(sg * den), -1.f), 1.f);
         }
         const float si = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-ci, ci, 1.f), 0.f));
         const float cr = fmaf(2.f * ci, ci, -1.f);      // cos(-2 i_2)
         const float sr = -2.f * ci * si;               // sin(-2 i_2)
-        const float inv = 1.f / Ic;
+        const float inv = __builtin_amdgcn_rcpf(Ic);
         const float Qm = fmaf(Qc, cr, Uc * sr);
         const float Um = fmaf(-Qc, sr, Uc * cr);
         sI = 1.f;
@@ -1982,7 +2000,7 @@ struct mcs_ctx {
     // aspherical particles: phase-matrix tables (FP64, FP32), final Stokes
     bool polar = false;
     int32_t pm_n = 0;
-    size_t pm_floats = 0;        // FP32 table floats (n_tables x 8 x pm_n)
+    size_t pm_floats = 0;        // FP32 table floats (n_tables x pm_stride)
     DevBuf d_pm, d_pmf, d_stokes;
     bool last_stokes = false;
     int last_mode = MCS_NATIVE;
@@ -2300,17 +2318,27 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     int r;
     if (any) {
         const size_t nv = (size_t)n_tables * 7 * n;
-        std::vector<float> tf((size_t)n_tables * 8 * n);
+        const size_t stride = (size_t)14 * n + PM_GUIDE + 1;
+        std::vector<float> tf((size_t)n_tables * stride, 0.f);
+        std::vector<double> cum(n);
         for (int t = 0; t < n_tables; ++t) {
             const double *x = tables + (size_t)t * 7 * n;
-            float *f = tf.data() + (size_t)t * 8 * n;
+            float *f = tf.data() + (size_t)t * stride;
             for (int k = 0; k < 7 * n; ++k) f[k] = (float)x[k];
-            double cum = 0.0;   // integral of the linear P11 from x[0]
-            f[7 * n] = 0.f;
-            for (int k = 0; k + 1 < n; ++k) {
-                cum += 0.5 * (x[n + k] + x[n + k + 1]) * (x[k + 1] - x[k]);
-                f[7 * n + k + 1] = (float)cum;
+            cum[0] = 0.0;   // integral of the linear P11 from x[0]
+            for (int k = 0; k + 1 < n; ++k) cum[k + 1] = cum[k] + 0.5 * (x[n + k] + x[n + k + 1]) * (x[k + 1] - x[k]);
+            for (int k = 0; k < n; ++k) f[7 * n + k] = (float)cum[k];
+            for (int e = 0; e < 6; ++e)
+                for (int k = 0; k + 1 < n; ++k)
+                    f[(8 + e) * n + k] = (float)((x[(1 + e) * n + k + 1] - x[(1 + e) * n + k]) / (x[k + 1] - x[k]));
+            int *guide = reinterpret_cast<int *>(f + 14 * n);
+            int j = 0;
+            for (int g = 0; g <= PM_GUIDE; ++g) {
+                const double target = cum[n - 1] * g / PM_GUIDE;
+                while (j < n - 2 && cum[j + 1] <= target) ++j;
+                guide[g] = j;
             }
+            guide[PM_GUIDE] = n - 2;
         }
         if ((r = ensure(c, c->d_pm, sizeof(double) * nv))) return r;
         if ((r = ensure(c, c->d_pmf, sizeof(float) * tf.size()))) return r;
@@ -2324,7 +2352,7 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     c->dbands.swap(db);
     c->polar = any;
     c->pm_n = any ? n : 0;
-    c->pm_floats = any ? (size_t)n_tables * 8 * n : 0;
+    c->pm_floats = any ? (size_t)n_tables * ((size_t)14 * n + PM_GUIDE + 1) : 0;
     c->gkey = 0;
     return MCS_OK;
 }
@@ -2533,6 +2561,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     P.pm = (const double *)c->d_pm.p;
     P.pmf = (const float *)c->d_pmf.p;
     P.pm_n = c->pm_n;
+    P.pm_stride = c->pm_n * 14 + PM_GUIDE + 1;
     P.pm_lds_floats = (c->polar && c->pm_floats * sizeof(float) <= 48 * 1024) ? (int32_t)c->pm_floats : 0;
     P.n_total = (uint64_t)n;
     if (records && c->polar) {
