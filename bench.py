@@ -213,7 +213,7 @@ def polar_block(w, no_cpu):
         from monte_carlompi_amd.workloads import RHO_ICE, RHO_SNW
         from monte_carlompi_amd import optics
         p = SNOW_POLAR
-        m = 1000
+        m = 3000
         t0 = time.perf_counter()
         opolar.polar_run(1, m, p['wvl'], 1e-12, p['r_eff'], optics.aspherical_dir(d, p['shape'], p['roughness'], p['wvl']),
                          False, p['stokes0'], lambda w_: dict(ssa=np.zeros(w_.shape), ext=np.zeros(w_.shape)),
