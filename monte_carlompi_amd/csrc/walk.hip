@@ -229,6 +229,18 @@ __device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v
     return r;
 }
 
+// The state-independent part of one step of the polarised walk (exact
+// sampler): free path and absorption word from the step's main Philox block,
+// cos theta and the phase matrix there, the first 2 phi candidate and its
+// acceptance uniform.  One lane can build it for any step of any photon.
+struct PolarPacket {
+    float dtau;
+    uint32_t w;
+    float ct, r2, acc;
+    float p11, p12, p22, p33, p43, p44;
+    int seg;
+};
+
 template <bool POLAR>
 struct NativeWalkerT {
     using Real = float;
@@ -455,6 +467,16 @@ struct NativeWalkerT {
     // been absorbed
     __device__ __forceinline__ bool step(const KParams &P)
     {
+        if constexpr (POLAR) {
+            if (pm >= 0 && pm_exact) {
+                const u32x4 cur = r;
+                r = philox_step(pre, i + 2u, P.key0, P.key1);
+                const u32x4 pb = philox4x32_10(u32x4{photon_id(), case_id(P), i + 1u, 2u}, P.key0, P.key1);
+                PolarPacket k;
+                make_packet(cur, pb, pm_table(P), P.pm_n, pm_mass, k);
+                return polar_step(P, k);
+            }
+        }
         ++i;
         const u32x4 cur = r;
         r = philox_step(pre, i + 1u, P.key0, P.key1);
@@ -656,10 +678,6 @@ struct NativeWalkerT {
     {
         const float *T = pm_table(P);
         const int n = P.pm_n;
-        const float c2 = __builtin_amdgcn_cosf(rev2);
-        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * __builtin_amdgcn_sinf(rev2);
-        const float Qs = fmaf(sQ, c2, sU * s2);
-        const float Us = fmaf(-sQ, s2, sU * c2);
         const int j = seg >= 0 ? seg : grid_index(T, n, cost);
         // linear in the segment (slopes precomputed), clamped to the grid
         const float dx = fminf(fmaxf(cost, T[0]), T[n - 1]) - T[j];
@@ -669,6 +687,17 @@ struct NativeWalkerT {
         const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
+        stokes_apply(cost, rev2, muz_n, mz0, p11, p12, p22, p33, p43, p44);
+    }
+
+    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float muz_n, float mz0, float p11, float p12,
+                                                 float p22, float p33, float p43, float p44)
+    {
+#pragma clang fp contract(off)
+        const float c2 = __builtin_amdgcn_cosf(rev2);
+        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * __builtin_amdgcn_sinf(rev2);
+        const float Qs = fmaf(sQ, c2, sU * s2);
+        const float Us = fmaf(-sQ, s2, sU * c2);
         const float Ic = fmaf(sI, p11, Qs * p12);
         const float Qc = fmaf(sI, p12, Qs * p22);
         const float Uc = fmaf(Us, p33, -sV * p43);
@@ -690,6 +719,114 @@ struct NativeWalkerT {
         sQ = Qm * inv;
         sU = Um * inv;
         sV = Vc * inv;
+    }
+
+    // Packet of one step (see PolarPacket) from its main Philox block and its
+    // block (photon, case, step, 2), for a band with the exact sampler: cos
+    // theta by inverting the integral of the linear S11 (guided search), the
+    // six elements there.
+    __device__ static void make_packet(const u32x4 &mainb, const u32x4 &pb, const float *T, int n, float mass,
+                                       PolarPacket &o)
+    {
+        // no implicit contraction in the packet or the step that uses it: a
+        // packet built in another lane and read from LDS must give the same
+        // roundings as one built in place (explicit fmaf only)
+#pragma clang fp contract(off)
+        o.dtau = free_path(mainb);
+        o.w = mainb.w;
+        const float *cum = T + 7 * n;
+        const float u = u01f(pb.x);
+        const float t = u * mass;
+        const int *guide = reinterpret_cast<const int *>(T + 14 * n);
+        const int gk = min((int)(u * (float)PM_GUIDE), PM_GUIDE - 1);
+        int lo = guide[gk], hi = guide[gk + 1] + 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cum[mid] <= t) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const float x0 = T[j], h = T[j + 1] - T[j];
+        const float y0 = T[n + j];
+        const float slope = T[8 * n + j];
+        const float dm = fmaxf(t - cum[j], 0.f);
+        const float s = 2.f * dm *
+                        __builtin_amdgcn_rcpf(y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
+        const float ct = fminf(x0 + fminf(fmaxf(s, 0.f), h), T[n - 1]);
+        const float dx = ct - x0;
+        o.ct = ct;
+        o.seg = j;
+        o.p11 = fmaf(slope, dx, y0);
+        o.p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
+        o.p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
+        o.p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
+        o.p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
+        o.p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
+        o.r2 = 2.f * u01f(pb.y);
+        o.acc = u01f(pb.z);
+    }
+
+    // One step of an exact-sampler photon given its packet: 2 phi by
+    // rejection against the current Stokes vector (retries draw blocks
+    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
+    // one-lane walk and the tail mode below both step through here, so a
+    // photon's record does not depend on which of them walked it.
+    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        float r2 = k.r2, acc = k.acc;
+        const float bound = k.p11 + fabsf(k.p12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        for (uint32_t kk = 1;; ++kk) {
+            const float val = fmaf(sI, k.p11, k.p12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
+            if (acc * bound <= val) break;
+            if (kk >= POLAR_MAX_TRIES) {
+                atomicAdd(P.ctl + 2, 1ull);
+                break;
+            }
+            const u32x4 q = philox4x32_10(u32x4{photon_id(), case_id(P), i, 2u + kk}, P.key0, P.key1);
+            r2 = 2.f * u01f(q.x);
+            acc = u01f(q.y);
+        }
+        const float cost = k.ct;
+        const float rev = 0.5f * r2;
+        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
+        const float sinp = __builtin_amdgcn_sinf(rev);
+        const float cosp = __builtin_amdgcn_cosf(rev);
+        const float s2 = fmaf(-muz, muz, 1.f);
+        const float st = sint * __builtin_amdgcn_rsqf(s2);
+        const float mz0 = muz;
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        muz = fmaf(-s2 * st, cosp, muz * cost);
+        mux = gx; muy = gy;
+        if (s2 < 1e-7f) {
+            const float sg = mz0 > 0.f ? 1.f : -1.f;
+            mux = sint * cosp;
+            muy = sg * sint * sinp;
+            muz = sg * cost;
+        }
+        stokes_apply(cost, r2, muz, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        const float dtau = k.dtau;
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        if (!(z > 0.f || z < -tau_tot || k.w < t_abs_all)) return false;
+        if (z > 0.f) {                               // mc3D:1390-1397
+            tau_acc -= fminf(z / muz, dtau);
+            cond = 1;
+            return true;
+        }
+        if (z < -tau_tot) {                          // mc3D:1399-1459
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
+            z = -tau_tot;
+            if (lamb && reflects(P)) {
+                r = philox_step(pre, i + 1u, P.key0, P.key1);   // the re-emission step's block
+                return reemit(P);
+            }
+            cond = 2;
+            return true;
+        }
+        cond = (k.w < t_abs5) ? 5 : 4;
+        return true;
     }
 
     // Lambertian re-emission of an aspherical-grain photon (direction set by
@@ -1734,6 +1871,68 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
     }
 }
 
+// Tail mode of the polarised walk.  Once the queue is empty and a wave holds
+// at most 16 photons (all with the exact sampler), the wave works in rounds:
+// the G = 64 / 2^ceil(log2 L) lanes of group g build the packets of the next
+// G steps of the group's photon (Philox blocks, cos theta, phase matrix,
+// first phi candidate: everything that does not depend on the photon's
+// state), then each photon's own lane walks those G steps through
+// polar_step.  The serial chain of a step keeps only the phi acceptance,
+// rotation, Stokes update and events.  Records equal the one-lane walk's.
+// LDS after the tables: 16 owner lanes and 64 packets per wave.
+__device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wave)
+{
+    extern __shared__ float pm_lds_tab[];
+    int *const owners = reinterpret_cast<int *>(pm_lds_tab + P.pm_lds_floats);
+    PolarPacket *const pk = reinterpret_cast<PolarPacket *>(owners + (blockDim.x >> 6) * 16) + wave * 64;
+    int *const own = owners + wave * 16;
+    const int lane = lane_id();
+    for (;;) {
+        const unsigned long long live = __ballot(state == LIVE);
+        if (live == 0ull) break;
+        const int L = __popcll(live);
+        const int lg = (L <= 1) ? 0 : 32 - __clz(L - 1);
+        const int G = 64 >> lg;
+        if (state == LIVE) own[lane_rank(live)] = lane;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int g = lane >> (6 - lg);
+        const int o = own[g < L ? g : 0];
+        const uint32_t ck0 = bperm(o, w.pre.ck0), h0 = bperm(o, w.pre.h0), l0 = bperm(o, w.pre.l0);
+        const uint32_t io = bperm(o, w.i);
+        const int pmo = (int)bperm(o, (uint32_t)w.pm);
+        const float mass = bperm(o, w.pm_mass);
+        if (g < L) {
+            const PhiloxPre pre{ck0, h0, l0};
+            const uint32_t st = io + 1u + (uint32_t)(lane & (G - 1));
+            const u32x4 mainb = philox_step(pre, st, P.key0, P.key1);
+            const u32x4 pb = philox4x32_10(u32x4{l0 * 0x991A7CDBu, ck0 ^ P.key0, st, 2u}, P.key0, P.key1);
+            const float *T = (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + (size_t)pmo * (size_t)P.pm_stride;
+            PolarPacket q;
+            PolarWalker::make_packet(mainb, pb, T, P.pm_n, mass, q);
+            pk[lane] = q;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (state == LIVE) {
+            const int gme = lane_rank(live);
+            for (int k = 0; k < G; ++k) {
+                const uint32_t i0 = w.i;
+                const PolarPacket q = pk[gme * G + k];
+                const bool d = w.polar_step(P, q);
+                if (d || w.i >= P.max_steps) {
+                    if (!d) w.cond = 0;
+                    state = DONE;
+                    break;
+                }
+                if (w.i != i0 + 1u) break;   // a bottom re-emission took a step of its own
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
 // Wave issue priority from the oldest live photon (i = its step count, 0 for
 // other lanes): long walks set the tail of a launch, so their waves win issue
 // slots over young ones.
@@ -1816,9 +2015,21 @@ k_walk(const KParams P)
                 break;
             }
         }
+        constexpr bool PTAIL = W::STOKES && !W::COOP && sizeof(typename W::Real) == 4;
+        const int ptail = PTAIL ? min(P.coop_live, 16) : 0;
+        if constexpr (PTAIL) {
+            // polarised walk: queue empty, few photons, all with the exact sampler
+            if (ptail > 0 && lds_load(&bs.exhausted) && __popcll(live) <= ptail &&
+                __ballot(state == LIVE && !(w.pm >= 0 && w.pm_exact)) == 0ull) {
+                __builtin_amdgcn_s_setprio(3);
+                polar_tail(w, state, P, wave);
+                retire(bs, P, w, state, my_slot, my_gid);
+                break;
+            }
+        }
         // leave the hot loop once `thresh` lanes are not live (with the queue
         // empty: once the cooperative tail can take over)
-        const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - P.coop_live : 64)
+        const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - P.coop_live : 64 - ptail)
                                                    : min(64, (64 - __popcll(live)) + (int)P.refill_min);
         // ---------------- hot: step every live photon
         for (uint32_t iter = 1;; ++iter) {
@@ -2406,9 +2617,11 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     if (events && hipEventRecord(c->ev0, c->stream) != hipSuccess)
         return set_err(c, MCS_ERR_HIP, "hipEventRecord failed");
     if (mode == MCS_NATIVE && c->polar) {
-        // aspherical particles: one photon per lane (no cooperative tail);
-        // tables up to 48 KB ride in LDS
-        const size_t lds = (size_t)P.pm_lds_floats * sizeof(float);
+        // aspherical particles: one photon per lane, then the tail mode of
+        // polar_tail; tables up to 16 KB ride in LDS
+        // + the tail mode's owner lanes and packets (16 ints and 64 packets a wave)
+        const size_t lds = (size_t)P.pm_lds_floats * sizeof(float) +
+                           (size_t)(block / 64) * (16 * sizeof(int) + 64 * sizeof(PolarPacket));
         hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
     } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
@@ -2562,7 +2775,7 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     P.pmf = (const float *)c->d_pmf.p;
     P.pm_n = c->pm_n;
     P.pm_stride = c->pm_n * 14 + PM_GUIDE + 1;
-    P.pm_lds_floats = (c->polar && c->pm_floats * sizeof(float) <= 48 * 1024) ? (int32_t)c->pm_floats : 0;
+    P.pm_lds_floats = (c->polar && c->pm_floats * sizeof(float) <= 16 * 1024) ? (int32_t)c->pm_floats : 0;
     P.n_total = (uint64_t)n;
     if (records && c->polar) {
         if ((r = ensure(c, c->d_stokes, 3 * rsz * (size_t)std::max<int64_t>(n, 1)))) return r;

@@ -157,3 +157,60 @@ def test_polar_tables_are_validated(walker):
     walker.set_cases([Case(polar=PolarTables(x=x, Y=[Y], cutoff=2), **base)])
     walker.walk(mode='native', seed=1, records=True)
     assert walker.stats()['photons'] == 10
+
+
+def test_polar_and_hg_cases_in_one_launch(walker, tmp_path):
+    """A launch mixing an aspherical case and a sphere (HG) case runs the
+    polarised kernel for both: each case keeps the law of its own launch
+    (exit fractions within 4 sigma), HG photons read Stokes (0, 0, 0), and a
+    hipGraph replay of the mixed launch gives the same records as the eager
+    launch with coop off (per-photon Philox streams)."""
+    from monte_carlompi_amd.workloads import snow_1m_case, snow_polar_case
+    n = 200_000
+    pc = snow_polar_case(n, str(tmp_path))
+    hc = snow_1m_case(n)
+    hc.tau_tot = 20.0                          # short walks
+    pc.tau_tot = 20.0
+    walker.set_launch(512, 2, 2048, 16, 0)
+    fr = {}
+    for name, cases in (('polar', [pc]), ('hg', [hc]), ('mixed', [pc, hc])):
+        walker.set_cases(cases)
+        walker.walk(mode='native', seed=11, records=True)
+        rec = walker.records()
+        fr[name] = [np.bincount(rec['condition'][k * n:(k + 1) * n], minlength=6) / n for k in range(len(cases))]
+        if name == 'mixed':
+            st = walker.stokes()
+            assert np.all(st['Q_n'][n:] == 0) and np.all(st['U_n'][n:] == 0) and np.all(st['V_n'][n:] == 0)
+            assert np.any(st['Q_n'][:n] != 0)
+            eager = rec
+            walker.walk(mode='native', seed=11, records=True, use_graph=True)
+            walker.walk(mode='native', seed=11, records=True, use_graph=True)
+            g = walker.records()
+            for f in ('condition', 'n_scat', 'theta_n', 'phi_n', 'path_length'):
+                np.testing.assert_array_equal(g[f], eager[f], err_msg=f)
+    for k, ref in ((0, fr['polar'][0]), (1, fr['hg'][0])):
+        got = fr['mixed'][k]
+        sd = np.sqrt(2 * ref * (1 - ref) / n) + 1e-12
+        assert np.all(np.abs(got - ref) <= 4 * sd + 1e-9), (k, got, ref)
+    walker.set_launch(1024, 1, 2048, 16, 16)
+
+
+def test_polar_tail_mode_gives_the_one_lane_records(walker, tmp_path):
+    """The tail mode (idle lanes build the packets of a photon's next steps)
+    walks every photon through the same step code on the same Philox blocks:
+    records and Stokes vectors equal the one-lane walk's bit for bit, for any
+    launch shape."""
+    from monte_carlompi_amd.workloads import snow_polar_case
+    case = snow_polar_case(150_000, str(tmp_path))
+    walker.set_cases([case])
+    runs = []
+    for shape in [(512, 2, 2048, 16, 0), (512, 2, 2048, 16, 16), (256, 4, 1024, 8, 8), (1024, 1, 600, 16, 16)]:
+        walker.set_launch(*shape)
+        walker.walk(mode='native', seed=21, records=True)
+        rec = walker.records()
+        rec.update(walker.stokes())
+        runs.append(rec)
+    for r in runs[1:]:
+        for f in ('condition', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'Q_n', 'U_n', 'V_n'):
+            np.testing.assert_array_equal(r[f], runs[0][f], err_msg=f)
+    walker.set_launch(1024, 1, 2048, 16, 16)
