@@ -2951,6 +2951,39 @@ int mcs_last_stats(const mcs_ctx *c, mcs_stats *out)
     return MCS_OK;
 }
 
+// Put back the phase-matrix state of the cases (mcs_set_cases clears it):
+// the polar fields of the case and band rows, re-uploaded; the device tables
+// themselves are untouched by mcs_set_cases.
+static int restore_polar(mcs_ctx *c, const std::vector<DevCase> &dc, const std::vector<DevBand> &db, bool polar,
+                         int32_t pm_n, size_t pm_floats)
+{
+    if (!polar) return MCS_OK;
+    for (size_t k = 0; k < c->dcases.size(); ++k) {
+        DevCase &D = c->dcases[k];
+        const DevCase &S = dc[k];
+        D.polar = S.polar;
+        for (int q = 0; q < 4; ++q) { D.s0[q] = S.s0[q]; D.s0f[q] = S.s0f[q]; }
+        D.cmax = S.cmax; D.ccut = S.ccut; D.cmin = S.cmin;
+        D.fcmax = S.fcmax; D.fccut = S.fccut; D.fcmin = S.fcmin;
+    }
+    for (size_t k = 0; k < c->dbands.size(); ++k) {
+        DevBand &B = c->dbands[k];
+        const DevBand &S = db[k];
+        B.pm = S.pm; B.pm_exact = S.pm_exact; B.fM = S.fM;
+        B.pA1 = S.pA1; B.pB1 = S.pB1; B.pA2 = S.pA2; B.pB2 = S.pB2;
+        B.fA1 = S.fA1; B.fB1 = S.fB1; B.fA2 = S.fA2; B.fB2 = S.fB2;
+    }
+    HIPCHK(c, hipMemcpyAsync(c->d_cases.p, c->dcases.data(), sizeof(DevCase) * c->dcases.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->d_bands.p, c->dbands.data(), sizeof(DevBand) * c->dbands.size(), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->polar = true;
+    c->pm_n = pm_n;
+    c->pm_floats = pm_floats;
+    return MCS_OK;
+}
+
 int mcs_autotune(mcs_ctx *c, const mcs_launch *cand, int32_t n, int64_t photons_cap, int32_t repeats,
                  uint64_t seed, double *times_ms, mcs_launch *best)
 {
@@ -2963,8 +2996,16 @@ int mcs_autotune(mcs_ctx *c, const mcs_launch *cand, int32_t n, int64_t photons_
     if (photons_cap > 0)
         for (auto &k : capped) k.n_photons = std::min<int64_t>(k.n_photons, photons_cap);
     const mcs_launch keep = c->launch;
+    // the phase-matrix state survives the capped copy (aspherical cases are
+    // timed with their own kernel)
+    const std::vector<DevCase> pdc = c->dcases;
+    const std::vector<DevBand> pdb = c->dbands;
+    const bool polar = c->polar;
+    const int32_t pm_n = c->pm_n;
+    const size_t pm_floats = c->pm_floats;
     int r = mcs_set_cases(c, capped.data(), (int32_t)capped.size(), bands.data(), (int32_t)bands.size());
     if (r) return r;
+    if ((r = restore_polar(c, pdc, pdb, polar, pm_n, pm_floats))) return r;
     double best_t = 1e300;
     int best_i = -1;
     for (int k = 0; k < n; ++k) {
@@ -2988,6 +3029,7 @@ int mcs_autotune(mcs_ctx *c, const mcs_launch *cand, int32_t n, int64_t photons_
         if (med < best_t) { best_t = med; best_i = k; }
     }
     int r2 = mcs_set_cases(c, saved.data(), (int32_t)saved.size(), bands.data(), (int32_t)bands.size());
+    if (!r2) r2 = restore_polar(c, pdc, pdb, polar, pm_n, pm_floats);
     if (r) { c->launch = keep; return r; }
     if (r2) return r2;
     *best = cand[best_i];

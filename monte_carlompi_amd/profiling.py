@@ -51,7 +51,7 @@ def run_sweep(repeats=2, launch=(256, 6, 8192, 24)):
                                                                       st['steps']), flush=True)
 
 
-def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 0), photons=1_000_000):
+def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 16), photons=1_000_000):
     """The aspherical-grain workload of bench.py (1e6 photons, droxtal phase
     matrices, polarised light) `repeats` times (a workload to profile on its
     own)."""

@@ -214,3 +214,25 @@ def test_polar_tail_mode_gives_the_one_lane_records(walker, tmp_path):
         for f in ('condition', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'Q_n', 'U_n', 'V_n'):
             np.testing.assert_array_equal(r[f], runs[0][f], err_msg=f)
     walker.set_launch(1024, 1, 2048, 16, 16)
+
+
+def test_autotune_keeps_the_phase_matrices(walker, tmp_path):
+    """mcs_autotune re-uploads capped copies of the cases: the polar state
+    must survive it (the candidates are timed on the polarised kernel and the
+    walk afterwards still carries Stokes vectors)."""
+    from monte_carlompi_amd.workloads import snow_polar_case
+    case = snow_polar_case(100_000, str(tmp_path))
+    walker.set_cases([case])
+    walker.set_launch(512, 2, 1024, 16, 16)
+    walker.walk(mode='native', seed=3, records=True)
+    ref = walker.records()
+    ref.update(walker.stokes())
+    shape, t = walker.autotune([(512, 2, 1024, 16, 16), (512, 2, 2048, 16, 8)], photons_cap=20_000, repeats=1, seed=5)
+    walker.set_launch(512, 2, 1024, 16, 16)
+    walker.walk(mode='native', seed=3, records=True)
+    got = walker.records()
+    got.update(walker.stokes())
+    for f in ('condition', 'n_scat', 'theta_n', 'Q_n', 'U_n', 'V_n'):
+        np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
+    assert np.any(got['Q_n'] != 0)
+    walker.set_launch(1024, 1, 2048, 16, 16)
