@@ -404,7 +404,7 @@ struct NativeWalkerT {
             // the step after a bottom reflection still draws its angles from
             // the phase matrix (the azimuth is used), then resets the Stokes
             // vector (mc3D:1238-1285)
-            if (pm >= 0) polar_lambertian(P, cur, false);
+            if (pm >= 0) polar_lambertian(P, false);
             sI = 1.f; sQ = 0.f; sU = 0.f; sV = 0.f;
         }
         const float dtau = free_path(cur);
@@ -446,7 +446,7 @@ struct NativeWalkerT {
                 // Lambertian surface of aspherical grains: azimuth from the
                 // phase matrix, Stokes vector scattered at the Lambertian
                 // angle (mc3D:1238-1250, 1287-1342)
-                if (pm >= 0) polar_lambertian(P, cur, true);
+                if (pm >= 0) polar_lambertian(P, true);
             }
             const float dtau = free_path(cur);
             z = fmaf(dtau, muz, z);
@@ -829,16 +829,15 @@ struct NativeWalkerT {
         return true;
     }
 
-    // Lambertian re-emission of an aspherical-grain photon (direction set by
-    // lambertian_direction from cur): the azimuth is the phase-matrix
+    // Lambertian re-emission of an aspherical-grain photon (cos theta already
+    // set by lambertian_direction): the azimuth is the phase-matrix
     // sampler's (the reference keeps phi_rand there); with `update` the
     // Stokes vector scatters at the Lambertian angle from mz0 = 1.
-    __device__ void polar_lambertian(const KParams &P, const u32x4 &cur, bool update)
+    __device__ void polar_lambertian(const KParams &P, bool update)
     {
         float ct, rev2;
         int seg;
         polar_angles(P, ct, rev2, seg);
-        (void)cur;
         const float cost = muz;
         const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
         const float rv = 0.5f * rev2;
