@@ -254,7 +254,7 @@ int mcs_philox(mcs_ctx *ctx, const uint32_t *ctr, uint32_t k0, uint32_t k1,
  * trained and served with hand-written BF16 MFMA GEMM kernels (FP32
  * accumulate) and an FP32 Adam.  No reference counterpart: the reference
  * always runs the Monte Carlo; these entry points serve cases the plain
- * walk would otherwise have to simulate, and the walk stays the judge.
+ * walk would otherwise have to simulate, and the walk stays the arbiter of accuracy.
  *
  * Parameter layout (flat float32, unpadded): for layer l = 0..n_hidden_layers
  * the weight W_l [rows_l x cols_l] row-major then its bias b_l [rows_l];
