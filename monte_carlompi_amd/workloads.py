@@ -93,6 +93,7 @@ def spectral_bands(wvl0: float, fwhm: float, max_bands: int = 64, tail: float = 
 # (phase-matrix + Stokes) walk can run: an ice-like P11 (forward peak, 22 deg
 # halo, weak backscatter) and smooth, bounded ratios P12/P11 .. P44/P11.
 YANG_WVLS = np.round(np.arange(0.30, 2.0001, 0.05), 2)
+YANG_WVLS_FAR_IR = np.round(np.arange(16.5, 30.001, 0.5), 2)
 YANG_REFF = (50.0, 100.0, 200.0)
 
 
@@ -128,11 +129,13 @@ def synthetic_phase_matrix(theta_deg, wvl, reff):
 
 def write_synthetic_yang(optics_dir, shape='droxtal', roughness='smooth', wvls=YANG_WVLS, reffs=YANG_REFF):
     """Write isca.dat and P11..P44.dat under
-    ``optics_dir/ice_optics/0.2-15.25/<shape>/<roughness>``: rows ordered by
-    wavelength then size; the projected area and volume give r_eff = 3/4 V / A
-    exactly ``reffs``.  Returns the directory."""
+    ``optics_dir/ice_optics/<band>/<shape>/<roughness>`` (band 0.2-15.25 or,
+    for far-infrared ``wvls``, 16.4-99.0): rows ordered by wavelength then
+    size; the projected area and volume give r_eff = 3/4 V / A exactly
+    ``reffs``.  Returns the directory."""
     from . import optics
-    d = os.path.join(optics_dir, 'ice_optics', '0.2-15.25', optics.SHAPE_DIRS[shape],
+    band = '16.4-99.0' if float(np.min(wvls)) >= 16.4 else '0.2-15.25'
+    d = os.path.join(optics_dir, 'ice_optics', band, optics.SHAPE_DIRS[shape],
                      optics.ROUGHNESS_DIRS[roughness])
     theta = yang_theta_grid()
     rows = dict(wvl=[], max_dim=[], volume=[], area=[], q_ext=[], ssa=[], g=[])
@@ -145,7 +148,8 @@ def write_synthetic_yang(optics_dir, shape='droxtal', roughness='smooth', wvls=Y
             rows['volume'].append(area * r / 0.75)
             rows['area'].append(area)
             rows['q_ext'].append(2.0 + 0.02 * np.sin(3.0 * w))
-            rows['ssa'].append(1.0 - 2e-4 * (1.0 + 8.0 * w * w) * (r / 100.0))
+            ab = 2e-4 * (1.0 + 8.0 * w * w) * (r / 100.0)      # stronger absorption in the infrared
+            rows['ssa'].append(1.0 - ab if ab < 0.05 else 0.95 - 0.2 * ab / (1.0 + ab))
             el = synthetic_phase_matrix(theta, w, r)
             mu = np.cos(np.deg2rad(theta))
             rows['g'].append(float(0.5 * np.trapezoid((mu * el['P11'])[::-1], mu[::-1])))

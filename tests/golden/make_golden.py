@@ -230,6 +230,13 @@ POLAR_CASES = [
     dict(name='asph_hg', seed=2007, n=400, wvl0=1.3, fwhm=0.085, rds=100, theta0=30.0, lb=True,
          R=0.5, tau=6.0, imp_cnc=0.0, shape='hexagonal plate', rough='severely rough', HG=True,
          stokes=[1, 0, 0, 0]),
+    # far infrared (the 16.4-99.0 um tables), another shape and roughness
+    dict(name='pm_far_ir', seed=2008, n=250, wvl0=20.0, fwhm=1.0, rds=200, theta0=40.0, lb=True,
+         R=0.6, tau=3.0, imp_cnc=0.0, shape='solid bullet rosette', rough='moderately rough', HG=False,
+         stokes=[1, 0, 1, 0], far_ir=True),
+    dict(name='asph_hg_column', seed=2009, n=300, wvl0=0.5, fwhm=0.05, rds=50, theta0=0.0, lb=False,
+         R=0.0, tau=4.0, imp_cnc=0.0, shape='hollow hexagonal column', rough='smooth', HG=True,
+         stokes=[1, 0, 0, 0]),
 ]
 _INT = re.compile(r'np\.int64\(([^)]*)\)')
 
@@ -258,7 +265,8 @@ def polar_main():
         optics = os.path.join(tmp, 'optics_' + c['name'])
         os.makedirs(os.path.join(optics, 'mie', 'snicar'))
         write_nc(os.path.join(optics, 'mie', 'snicar', 'imp.nc'), imp_table(0.3, 12000.0))
-        workloads.write_synthetic_yang(optics, shape=c['shape'], roughness=c['rough'])
+        workloads.write_synthetic_yang(optics, shape=c['shape'], roughness=c['rough'],
+                                       wvls=workloads.YANG_WVLS_FAR_IR if c.get('far_ir') else workloads.YANG_WVLS)
         outdir = os.path.join(tmp, 'out_' + c['name'])
         np.random.seed(c['seed'])
         mc = monte_carlo3D.MonteCarlo(tau_tot=c['tau'], imp_cnc=c['imp_cnc'], output_dir=outdir,

@@ -26,4 +26,5 @@ def table_dir(root, case):
     """Write the synthetic Yang-layout tables the case was run on; returns
     the directory of its shape and roughness."""
     from monte_carlompi_amd import workloads
-    return workloads.write_synthetic_yang(root, shape=case['shape'], roughness=case['rough'])
+    wvls = workloads.YANG_WVLS_FAR_IR if case.get('far_ir') else workloads.YANG_WVLS
+    return workloads.write_synthetic_yang(root, shape=case['shape'], roughness=case['rough'], wvls=wvls)
