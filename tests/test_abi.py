@@ -38,6 +38,35 @@ def test_struct_sizes_match_header():
     assert ctypes.sizeof(_lib.McsBand) == 64
     assert ctypes.sizeof(_lib.McsWalkArgs) == 56
     assert ctypes.sizeof(_lib.McsLaunch) == 20
+    assert ctypes.sizeof(_lib.McsPolarCase) == 40
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof / offsetof of every ABI struct from the C compiler on
+    include/mcsnow.h equal the ctypes mirrors."""
+    import shutil
+    import subprocess
+    if shutil.which('gcc') is None:
+        pytest.skip('no C compiler')
+    structs = {'mcs_case': _lib.McsCase, 'mcs_band': _lib.McsBand, 'mcs_walk_args': _lib.McsWalkArgs,
+               'mcs_launch': _lib.McsLaunch, 'mcs_records': _lib.McsRecords, 'mcs_stats': _lib.McsStats,
+               'mcs_polar_case': _lib.McsPolarCase}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mcsnow.h"', 'int main(void) {']
+    for cname, cls in structs.items():
+        lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in cls._fields_:
+            cf = 'async' if f == 'async_' else f
+            lines.append('printf("%s.%s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, cf))
+    lines.append('return 0; }')
+    src = tmp_path / 'layout.c'
+    src.write_text('\n'.join(lines))
+    exe = tmp_path / 'layout'
+    subprocess.check_call(['gcc', '-I', os.path.join(ROOT, 'include'), str(src), '-o', str(exe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)]).decode().splitlines())
+    for cname, cls in structs.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got['%s.%s' % (cname, f)]) == getattr(cls, f).offset, (cname, f)
     assert ctypes.sizeof(_lib.McsRecords) == 48
     assert ctypes.sizeof(_lib.McsStats) == 40
 
