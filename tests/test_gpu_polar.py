@@ -240,3 +240,39 @@ def test_autotune_keeps_the_phase_matrices(walker, tmp_path):
         np.testing.assert_array_equal(got[f], ref[f], err_msg=f)
     assert np.any(got['Q_n'] != 0)
     walker.set_launch(1024, 1, 2048, 16, 16)
+
+
+@pytest.mark.parametrize('cap', [2, 7, 40])
+def test_polar_step_cap_with_lambertian_reemission(walker, tmp_path, cap):
+    """Aspherical grains over a white Lambertian bottom in a thin layer
+    (frequent re-emissions, each taking a step of its own): no record
+    exceeds max_steps, capped photons stop at exactly max_steps, in the
+    one-lane walk and in the tail mode (one chunk per wave), and the two
+    give the same records."""
+    from monte_carlompi_amd._lib import McsError
+    from monte_carlompi_amd.workloads import snow_polar_case
+    case = snow_polar_case(64 * walker.cus, str(tmp_path))
+    case.tau_tot, case.r_lambertian = 0.5, 1.0
+    walker.set_cases([case])
+    runs = []
+    try:
+        for coop in (0, 16):
+            walker.set_launch(64, 1, 64, 1, coop)
+            try:
+                walker.walk(mode='native', seed=4, records=True, max_steps=cap)
+            except McsError as e:
+                assert 'STEP_CAP' in str(e)
+            r = walker.records()
+            r.update(walker.stokes())
+            c = walker.counts()[0]
+            assert c[0] == case.n_photons
+            assert np.all(r['n_scat'] <= cap - 1)
+            capped = r['condition'] == 0
+            assert int(capped.sum()) == int(c[6])
+            assert np.all(r['n_scat'][capped] == cap - 1)
+            assert int(c[8]) == int(np.sum(r['n_scat'] + 1))
+            runs.append(r)
+        for f in ('condition', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'Q_n', 'U_n', 'V_n'):
+            np.testing.assert_array_equal(runs[1][f], runs[0][f], err_msg=f)
+    finally:
+        walker.set_launch()
