@@ -504,7 +504,7 @@ struct NativeWalkerT {
         // issues none of its instructions
         const float s2 = fmaf(-muz, muz, 1.f);
         const float st = sint * __builtin_amdgcn_rsqf(s2);
-        const float mz0 = muz;
+        const float mz0 = muz, mx0 = mux, my0 = muy;
         const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
         const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
         muz = fmaf(-s2 * st, cosp, muz * cost);
@@ -518,7 +518,7 @@ struct NativeWalkerT {
             }
         }
         if constexpr (POLAR) {
-            if (pm >= 0) stokes_update(P, cost, rev2, muz, mz0, seg);
+            if (pm >= 0) stokes_update(P, cost, rev2, mx0, my0, mz0, seg);
         }
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
@@ -674,7 +674,7 @@ struct NativeWalkerT {
     // (sign -1 unless phi_sca = pi, as in the reference; cos i_2 = 0 where
     // den = 0), normalise to I = 1.  The doubled angles of the return
     // rotation come from cos i_2 directly: cos 2i = 2c^2 - 1, sin 2i = 2c s.
-    __device__ void stokes_update(const KParams &P, float cost, float rev2, float muz_n, float mz0, int seg)
+    __device__ void stokes_update(const KParams &P, float cost, float rev2, float mx0, float my0, float mz0, int seg)
     {
         const float *T = pm_table(P);
         const int n = P.pm_n;
@@ -687,11 +687,13 @@ struct NativeWalkerT {
         const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        stokes_apply(cost, rev2, muz_n, mz0, p11, p12, p22, p33, p43, p44);
+        stokes_apply(cost, rev2, mx0, my0, mz0, p11, p12, p22, p33, p43, p44);
     }
 
-    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float muz_n, float mz0, float p11, float p12,
-                                                 float p22, float p33, float p43, float p44)
+    // (mx0, my0, mz0): the direction before the scattering; the photon's
+    // direction members hold the one after it
+    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float mx0, float my0, float mz0, float p11,
+                                                 float p12, float p22, float p33, float p43, float p44)
     {
 #pragma clang fp contract(off)
         const float c2 = __builtin_amdgcn_cosf(rev2);
@@ -702,15 +704,23 @@ struct NativeWalkerT {
         const float Qc = fmaf(sI, p12, Qs * p22);
         const float Uc = fmaf(Us, p33, -sV * p43);
         const float Vc = fmaf(Us, p43, sV * p44);
+        const float muz_n = muz;
         const float den = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f) *
                                                  fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
-        float ci = 0.f;
+        // i_2 in [0, pi]: cos i_2 as the reference forms it; sin i_2 >= 0 from
+        // its exact identity (den sin i_2)^2 = (e_z . (u_0 x u_n))^2 -- the
+        // Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2 - cos^2 --
+        // instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1 (scattering
+        // plane close to the meridian plane) would turn an FP32 rounding of
+        // cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 = 0.
+        float ci = 0.f, si = 1.f;
         if (den != 0.f) {
+            const float rd = __builtin_amdgcn_rcpf(den);
             const float sg = (rev2 == 1.f) ? 1.f : -1.f;
-            ci = fminf(fmaxf(fmaf(muz_n, cost, -mz0) * __builtin_amdgcn_rcpf(sg * den), -1.f), 1.f);
+            ci = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
+            si = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
         }
-        const float si = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-ci, ci, 1.f), 0.f));
-        const float cr = fmaf(2.f * ci, ci, -1.f);      // cos(-2 i_2)
+        const float cr = fmaf(-2.f * si, si, 1.f);     // cos(-2 i_2) = 1 - 2 sin^2 i_2
         const float sr = -2.f * ci * si;               // sin(-2 i_2)
         const float inv = __builtin_amdgcn_rcpf(Ic);
         const float Qm = fmaf(Qc, cr, Uc * sr);
@@ -794,7 +804,7 @@ struct NativeWalkerT {
         const float cosp = __builtin_amdgcn_cosf(rev);
         const float s2 = fmaf(-muz, muz, 1.f);
         const float st = sint * __builtin_amdgcn_rsqf(s2);
-        const float mz0 = muz;
+        const float mz0 = muz, mx0 = mux, my0 = muy;
         const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
         const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
         muz = fmaf(-s2 * st, cosp, muz * cost);
@@ -805,7 +815,7 @@ struct NativeWalkerT {
             muy = sg * sint * sinp;
             muz = sg * cost;
         }
-        stokes_apply(cost, r2, muz, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        stokes_apply(cost, r2, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
         const float dtau = k.dtau;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
@@ -843,7 +853,7 @@ struct NativeWalkerT {
         const float rv = 0.5f * rev2;
         mux = sint * __builtin_amdgcn_cosf(rv);
         muy = sint * __builtin_amdgcn_sinf(rv);
-        if (update) stokes_update(P, cost, rev2, muz, 1.f, -1);
+        if (update) stokes_update(P, cost, rev2, 0.f, 0.f, 1.f, -1);
     }
 
     __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const

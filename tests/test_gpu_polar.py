@@ -276,3 +276,32 @@ def test_polar_step_cap_with_lambertian_reemission(walker, tmp_path, cap):
             np.testing.assert_array_equal(runs[1][f], runs[0][f], err_msg=f)
     finally:
         walker.set_launch()
+
+
+@pytest.mark.parametrize('stokes0', [(1, 0, 0, 0), (1, 1, 0, 0)])
+def test_polar_native_single_scattering_closed_form(walker, tmp_path, stokes0):
+    """Singly reflected photons of a thin layer under normal incidence, native
+    FP32 walk (one-lane and tail mode): their final Stokes vectors equal the
+    phase matrix's closed form (tests/test_oracle_polar.py) within FP32
+    rounding."""
+    from tests.test_oracle_polar import single_scatter_expectations
+    case = dict(CASES[0], n=200, seed=31, fwhm=1e-12, theta0=0.0, lb=False, R=0.0, tau=0.5,
+                stokes=list(stokes0))
+    records, par, values, offsets, reff, tabs, bands = run_case(case, str(tmp_path))
+    g, pband = gpu_case(case, par, tabs, bands)
+    g.n_photons = 300_000
+    walker.set_cases([g])
+    try:
+        for coop in (0, 16):
+            walker.set_launch(512, 2, 2048, 16, coop)
+            walker.walk(mode='native', seed=8, records=True)
+            r = walker.records()
+            st = walker.stokes()
+            recs = list(zip(r['condition'], r['wvn'], r['theta_n'], r['phi_n'], r['n_scat'], r['path_length'],
+                            r['snow_depth'], st['Q_n'], st['U_n'], st['V_n']))
+            got, exp = single_scatter_expectations(recs, tabs, bands, par['wvl'][0], stokes0)
+            assert got[0].size > 5000
+            for a, b in zip(got, exp):
+                np.testing.assert_allclose(a, b, rtol=0, atol=2e-5)
+    finally:
+        walker.set_launch(1024, 1, 2048, 16, 16)

@@ -109,3 +109,40 @@ def test_stokes_update_closed_forms():
     np.testing.assert_allclose(out, [1.0, -0.3, 0.2, 0.1], atol=1e-15)
     out = polar.stokes_update(pol, s, 0.3, np.cos(0.4), -0.2, 0.6)
     assert out[0] == 1.0 and np.hypot(out[1], out[2]) == pytest.approx(np.hypot(s[1], s[2]), abs=1e-12)
+
+
+def single_scatter_expectations(recs, tabs, bands, key, stokes0):
+    """Closed form of one scattering at normal incidence (muz_0 = -1, so the
+    scattering plane is the exit direction's meridian plane and the return
+    rotation is by pi): a photon reflected after one scattering at cos theta
+    = -cos(theta_n) has Q_n = (P12 + c P22) / (P11 + c P12) with c =
+    cos(2 phi_n) Q_0, and |U_n|, |V_n| = |sin 2 phi_n| Q_0 (P33, P43) /
+    (P11 + c P12).  Returns (got, expected) arrays for such photons."""
+    cols = list(zip(*recs))
+    cond = np.asarray(cols[0]); ns = np.asarray(cols[4])
+    th = np.asarray(cols[2], float); ph = np.asarray(cols[3], float)
+    q = np.asarray(cols[7], float); u = np.asarray(cols[8], float); v = np.asarray(cols[9], float)
+    m = (cond == 1) & (ns == 1)
+    mu = -np.cos(th[m])
+    Y = bands[key]['Y']
+    P = {e: np.interp(mu, tabs['x'], Y[e]) for e in polar.ELEMENTS}
+    c = np.cos(2 * ph[m]) * stokes0[1]
+    s = np.abs(np.sin(2 * ph[m])) * stokes0[1]
+    den = P['P11'] + c * P['P12']
+    return (q[m], np.abs(u[m]), np.abs(v[m])), ((P['P12'] + c * P['P22']) / den, np.abs(s * P['P33'] / den),
+                                                 np.abs(s * P['P43'] / den))
+
+
+@pytest.mark.parametrize('stokes0', [(1, 0, 0, 0), (1, 1, 0, 0)])
+def test_polar_oracle_single_scattering_closed_form(tmp_path, stokes0):
+    """Independent of the reference's run: singly reflected photons of a thin
+    layer under normal incidence carry the Stokes vector the phase matrix
+    gives in closed form (unpolarised: Q_n = P12 / P11, U_n = V_n = 0)."""
+    case = dict(CASES[0], n=8000, seed=31, fwhm=1e-12, theta0=0.0, lb=False, R=0.0, tau=0.5,
+                stokes=list(stokes0))
+    recs, par, values, offsets, reff, tabs, bands = run_case(case, str(tmp_path))
+    key = par['wvl'][0]
+    got, exp = single_scatter_expectations(recs, tabs, bands, key, stokes0)
+    assert got[0].size > 30
+    for g, e in zip(got, exp):
+        np.testing.assert_allclose(g, e, rtol=1e-9, atol=1e-12)
