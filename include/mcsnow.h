@@ -122,7 +122,10 @@ typedef struct {
                                   block keeps 4 chunks in flight and a chunk
                                   holds its tally slot until its longest walk
                                   ends: chunks under ~2x the block's threads
-                                  leave lanes idle on long-tailed walks       */
+                                  leave lanes idle while the queue still has
+                                  chunks (a block whose four slots all hold
+                                  walking chunks reads the queue counter to
+                                  enter the cooperative tail once it is dry) */
     int32_t refill_min;        /* idle lanes that trigger a wave refill 1..64 */
     int32_t coop_live;         /* native mode, once the work queue is empty: a
                                   wave with at most this many live photons

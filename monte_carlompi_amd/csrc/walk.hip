@@ -1870,6 +1870,15 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
                                      ((unsigned long long)(seq_now + 1) << 32) | ((unsigned long long)ns << CUR_SLOT_SHIFT),
                                      __ATOMIC_RELAXED);
                 advanced = true;
+            } else if (ns < 0) {
+                // every slot still holds a walking chunk: no acquisition can
+                // tell this block that the queue ran dry, so read the counter
+                // (otherwise small chunks would keep the block out of the
+                // cooperative tail until its last chunk ends)
+                unsigned int taken = 0u;
+                if (lane == 0) taken = atomicOr(P.chunk_counter, 0u);
+                taken = (unsigned int)bcast((int)taken, 0);
+                if (taken >= P.n_chunks && lane == 0) __atomic_store_n(&bs.exhausted, 1, __ATOMIC_RELAXED);
             }
         } else if (seq_now != seq) {
             advanced = true;  // someone else advanced: retry the handout

@@ -272,3 +272,30 @@ def test_hand_over_across_cases(walker):
             z = two_sample_z(int(np.sum(ka == cond)), ka.size, int(np.sum(kb == cond)), kb.size)
             assert abs(z) < 3.5, (k, cond, z)
         assert int(ca[k, U_STEPS]) == int(np.sum(a['n_scat'][sel] + 1))
+
+
+@pytest.mark.parametrize('shape', [(1024, 1, 977, 8, 32), (640, 2, 977, 8, 32), (256, 2, 300, 4, 16)])
+def test_small_chunks_reach_the_cooperative_tail(walker, shape):
+    """Chunks smaller than a block's lanes can fill all four tally slots with
+    walking chunks; such a block can acquire nothing, so it must learn from
+    the global counter that the queue ran dry -- else its long walks finish
+    one lane each (5 ms instead of 0.8 ms for the 1e6-photon case).  Checked
+    through the cooperative photon-steps (count slot 9) and the law."""
+    from monte_carlompi_amd.workloads import snow_1m_case
+    walker.set_cases([snow_1m_case(400_000)])
+    walker.set_launch(*shape)
+    try:
+        walker.walk(mode='native', seed=17, records=True)
+        c = walker.counts()[0]
+        assert int(c[0]) == 400_000
+        assert float(c[U_COOP]) / float(c[U_STEPS]) > 0.1, c
+        r = walker.records()
+        walker.set_launch(1024, 1, 2048, 16, 0)
+        walker.walk(mode='native', seed=18, records=True)
+        r0 = walker.records()
+        for cond in (1, 2, 4):
+            z = two_sample_z(np.sum(r['condition'] == cond), r['condition'].size,
+                             np.sum(r0['condition'] == cond), r0['condition'].size)
+            assert abs(z) < 4, (cond, z)
+    finally:
+        walker.set_launch()
