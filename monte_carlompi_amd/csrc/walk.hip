@@ -687,17 +687,20 @@ struct NativeWalkerT {
         const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        stokes_apply(cost, rev2, mx0, my0, mz0, p11, p12, p22, p33, p43, p44);
+        stokes_apply(cost, rev2, __builtin_amdgcn_cosf(rev2), __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12,
+                     p22, p33, p43, p44);
     }
 
     // (mx0, my0, mz0): the direction before the scattering; the photon's
     // direction members hold the one after it
-    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float mx0, float my0, float mz0, float p11,
-                                                 float p12, float p22, float p33, float p43, float p44)
+    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions)
+    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float c2r, float s2r, float mx0, float my0,
+                                                 float mz0, float p11, float p12, float p22, float p33, float p43,
+                                                 float p44)
     {
 #pragma clang fp contract(off)
-        const float c2 = __builtin_amdgcn_cosf(rev2);
-        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * __builtin_amdgcn_sinf(rev2);
+        const float c2 = c2r;
+        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * s2r;
         const float Qs = fmaf(sQ, c2, sU * s2);
         const float Us = fmaf(-sQ, s2, sU * c2);
         const float Ic = fmaf(sI, p11, Qs * p12);
@@ -786,8 +789,15 @@ struct NativeWalkerT {
         ++i;
         float r2 = k.r2, acc = k.acc;
         const float bound = k.p11 + fabsf(k.p12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        // sin and cos of the azimuth (r2 / 2 revolutions) serve the rotation;
+        // those of 2 phi follow from them (one transcendental pair a step)
+        float sinp, cosp, c2r, s2r;
         for (uint32_t kk = 1;; ++kk) {
-            const float val = fmaf(sI, k.p11, k.p12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
+            sinp = __builtin_amdgcn_sinf(0.5f * r2);
+            cosp = __builtin_amdgcn_cosf(0.5f * r2);
+            c2r = fmaf(cosp, cosp, -sinp * sinp);
+            s2r = 2.f * sinp * cosp;
+            const float val = fmaf(sI, k.p11, k.p12 * fmaf(sQ, c2r, sU * s2r));
             if (acc * bound <= val) break;
             if (kk >= POLAR_MAX_TRIES) {
                 atomicAdd(P.ctl + 2, 1ull);
@@ -798,10 +808,7 @@ struct NativeWalkerT {
             acc = u01f(q.y);
         }
         const float cost = k.ct;
-        const float rev = 0.5f * r2;
         const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
-        const float sinp = __builtin_amdgcn_sinf(rev);
-        const float cosp = __builtin_amdgcn_cosf(rev);
         const float s2 = fmaf(-muz, muz, 1.f);
         const float st = sint * __builtin_amdgcn_rsqf(s2);
         const float mz0 = muz, mx0 = mux, my0 = muy;
@@ -815,7 +822,7 @@ struct NativeWalkerT {
             muy = sg * sint * sinp;
             muz = sg * cost;
         }
-        stokes_apply(cost, r2, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        stokes_apply(cost, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
         const float dtau = k.dtau;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
