@@ -305,3 +305,28 @@ def test_polar_native_single_scattering_closed_form(walker, tmp_path, stokes0):
                 np.testing.assert_allclose(a, b, rtol=0, atol=2e-5)
     finally:
         walker.set_launch(1024, 1, 2048, 16, 16)
+
+
+def test_polar_replay_large_stream(walker, tmp_path):
+    """10,000 photons of the spectral, impure, Q-polarised-ish case with a
+    Lambertian bottom (3.8e6 deviates): the FP64 replay walk takes the
+    oracle's rejection decisions and Stokes vectors photon for photon.  The
+    device's float64 cos/sin/acos/atan and numpy's may differ in the last
+    ulp, so a decision could flip in principle; none may here beyond 0.1 %,
+    and every other photon must agree to 1e-9."""
+    case = [c for c in CASES if c['name'] == 'pm_spectral_u'][0]
+    case = dict(case, n=10_000, seed=4242)
+    records, par, values, offsets, reff, tabs, bands = run_case(case, str(tmp_path))
+    g, pband = gpu_case(case, par, tabs, bands)
+    walker.set_cases([g])
+    walker.walk(mode='replay', records=True, photon_band=pband, replay=(values, offsets))
+    got = walker.records()
+    got.update(walker.stokes())
+    cols = list(zip(*records))
+    ref = dict(condition=np.asarray(cols[0]), n_scat=np.asarray(cols[4]), theta_n=np.asarray(cols[2], float),
+               Q_n=np.asarray(cols[7], float), U_n=np.asarray(cols[8], float), V_n=np.asarray(cols[9], float))
+    same = (got['condition'] == ref['condition']) & (got['n_scat'] == ref['n_scat'])
+    assert np.mean(~same) <= 1e-3, np.sum(~same)
+    for f in ('theta_n', 'Q_n', 'U_n', 'V_n'):
+        np.testing.assert_allclose(got[f][same], ref[f][same], rtol=1e-9, atol=1e-9, err_msg=f)
+    print('polar replay 1e4: %d of %d photons differ' % (np.sum(~same), same.size), flush=True)
