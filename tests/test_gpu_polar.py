@@ -330,3 +330,28 @@ def test_polar_replay_large_stream(walker, tmp_path):
     for f in ('theta_n', 'Q_n', 'U_n', 'V_n'):
         np.testing.assert_allclose(got[f][same], ref[f][same], rtol=1e-9, atol=1e-9, err_msg=f)
     print('polar replay 1e4: %d of %d photons differ' % (np.sum(~same), same.size), flush=True)
+
+
+def test_polar_bench_workload_follows_the_oracle_law(walker, tmp_path):
+    """The bench.py polar workload (1 m of snow, droxtal phase matrices,
+    Stokes (1, 1, 0, 0), Lambertian bottom; walks of ~66 steps with a long
+    tail, most of it in the tail mode): 1e6 GPU photons against 3,000 oracle
+    photons -- exit fractions, reflected zenith angles, mean final Q, U, V."""
+    from monte_carlompi_amd import optics
+    from monte_carlompi_amd.workloads import RHO_ICE, RHO_SNW, SNOW_POLAR, snow_polar_case
+    d = str(tmp_path)
+    case = snow_polar_case(1_000_000, d)
+    p = SNOW_POLAR
+    recs, par, values, offsets, reff, tabs, bands = polar.polar_run(
+        9, 3000, p['wvl'], 1e-12, p['r_eff'], optics.aspherical_dir(d, p['shape'], p['roughness'], p['wvl']), False,
+        p['stokes0'], lambda w: dict(ssa=np.zeros(w.shape), ext=np.zeros(w.shape)), 0.0, case.tau_tot, RHO_SNW,
+        RHO_ICE, p['theta0'], True, p['r_lambertian'])
+    cols = list(zip(*recs))
+    o = dict(condition=np.asarray(cols[0]), theta_n=np.asarray(cols[2], float), Q_n=np.asarray(cols[7], float),
+             U_n=np.asarray(cols[8], float), V_n=np.asarray(cols[9], float))
+    walker.set_cases([case])
+    walker.set_launch(1024, 1, 2048, 16, 16)
+    walker.walk(mode='native', seed=77, records=True)
+    rec = walker.records()
+    st = walker.stokes()
+    assert_same_law(o, rec, st, polarised=True)
