@@ -25,13 +25,14 @@ METRIC = 'photons/sec on 1 GPU; BRF cases/sec at ≤1% albedo err vs 1e6-photon 
 # 1e6 photons over every block; the cooperative tail mode finishes the walks
 # still running when the queue is empty, and 16-wave blocks spread those
 # photons over the waves of the block that run out of work.  Chunks sized for
-# exactly two per 1024-thread block keep any block from ending the queue a
-# whole chunk behind.
+# exactly two (or four) per 1024-thread block keep any block from ending the
+# queue a whole chunk behind.
 def snow_1m_candidates(n_photons, cus):
     from monte_carlompi_amd.gpu import balanced_chunk
     b1 = balanced_chunk(n_photons, 1, cus)
+    b4 = balanced_chunk(n_photons, 1, cus, per_block=4)
     return [(1024, 1, b1, 16, 32), (1024, 1, b1, 16, 24), (1024, 1, b1, 8, 32), (1024, 1, 2048, 16, 32),
-            (1024, 1, 2048, 16, 24), (512, 2, b1, 16, 32), (256, 5, 512, 16, 16), (512, 3, 1024, 16, 16)]
+            (1024, 1, 2048, 16, 24), (512, 2, b1, 16, 32), (1024, 1, b4, 16, 32), (256, 5, 512, 16, 16)]
 
 
 N_PHOTONS = 1_000_000
