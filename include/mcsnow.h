@@ -131,7 +131,11 @@ typedef struct {
                                   wave with at most this many live photons
                                   (0..32) walks them cooperatively, 64 / L
                                   lanes per photon stepping it in parallel;
-                                  0 = off.  Default 16.                       */
+                                  0 = off.  Default 16.  Polar cases
+                                  (mcs_set_polar): a wave with at most
+                                  min(coop_live, 16) photons builds their next
+                                  steps' state-independent part in its idle
+                                  lanes (records unchanged by this)          */
 } mcs_launch;
 
 /* Host output arrays (each sum(n_photons) long, photon order; any may be
