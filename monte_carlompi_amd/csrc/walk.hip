@@ -585,51 +585,18 @@ struct NativeWalkerT {
             // The same law without the two-area rejection loop (whose
             // acceptance is ~10 % on a forward-peaked P11): cos theta from
             // the marginal S11 by inverting its piecewise-quadratic integral
-            // (one search over the 8th row), then 2 phi from
-            // 1 + (S12 / S11) (Q cos 2phi + U sin 2phi) by rejection against
-            // 1 + |S12 / S11| sqrt(Q^2 + U^2) (acceptance >= 1/2, no table
-            // reads).  Philox block (photon, case, i, 2 + k), k = 0, 1, ...
-            const float *cum = T + 7 * n;
-            const uint32_t ph = photon_id();
-            u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u}, P.key0, P.key1);
-            const float u = u01f(q.x);
-            const float t = u * pm_mass;
-            // the guide brackets the segment: search only between two entries
-            const int *guide = reinterpret_cast<const int *>(T + 14 * n);
-            const int gk = min((int)(u * (float)PM_GUIDE), PM_GUIDE - 1);
-            int lo = guide[gk], hi = guide[gk + 1] + 1;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (cum[mid] <= t) lo = mid; else hi = mid;
-            }
-            const int j = lo;
-            const float x0 = T[j], h = T[j + 1] - T[j];
-            const float y0 = T[n + j];
-            const float slope = T[8 * n + j];
-            const float dm = fmaxf(t - cum[j], 0.f);
-            // y0 s + slope s^2 / 2 = dm, in the form stable for either sign of slope
-            const float s = 2.f * dm *
-                            __builtin_amdgcn_rcpf(y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
-            const float ds = fminf(fmaxf(s, 0.f), h);
-            const float ct = fminf(x0 + ds, T[n - 1]);
-            const float s11 = fmaf(slope, ct - x0, y0);
-            const float s12 = fmaf(T[9 * n + j], ct - x0, T[2 * n + j]);
-            const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
-            float r2 = 2.f * u01f(q.y), acc = u01f(q.z);
-            for (uint32_t k = 1;; ++k) {
-                const float val = fmaf(sI, s11, s12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
-                if (acc * bound <= val) break;
-                if (k >= POLAR_MAX_TRIES) {
-                    atomicAdd(P.ctl + 2, 1ull);
-                    break;
-                }
-                q = philox4x32_10(u32x4{ph, cs, i, 2u + k}, P.key0, P.key1);
-                r2 = 2.f * u01f(q.x);
-                acc = u01f(q.y);
-            }
-            cost = ct;
+            // (make_packet), then 2 phi from 1 + (S12 / S11) (Q cos 2phi +
+            // U sin 2phi) by rejection against 1 + |S12 / S11| sqrt(Q^2 +
+            // U^2) (polar_phi: acceptance >= 1/2, no table reads).  Philox
+            // block (photon, case, i, 2 + k), k = 0, 1, ...
+            const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 2u}, P.key0, P.key1);
+            PolarPacket k;
+            make_packet(q, q, T, n, pm_mass, k);   // its free path and absorption word go unused
+            float r2 = k.r2, sinp, cosp, c2r, s2r;
+            polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
+            cost = k.ct;
             rev2 = r2;
-            seg = j;
+            seg = k.seg;
             return;
         }
         const DevCase &C = P.cases[cs];
@@ -778,26 +745,22 @@ struct NativeWalkerT {
         o.acc = u01f(pb.z);
     }
 
-    // One step of an exact-sampler photon given its packet: 2 phi by
-    // rejection against the current Stokes vector (retries draw blocks
-    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
-    // one-lane walk and the tail mode below both step through here, so a
-    // photon's record does not depend on which of them walked it.
-    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
+    // 2 phi given cos theta: the first candidate r2 (revolutions) and its
+    // acceptance uniform come with the packet, retries draw blocks (photon,
+    // case, i, 2 + k); target I S11 + S12 (Q cos 2phi + U sin 2phi), bound
+    // S11 + |S12| sqrt(Q^2 + U^2).  Returns the accepted r2 with the sine and
+    // cosine of the azimuth (r2 / 2) and of 2 phi (one transcendental pair).
+    __device__ __forceinline__ void polar_phi(const KParams &P, float s11, float s12, float acc, float &r2, float &sinp,
+                                              float &cosp, float &c2r, float &s2r) const
     {
 #pragma clang fp contract(off)
-        ++i;
-        float r2 = k.r2, acc = k.acc;
-        const float bound = k.p11 + fabsf(k.p12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
-        // sin and cos of the azimuth (r2 / 2 revolutions) serve the rotation;
-        // those of 2 phi follow from them (one transcendental pair a step)
-        float sinp, cosp, c2r, s2r;
+        const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
         for (uint32_t kk = 1;; ++kk) {
             sinp = __builtin_amdgcn_sinf(0.5f * r2);
             cosp = __builtin_amdgcn_cosf(0.5f * r2);
             c2r = fmaf(cosp, cosp, -sinp * sinp);
             s2r = 2.f * sinp * cosp;
-            const float val = fmaf(sI, k.p11, k.p12 * fmaf(sQ, c2r, sU * s2r));
+            const float val = fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r));
             if (acc * bound <= val) break;
             if (kk >= POLAR_MAX_TRIES) {
                 atomicAdd(P.ctl + 2, 1ull);
@@ -807,6 +770,19 @@ struct NativeWalkerT {
             r2 = 2.f * u01f(q.x);
             acc = u01f(q.y);
         }
+    }
+
+    // One step of an exact-sampler photon given its packet: 2 phi by
+    // rejection against the current Stokes vector (retries draw blocks
+    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
+    // one-lane walk and the tail mode below both step through here, so a
+    // photon's record does not depend on which of them walked it.
+    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        float r2 = k.r2, sinp, cosp, c2r, s2r;
+        polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
         const float cost = k.ct;
         const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
         const float s2 = fmaf(-muz, muz, 1.f);
