@@ -31,1617 +31,11 @@
 
 #include "mcsnow.h"
 #include "philox.h"
+#include "walk_state.h"
+#include "walkers.h"
+#include "coop.h"
 
 namespace mcs {
-
-constexpr int SLOTS = 4;
-constexpr int MAX_WAVES = 16;     // waves per block (1024 threads)
-static_assert((SLOTS & (SLOTS - 1)) == 0, "SLOTS is a power of two");
-constexpr int TF = MCS_TALLY_F64;
-constexpr int TU = MCS_TALLY_U64;
-constexpr double PI_D = 3.141592653589793;
-constexpr float TWO_PI_F = 6.283185307179586f;
-constexpr float LN2_F = 0.6931471805599453f;
-
-// tally layout (doubles), see mcs_read_tallies
-constexpr int T_QDOWN = MCS_BRF_BINS;        // 648
-constexpr int T_QCOND = MCS_BRF_BINS + 1;    // 649..653 by condition 1..5
-constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
-constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
-// counts (uint64)
-constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED = 10;
-// rejection attempts per scattering angle before the walk reports an error
-// (ctl[2]); valid tables accept within a few
-constexpr uint32_t POLAR_MAX_TRIES = 1u << 20;
-constexpr int PM_GUIDE = 256;   // guide cells of the FP32 phase-matrix tables
-
-struct DevCase {
-    double tau_tot, mux0, muz0, r_lamb;
-    unsigned long long t_refl;  // reflect iff 32 random bits < t_refl (R * 2^32)
-    float tau_tot_f, mux0_f, muz0_f;
-    int32_t lamb, band_begin, n_bands, surf;
-    int64_t n_photons, gid_base;
-    // aspherical particles: initial Stokes vector and the rejection sampler's
-    // cos(theta) range and split (mc3D:858-883, 1132)
-    int32_t polar;
-    double s0[4], cmax, ccut, cmin;
-    float s0f[4], fcmax, fccut, fcmin;
-};
-
-struct DevBand {
-    double wvn, g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, cdf;
-    // HG inverse CDF from the 32 Philox bits x: t = hg_c / (hg_e x + hg_d),
-    // cos = hg_a (hg_b - t^2), with hg_e = 2g 2^-32 and hg_d = 1 - g + 2g 2^-33
-    // (u = (x + 1/2) 2^-32 folded in); isotropic bands: hg_e = 0, hg_d = 1
-    float hg_a, hg_b, hg_c, hg_d, hg_e;
-    float inv_ext;                       // 1/ext_cff [m]
-    uint32_t t_abs5;                     // bits < P_imp (1 - ssa_imp) 2^32: absorbed by impurity
-    uint32_t t_abs_all;                  // bits < [P_imp (1 - ssa_imp) + (1 - P_imp)(1 - ssa_ice)] 2^32: absorbed
-    int32_t iso;                         // g == 0
-    // alias table of the case's band distribution (Walker): column j of the
-    // case keeps band j iff the fraction bits < alias_t, else takes alias_k
-    uint32_t alias_t;
-    int32_t alias_k;
-    // phase-matrix table of the band (-1: Henyey-Greenstein) and the maxima
-    // of |P11| and |P12| on either side of the cutoff (mc3D:926-941)
-    int32_t pm;
-    double pA1, pB1, pA2, pB2;
-    float fA1, fB1, fA2, fB2;
-    // native mode: |P12| <= P11 on the whole grid, so I S11 + S12 (Q cos 2phi
-    // + U sin 2phi) >= 0 for |(Q, U)| <= I and its cos(theta) marginal is S11:
-    // the walk samples it exactly by inversion (fM = integral of S11)
-    int32_t pm_exact;
-    float fM;
-};
-
-// probability -> 32-bit threshold (bits < t has probability p)
-__host__ __device__ inline uint32_t prob_bits_dev(double p)
-{
-    const double t = p * 4294967296.0;
-    if (!(t > 0.0)) return 0u;
-    if (t >= 4294967295.0) return 0xFFFFFFFFu;
-    return (uint32_t)(t + 0.5);
-}
-
-struct KParams {
-    const DevCase *cases;
-    const DevBand *bands;
-    const int32_t *case_chunk0;  // n_cases + 1: first chunk of each case
-    int32_t n_cases;
-    int32_t chunk;
-    uint32_t n_chunks;
-    uint32_t refill_min;
-    uint32_t key0, key1;
-    uint32_t max_steps;
-    int32_t coop_live;           // tail mode at <= this many live photons per wave (0: off)
-    const uint8_t *photon_band;  // nullable
-    const double *rdev;          // replay
-    const int64_t *roff;
-    double *tally_f;
-    unsigned long long *tally_u;
-    unsigned int *chunk_counter;
-    unsigned long long *ctl;     // [0] replay slice errors, [1] failed hand-overs
-    uint32_t *rec_gid, *rec_nscat, *rec_cb;
-    void *rec_theta, *rec_phi, *rec_path;
-    // phase-matrix tables.  FP64 (replay): 7 rows of pm_n -- ascending cos
-    // theta, P11, P12, P22, P33, P43, P44.  FP32 (native), pm_stride floats
-    // per table: those 7 rows, the integral of the linear P11 from the first
-    // grid point, the six elements' slopes per grid segment, and a guide of
-    // PM_GUIDE + 1 segment indices (int bits) where that integral crosses
-    // k / PM_GUIDE of its total.  Final Stokes (Q, U, V) per photon in photon
-    // order (3 x n_total, nullable).
-    const double *pm;
-    const float *pmf;
-    int32_t pm_n;
-    int32_t pm_lds_floats;       // > 0: the FP32 tables are copied into dynamic LDS at block start
-    int32_t pm_stride;
-    int32_t pad1;
-    void *rec_stokes;
-    uint64_t n_total;
-};
-
-// ---------------------------------------------------------------- LDS state
-struct SlotHdr {
-    unsigned long long seqcount;  // (block-local chunk seq << 32) | count
-    unsigned long long gid0;      // global id of the chunk's first photon
-    unsigned int p0;              // first photon (within its case)
-    int case_id;
-    int remaining;                // photons of the chunk not finished
-    int rec_next;                 // records written so far
-    int state;                    // 0 free, 1 live
-    int pad;
-};
-
-struct SlotTally {
-    double f[TF];
-    unsigned long long u[TU];
-};
-
-struct BlockState {
-    unsigned long long cur;  // (seq << 32) | next photon offset in chunk
-    int lock;
-    int exhausted;
-    // hand-over of cooperative-tail photons between the block's waves
-    int holders;                  // waves not waiting for photons
-    unsigned int idle_mask;       // waves waiting for photons (bit per wave)
-    int ib_state[MAX_WAVES];      // inbox of each wave: 1 = photons posted
-    int ib_src[MAX_WAVES];        //   posting wave (its coop table holds them)
-    unsigned int ib_mask[MAX_WAVES];  //   their table entries
-    int out_posts[MAX_WAVES];     // posts of each wave not yet taken
-    SlotHdr hdr[SLOTS];
-    SlotTally tal[SLOTS];
-};
-
-__device__ __forceinline__ int lane_id() { return __lane_id(); }
-
-__device__ __forceinline__ int lane_rank(unsigned long long mask)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
-}
-
-__device__ __forceinline__ int bcast(int v, int src) { return __shfl(v, src, 64); }
-__device__ __forceinline__ unsigned long long bcast64(unsigned long long v, int src)
-{
-    return __shfl(v, src, 64);
-}
-
-template <class T>
-__device__ __forceinline__ T lds_load(const T *p)
-{
-    return __atomic_load_n(p, __ATOMIC_RELAXED);
-}
-
-// ----------------------------------------------------------- native walker
-// FP32 walk with one Philox4x32-10 call per step (4 uniforms: HG / Lambertian
-// cosine, azimuth, free path, absorption).  The common path is straight-line
-// code -- exits, absorption and near-vertical rotations sit behind rarely
-// taken branches (the last one wave-uniform) -- and the next step's Philox call
-// is issued in the same basic block as the current step's arithmetic: the two
-// chains are independent, so their latencies overlap.
-// np.interp(v, x, y) on an ascending grid of n points (what the reference's
-// interp1d(cos_theta[::-1], P[::-1]) evaluates, mc3D:842-857): y at the ends
-// and at grid points, else slope (v - x_j) + y_j with x_j <= v < x_j+1.
-template <class T>
-__device__ __forceinline__ int grid_index(const T *x, int n, T v)
-{
-    int lo = 0, hi = n - 1;
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (x[mid] <= v) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-template <class T>
-__device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v)
-{
-#pragma clang fp contract(off)
-    if (v <= x[0]) return y[0];
-    if (v >= x[n - 1]) return y[n - 1];
-    if (x[j] == v) return y[j];
-    const T slope = (y[j + 1] - y[j]) / (x[j + 1] - x[j]);
-    T r = slope * (v - x[j]) + y[j];
-    if (r != r) {
-        r = slope * (v - x[j + 1]) + y[j + 1];
-        if (r != r && y[j] == y[j + 1]) r = y[j];
-    }
-    return r;
-}
-
-// The state-independent part of one step of the polarised walk (exact
-// sampler): free path and absorption word from the step's main Philox block,
-// cos theta and the phase matrix there, the first 2 phi candidate and its
-// acceptance uniform.  One lane can build it for any step of any photon.
-struct PolarPacket {
-    float dtau;
-    uint32_t w;
-    float ct, r2, acc;
-    float p11, p12, p22, p33, p43, p44;
-    int seg;
-};
-
-template <bool POLAR>
-struct NativeWalkerT {
-    using Real = float;
-    static constexpr bool COOP = !POLAR;      // has the wave-cooperative tail mode below
-    static constexpr bool STOKES = POLAR;     // carries a Stokes vector (aspherical particles)
-    // the polarised walk is built for 3 waves per SIMD (168 VGPRs), which
-    // caps its blocks at 512 threads
-    static constexpr int MAX_BLOCK = POLAR ? 512 : 1024;
-    // k_walk is built for 5 (96 VGPRs) and 6 (80 VGPRs) waves per SIMD
-    float z, mux, muy, muz, tau_acc;
-    uint32_t i;
-    int band;
-    u32x4 r;                     // uniforms of the next step
-    // per-photon parameters (the photon and case ids live in `pre`; the
-    // bottom reflectance and 1 / ext_cff are read from the tables when needed)
-    float a, b, cc, d, e, tau_tot;
-    uint32_t t_abs5, t_abs_all;  // absorbed as condition 5 iff bits < t_abs5, else 4 iff < t_abs_all
-    bool iso, lamb;
-    int cond;
-    PhiloxPre pre;
-    // aspherical particles (POLAR): Stokes vector normalised to I = 1 and the
-    // band's phase-matrix table (-1: Henyey-Greenstein)
-    float sI, sQ, sU, sV;
-    int pm;
-    bool pm_exact;               // the band's exact sampler applies (see DevBand)
-    float pm_mass;               // integral of S11 over the grid
-
-    // Set up photon `photon` of case `case_id` and take its first step (or
-    // its whole walk in the Lambertian surface mode); true when it is done.
-    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
-    {
-        const DevCase &C = P.cases[case_id];
-        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
-        r = philox_step(pre, 1u, P.key0, P.key1);
-        const int bnd = select_band(P, C, gid, r.x);
-        band = bnd;
-        load_params(P, C, bnd);
-        z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
-        i = 0;
-        cond = 0;
-        if constexpr (POLAR) {
-            const DevBand &PB = P.bands[C.band_begin + bnd];
-            pm = C.polar ? PB.pm : -1;
-            pm_exact = PB.pm_exact != 0;   // per-photon constants stay in registers:
-            pm_mass = PB.fM;               // a step reads only the tables
-            sI = C.s0f[0]; sQ = C.s0f[1]; sU = C.s0f[2]; sV = C.s0f[3];
-        }
-        if (C.surf) return surface_walk(P, C);
-        // first step: along the incident direction (mc3D:1344-1347)
-        ++i;
-        const u32x4 cur = r;
-        r = philox_step(pre, 2u, P.key0, P.key1);
-        const float dtau = free_path(cur);
-        z = dtau * muz;
-        tau_acc = dtau;
-        bool done = false;
-        if (z < -tau_tot) {                          // unscattered through the layer
-            tau_acc -= fminf((z + tau_tot) / muz, dtau);
-            z = -tau_tot;
-            if (lamb && reflects(P)) {
-                done = reemit(P);
-            } else {
-                cond = 3;
-                done = true;
-            }
-        } else if (cur.w < t_abs_all) {
-            cond = (cur.w < t_abs5) ? 5 : 4;
-            done = true;
-        }
-        if (!done && i >= P.max_steps) {             // capped, as after any step
-            cond = 0;
-            done = true;
-        }
-        return done;
-    }
-
-    // the photon's wavelength bin: given per photon, or drawn from the case's
-    // band distribution with the first step's polar word x (the first step
-    // goes along the incident direction and draws no angle): the high bits of
-    // x n_bands pick an alias-table column, the low bits accept it or take its
-    // alias -- one table read instead of a search
-    __device__ static int select_band(const KParams &P, const DevCase &C, uint64_t gid, uint32_t x)
-    {
-        int bnd = 0;
-        if (P.photon_band) {
-            bnd = P.photon_band[gid];
-        } else if (C.n_bands > 1) {
-            const uint64_t m = (uint64_t)x * (uint32_t)C.n_bands;
-            const int j = (int)(m >> 32);
-            const DevBand &B = P.bands[C.band_begin + j];
-            bnd = (uint32_t)m < B.alias_t ? j : B.alias_k;
-        }
-        return bnd;
-    }
-
-    // Continue a photon handed over by another wave of the block: its Philox
-    // words and walk state; band and constants are derived again.
-    __device__ void resume(const KParams &P, const PhiloxPre &p, uint64_t gid, float z_, float tau_,
-                           float dx, float dy, float dz, uint32_t i_)
-    {
-        pre = p;
-        const uint32_t cs = case_id(P);
-        const DevCase &C = P.cases[cs];
-        band = select_band(P, C, gid, philox_step(pre, 1u, P.key0, P.key1).x);
-        load_params(P, C, band);
-        z = z_; tau_acc = tau_; mux = dx; muy = dy; muz = dz; i = i_;
-        cond = 0;
-        r = philox_step(pre, i + 1u, P.key0, P.key1);
-    }
-
-    // per-photon constants of (case, band)
-    __device__ __forceinline__ void load_params(const KParams &P, const DevCase &C, int bnd)
-    {
-        const DevBand &B = P.bands[C.band_begin + bnd];
-        a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
-        iso = B.iso != 0;
-        if (C.surf) {
-            // Lambertian surface: absorbed with probability 1 - R, split by species
-            const double q = 1.0 - C.r_lamb;
-            t_abs5 = prob_bits_dev(B.p_ext_imp * q);
-            t_abs_all = prob_bits_dev(q);
-        } else {
-            t_abs5 = B.t_abs5;
-            t_abs_all = B.t_abs_all;
-        }
-        tau_tot = C.tau_tot_f;
-        lamb = C.lamb != 0;
-    }
-
-    __device__ __forceinline__ static float free_path(const u32x4 &q)
-    {
-        return -LN2_F * __builtin_amdgcn_logf(u01f(q.z));
-    }
-
-    // Cosine-weighted upward direction, cos = sqrt(u): the pdf 2 sin cos of
-    // the rejection loop in mc3D:1242-1250 (Lambertian re-emission).
-    __device__ __forceinline__ void lambertian_direction(const u32x4 &q)
-    {
-        const float cost = __builtin_amdgcn_sqrtf(u01f(q.x));
-        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
-        const float rev = u01f(q.y);
-        mux = sint * __builtin_amdgcn_cosf(rev);
-        muy = sint * __builtin_amdgcn_sinf(rev);
-        muz = cost;
-    }
-
-    // After a Lambertian bottom reflection at step i (z = -tau_tot): the next
-    // step re-emits cosine-weighted upward and moves (mc3D:1420-1459); true
-    // when the photon is done.
-    __device__ bool reemit(const KParams &P)
-    {
-        if (i >= P.max_steps) {
-            cond = 0;
-            return true;
-        }
-        ++i;
-        const u32x4 cur = r;
-        r = philox_step(pre, i + 1u, P.key0, P.key1);
-        lambertian_direction(cur);
-        if constexpr (POLAR) {
-            // the step after a bottom reflection still draws its angles from
-            // the phase matrix (the azimuth is used), then resets the Stokes
-            // vector (mc3D:1238-1285)
-            if (pm >= 0) polar_lambertian(P, false);
-            sI = 1.f; sQ = 0.f; sU = 0.f; sV = 0.f;
-        }
-        const float dtau = free_path(cur);
-        z = fmaf(dtau, muz, z);
-        tau_acc += dtau;
-        if (z > 0.f) {
-            tau_acc -= fminf(z / muz, dtau);
-            cond = 1;
-            return true;
-        }
-        if (cur.w < t_abs_all) {
-            cond = (cur.w < t_abs5) ? 5 : 4;
-            return true;
-        }
-        return false;
-    }
-
-    // Lambertian surface (mc3D:1228-1238, 1385-1387): first event at z = 0,
-    // absorbed with probability 1 - R, else re-emitted cosine-weighted upward
-    // until the photon leaves.
-    __device__ bool surface_walk(const KParams &P, const DevCase &C)
-    {
-        (void)C;
-        ++i;
-        u32x4 cur = r;
-        if (cur.w < t_abs_all) {
-            cond = (cur.w < t_abs5) ? 5 : 4;
-            return true;
-        }
-        for (;;) {
-            if (i >= P.max_steps) {
-                cond = 0;
-                return true;
-            }
-            ++i;
-            cur = philox_step(pre, i, P.key0, P.key1);
-            lambertian_direction(cur);
-            if constexpr (POLAR) {
-                // Lambertian surface of aspherical grains: azimuth from the
-                // phase matrix, Stokes vector scattered at the Lambertian
-                // angle (mc3D:1238-1250, 1287-1342)
-                if (pm >= 0) polar_lambertian(P, true);
-            }
-            const float dtau = free_path(cur);
-            z = fmaf(dtau, muz, z);
-            tau_acc += dtau;
-            if (z > 0.f) {
-                tau_acc -= fminf(z / muz, dtau);
-                cond = 1;
-                return true;
-            }
-            if (cur.w < t_abs_all) {
-                cond = (cur.w < t_abs5) ? 5 : 4;
-                return true;
-            }
-        }
-    }
-
-    // one step (i >= 1: after the first); true when the photon has left or
-    // been absorbed
-    __device__ __forceinline__ bool step(const KParams &P)
-    {
-        if constexpr (POLAR) {
-            if (pm >= 0 && pm_exact) {
-                const u32x4 cur = r;
-                r = philox_step(pre, i + 2u, P.key0, P.key1);
-                const u32x4 pb = philox4x32_10(u32x4{photon_id(), case_id(P), i + 1u, 2u}, P.key0, P.key1);
-                PolarPacket k;
-                make_packet(cur, pb, pm_table(P), P.pm_n, pm_mass, k);
-                return polar_step(P, k);
-            }
-        }
-        ++i;
-        const u32x4 cur = r;
-        r = philox_step(pre, i + 1u, P.key0, P.key1);
-        const float dtau = free_path(cur);
-        // polar cosine: HG inverse CDF (mc3D:790-800) with the uniform's
-        // scale and offset folded into d and e; isotropic for g = 0
-        const float xf = (float)cur.x;
-        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
-        float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
-        // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
-        float rev = u01f(cur.y);
-        float rev2 = 0.f;
-        int seg = -1;
-        if constexpr (POLAR) {
-            if (pm >= 0) {   // aspherical: phase-matrix sampler
-                polar_angles(P, cost, rev2, seg);
-                rev = 0.5f * rev2;
-            }
-        }
-        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));   // |cost| <= 1
-        const float sinp = __builtin_amdgcn_sinf(rev);
-        const float cosp = __builtin_amdgcn_cosf(rev);
-        // rotation (mc3D:1255-1281); near-vertical incoming directions take the
-        // axis form, behind a wave-uniform branch so that the common path
-        // issues none of its instructions
-        const float s2 = fmaf(-muz, muz, 1.f);
-        const float st = sint * __builtin_amdgcn_rsqf(s2);
-        const float mz0 = muz, mx0 = mux, my0 = muy;
-        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-        muz = fmaf(-s2 * st, cosp, muz * cost);
-        mux = gx; muy = gy;
-        if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
-            if (s2 < 1e-7f) {
-                const float sg = mz0 > 0.f ? 1.f : -1.f;
-                mux = sint * cosp;
-                muy = sg * sint * sinp;
-                muz = sg * cost;
-            }
-        }
-        if constexpr (POLAR) {
-            if (pm >= 0) stokes_update(P, cost, rev2, mx0, my0, mz0, seg);
-        }
-        z = fmaf(dtau, muz, z);
-        tau_acc += dtau;
-        if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all)) return false;
-        // ---- rare: exits and absorption.  The depth passes an empty asm so
-        // that the tests below are issued here, not hoisted into every step.
-        float ze = z;
-        asm volatile("" : "+v"(ze));
-        if (ze > 0.f) {                              // mc3D:1390-1397
-            tau_acc -= fminf(z / muz, dtau);
-            cond = 1;
-            return true;
-        }
-        if (ze < -tau_tot) {                         // mc3D:1399-1459
-            tau_acc -= fminf((z + tau_tot) / muz, dtau);
-            z = -tau_tot;
-            if (lamb && reflects(P)) return reemit(P);
-            cond = 2;
-            return true;
-        }
-        // absorbed: one uniform decides species and absorption jointly, with
-        // P(cond 5) = P_imp (1 - omega_imp), P(cond 4) = (1 - P_imp)(1 - omega_ice)
-        // (mc3D:1374-1383, 1461-1466)
-        cond = (cur.w < t_abs5) ? 5 : 4;
-        return true;
-    }
-
-    // ids from the Philox words: ck0 = case ^ k0, l0 = lo(M0 * photon) with
-    // M0 odd, so photon = l0 * M0^-1 (mod 2^32), M0^-1 = 0x991A7CDB
-    __device__ __forceinline__ uint32_t case_id(const KParams &P) const { return pre.ck0 ^ P.key0; }
-    __device__ __forceinline__ uint32_t photon_id() const { return pre.l0 * 0x991A7CDBu; }
-
-    // Lambertian bottom: reflected with probability R, from Philox block
-    // (photon, case, step, 1)
-    __device__ __forceinline__ bool reflects(const KParams &P) const
-    {
-        const uint32_t cs = case_id(P);
-        const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 1u}, P.key0, P.key1);
-        return (unsigned long long)q.y < P.cases[cs].t_refl;
-    }
-
-    // ---- aspherical particles (POLAR only)
-    __device__ __forceinline__ const float *pm_table(const KParams &P) const
-    {
-        extern __shared__ float pm_lds_tab[];
-        const size_t off = (size_t)pm * (size_t)P.pm_stride;
-        return (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + off;
-    }
-
-    // Scattering angles of mc3D:905-1004: rejection sampling of
-    // I S11(cos) + S12(cos) (Q cos 2phi + U sin 2phi) over cos theta in two
-    // areas split at the cutoff (each bounded by |I| max|P11| + max|P12|
-    // sqrt(Q^2 + U^2), the reference's A + B C) and 2 phi uniform in
-    // [0, 4 pi).  Attempt k of step i draws Philox block (photon, case, i,
-    // 2 + k): area, cos theta, 2 phi, acceptance.  Returns cos theta and
-    // 2 phi in revolutions.
-    __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg) const
-    {
-        const uint32_t cs = case_id(P);
-        const float *T = pm_table(P);
-        const int n = P.pm_n;
-        seg = -1;
-        if (pm_exact) {
-            // The same law without the two-area rejection loop (whose
-            // acceptance is ~10 % on a forward-peaked P11): cos theta from
-            // the marginal S11 by inverting its piecewise-quadratic integral
-            // (make_packet), then 2 phi from 1 + (S12 / S11) (Q cos 2phi +
-            // U sin 2phi) by rejection against 1 + |S12 / S11| sqrt(Q^2 +
-            // U^2) (polar_phi: acceptance >= 1/2, no table reads).  Philox
-            // block (photon, case, i, 2 + k), k = 0, 1, ...
-            const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 2u}, P.key0, P.key1);
-            PolarPacket k;
-            make_packet(q, q, T, n, pm_mass, k);   // its free path and absorption word go unused
-            float r2 = k.r2, sinp, cosp, c2r, s2r;
-            polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
-            cost = k.ct;
-            rev2 = r2;
-            seg = k.seg;
-            return;
-        }
-        const DevCase &C = P.cases[cs];
-        const DevBand &B = P.bands[C.band_begin + band];
-        const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
-        const float ai = fabsf(sI);
-        const float mv1 = fmaf(ai, B.fA1, B.fB1 * cq), mv2 = fmaf(ai, B.fA2, B.fB2 * cq);
-        const float w1 = C.fcmax - C.fccut, w2 = C.fccut - C.fcmin;
-        const float a1 = w1 * mv1, a2 = w2 * mv2;
-        const float pct = a1 / (a1 + a2);
-        const uint32_t ph = photon_id();
-        for (uint32_t k = 0;; ++k) {
-            const u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u + k}, P.key0, P.key1);
-            const bool in1 = u01f(q.x) < pct;
-            const float u = u01f(q.y);
-            const float ct = in1 ? fmaf(u, w1, C.fccut) : fmaf(-u, w2, C.fccut);
-            const float r2 = 2.f * u01f(q.z);
-            const float r3 = 1.000001f * u01f(q.w) * (in1 ? mv1 : mv2);
-            const int j = grid_index(T, n, ct);
-            const float s11 = interp_at(T, T + n, n, j, ct);
-            const float s12 = interp_at(T, T + 2 * n, n, j, ct);
-            const float val = fmaf(sI, s11, s12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
-            if (r3 <= val) {
-                cost = ct;
-                rev2 = r2;
-                return;
-            }
-            if (k + 1 >= POLAR_MAX_TRIES) {
-                atomicAdd(P.ctl + 2, 1ull);
-                cost = ct;
-                rev2 = r2;
-                return;
-            }
-        }
-    }
-
-    // Stokes vector after scattering by (theta, phi) from direction z-cosine
-    // mz0 to muz_n (mc3D:1287-1342): rotate into the scattering plane by
-    // phi_sca = arccos(cos phi) (phi folded into [0, pi]: the double angle's
-    // sine changes sign for phi > pi), apply the phase matrix at cos theta,
-    // rotate back by -i_2 with cos i_2 = (muz_n cos - mz0) / (sign den)
-    // (sign -1 unless phi_sca = pi, as in the reference; cos i_2 = 0 where
-    // den = 0), normalise to I = 1.  The doubled angles of the return
-    // rotation come from cos i_2 directly: cos 2i = 2c^2 - 1, sin 2i = 2c s.
-    __device__ void stokes_update(const KParams &P, float cost, float rev2, float mx0, float my0, float mz0, int seg)
-    {
-        const float *T = pm_table(P);
-        const int n = P.pm_n;
-        const int j = seg >= 0 ? seg : grid_index(T, n, cost);
-        // linear in the segment (slopes precomputed), clamped to the grid
-        const float dx = fminf(fmaxf(cost, T[0]), T[n - 1]) - T[j];
-        const float p11 = fmaf(T[8 * n + j], dx, T[n + j]);
-        const float p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
-        const float p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
-        const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
-        const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
-        const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        stokes_apply(cost, rev2, __builtin_amdgcn_cosf(rev2), __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12,
-                     p22, p33, p43, p44);
-    }
-
-    // (mx0, my0, mz0): the direction before the scattering; the photon's
-    // direction members hold the one after it
-    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions)
-    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float c2r, float s2r, float mx0, float my0,
-                                                 float mz0, float p11, float p12, float p22, float p33, float p43,
-                                                 float p44)
-    {
-#pragma clang fp contract(off)
-        const float c2 = c2r;
-        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * s2r;
-        const float Qs = fmaf(sQ, c2, sU * s2);
-        const float Us = fmaf(-sQ, s2, sU * c2);
-        const float Ic = fmaf(sI, p11, Qs * p12);
-        const float Qc = fmaf(sI, p12, Qs * p22);
-        const float Uc = fmaf(Us, p33, -sV * p43);
-        const float Vc = fmaf(Us, p43, sV * p44);
-        const float muz_n = muz;
-        const float den = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f) *
-                                                 fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
-        // i_2 in [0, pi]: cos i_2 as the reference forms it; sin i_2 >= 0 from
-        // its exact identity (den sin i_2)^2 = (e_z . (u_0 x u_n))^2 -- the
-        // Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2 - cos^2 --
-        // instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1 (scattering
-        // plane close to the meridian plane) would turn an FP32 rounding of
-        // cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 = 0.
-        float ci = 0.f, si = 1.f;
-        if (den != 0.f) {
-            const float rd = __builtin_amdgcn_rcpf(den);
-            const float sg = (rev2 == 1.f) ? 1.f : -1.f;
-            ci = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
-            si = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
-        }
-        const float cr = fmaf(-2.f * si, si, 1.f);     // cos(-2 i_2) = 1 - 2 sin^2 i_2
-        const float sr = -2.f * ci * si;               // sin(-2 i_2)
-        const float inv = __builtin_amdgcn_rcpf(Ic);
-        const float Qm = fmaf(Qc, cr, Uc * sr);
-        const float Um = fmaf(-Qc, sr, Uc * cr);
-        sI = 1.f;
-        sQ = Qm * inv;
-        sU = Um * inv;
-        sV = Vc * inv;
-    }
-
-    // Packet of one step (see PolarPacket) from its main Philox block and its
-    // block (photon, case, step, 2), for a band with the exact sampler: cos
-    // theta by inverting the integral of the linear S11 (guided search), the
-    // six elements there.
-    __device__ static void make_packet(const u32x4 &mainb, const u32x4 &pb, const float *T, int n, float mass,
-                                       PolarPacket &o)
-    {
-        // no implicit contraction in the packet or the step that uses it: a
-        // packet built in another lane and read from LDS must give the same
-        // roundings as one built in place (explicit fmaf only)
-#pragma clang fp contract(off)
-        o.dtau = free_path(mainb);
-        o.w = mainb.w;
-        const float *cum = T + 7 * n;
-        const float u = u01f(pb.x);
-        const float t = u * mass;
-        const int *guide = reinterpret_cast<const int *>(T + 14 * n);
-        const int gk = min((int)(u * (float)PM_GUIDE), PM_GUIDE - 1);
-        int lo = guide[gk], hi = guide[gk + 1] + 1;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (cum[mid] <= t) lo = mid; else hi = mid;
-        }
-        const int j = lo;
-        const float x0 = T[j], h = T[j + 1] - T[j];
-        const float y0 = T[n + j];
-        const float slope = T[8 * n + j];
-        const float dm = fmaxf(t - cum[j], 0.f);
-        const float s = 2.f * dm *
-                        __builtin_amdgcn_rcpf(y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
-        const float ct = fminf(x0 + fminf(fmaxf(s, 0.f), h), T[n - 1]);
-        const float dx = ct - x0;
-        o.ct = ct;
-        o.seg = j;
-        o.p11 = fmaf(slope, dx, y0);
-        o.p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
-        o.p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
-        o.p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
-        o.p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
-        o.p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        o.r2 = 2.f * u01f(pb.y);
-        o.acc = u01f(pb.z);
-    }
-
-    // 2 phi given cos theta: the first candidate r2 (revolutions) and its
-    // acceptance uniform come with the packet, retries draw blocks (photon,
-    // case, i, 2 + k); target I S11 + S12 (Q cos 2phi + U sin 2phi), bound
-    // S11 + |S12| sqrt(Q^2 + U^2).  Returns the accepted r2 with the sine and
-    // cosine of the azimuth (r2 / 2) and of 2 phi (one transcendental pair).
-    __device__ __forceinline__ void polar_phi(const KParams &P, float s11, float s12, float acc, float &r2, float &sinp,
-                                              float &cosp, float &c2r, float &s2r) const
-    {
-#pragma clang fp contract(off)
-        const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
-        for (uint32_t kk = 1;; ++kk) {
-            sinp = __builtin_amdgcn_sinf(0.5f * r2);
-            cosp = __builtin_amdgcn_cosf(0.5f * r2);
-            c2r = fmaf(cosp, cosp, -sinp * sinp);
-            s2r = 2.f * sinp * cosp;
-            const float val = fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r));
-            if (acc * bound <= val) break;
-            if (kk >= POLAR_MAX_TRIES) {
-                atomicAdd(P.ctl + 2, 1ull);
-                break;
-            }
-            const u32x4 q = philox4x32_10(u32x4{photon_id(), case_id(P), i, 2u + kk}, P.key0, P.key1);
-            r2 = 2.f * u01f(q.x);
-            acc = u01f(q.y);
-        }
-    }
-
-    // One step of an exact-sampler photon given its packet: 2 phi by
-    // rejection against the current Stokes vector (retries draw blocks
-    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
-    // one-lane walk and the tail mode below both step through here, so a
-    // photon's record does not depend on which of them walked it.
-    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
-    {
-#pragma clang fp contract(off)
-        ++i;
-        float r2 = k.r2, sinp, cosp, c2r, s2r;
-        polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
-        const float cost = k.ct;
-        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
-        const float s2 = fmaf(-muz, muz, 1.f);
-        const float st = sint * __builtin_amdgcn_rsqf(s2);
-        const float mz0 = muz, mx0 = mux, my0 = muy;
-        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
-        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
-        muz = fmaf(-s2 * st, cosp, muz * cost);
-        mux = gx; muy = gy;
-        if (s2 < 1e-7f) {
-            const float sg = mz0 > 0.f ? 1.f : -1.f;
-            mux = sint * cosp;
-            muy = sg * sint * sinp;
-            muz = sg * cost;
-        }
-        stokes_apply(cost, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
-        const float dtau = k.dtau;
-        z = fmaf(dtau, muz, z);
-        tau_acc += dtau;
-        if (!(z > 0.f || z < -tau_tot || k.w < t_abs_all)) return false;
-        if (z > 0.f) {                               // mc3D:1390-1397
-            tau_acc -= fminf(z / muz, dtau);
-            cond = 1;
-            return true;
-        }
-        if (z < -tau_tot) {                          // mc3D:1399-1459
-            tau_acc -= fminf((z + tau_tot) / muz, dtau);
-            z = -tau_tot;
-            if (lamb && reflects(P)) {
-                r = philox_step(pre, i + 1u, P.key0, P.key1);   // the re-emission step's block
-                return reemit(P);
-            }
-            cond = 2;
-            return true;
-        }
-        cond = (k.w < t_abs5) ? 5 : 4;
-        return true;
-    }
-
-    // Lambertian re-emission of an aspherical-grain photon (cos theta already
-    // set by lambertian_direction): the azimuth is the phase-matrix
-    // sampler's (the reference keeps phi_rand there); with `update` the
-    // Stokes vector scatters at the Lambertian angle from mz0 = 1.
-    __device__ void polar_lambertian(const KParams &P, bool update)
-    {
-        float ct, rev2;
-        int seg;
-        polar_angles(P, ct, rev2, seg);
-        const float cost = muz;
-        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
-        const float rv = 0.5f * rev2;
-        mux = sint * __builtin_amdgcn_cosf(rv);
-        muy = sint * __builtin_amdgcn_sinf(rv);
-        if (update) stokes_update(P, cost, rev2, 0.f, 0.f, 1.f, -1);
-    }
-
-    __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const
-    {
-        theta = acosf(fminf(fmaxf(muz, -1.f), 1.f));
-        path = tau_acc * P.bands[P.cases[case_id].band_begin + band].inv_ext;
-        axis = false;
-        if (i == 1) {
-            phi = 0.f;
-        } else {
-            float ph = atan2f(muy, mux);
-            if (ph < 0.f) ph += TWO_PI_F;
-            phi = ph;
-            axis = (mux == 0.f) || (muy == 0.f);
-        }
-    }
-};
-
-using NativeWalker = NativeWalkerT<false>;
-using PolarWalker = NativeWalkerT<true>;
-
-// ----------------------------------------------------------- replay walker
-// FP64, the reference's operation order; contraction off so that every
-// product and sum rounds as numpy's float64 scalars do.
-struct ReplayWalker {
-    using Real = double;
-    static constexpr bool COOP = false;
-    static constexpr bool STOKES = true;
-    static constexpr int MAX_BLOCK = 256;
-    static constexpr int WAVES_PER_EU = 2;
-    double z, mux, muy, muz, muz2, path;
-    double u_ssa, u_ext, dtau0;
-    double g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, tau_tot, R;
-    int64_t pos, end;
-    uint32_t i;
-    int band;
-    bool bounce, lamb, err, surf;
-    int cond;
-    // aspherical particles: Stokes vector and the band's phase-matrix table
-    double sI, sQ, sU, sV;
-    int pm;
-    int case_id;
-
-    __device__ double draw(const KParams &P)
-    {
-        if (pos >= end) {
-            err = true;
-            return 0.5;
-        }
-        return P.rdev[pos++];
-    }
-
-    // set up the photon; its steps all run in step() (returns false)
-    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
-    {
-        (void)photon;
-        const DevCase &C = P.cases[case_id];
-        band = P.photon_band ? P.photon_band[gid] : 0;
-        const DevBand &B = P.bands[C.band_begin + band];
-        g = B.g; ssa_ice = B.ssa_ice; ssa_imp = B.ssa_imp; ext_cff = B.ext_cff;
-        p_ext_imp = B.p_ext_imp;
-        tau_tot = C.tau_tot; R = C.r_lamb; lamb = C.lamb != 0;
-        surf = C.surf != 0;
-        pos = P.roff[gid];
-        end = P.roff[gid + 1];
-        err = false;
-        this->case_id = case_id;
-        pm = C.polar ? B.pm : -1;
-        sI = C.s0[0]; sQ = C.s0[1]; sU = C.s0[2]; sV = C.s0[3];
-        dtau0 = -log(draw(P));
-        u_ssa = draw(P);
-        u_ext = draw(P);
-        z = 0.0; mux = C.mux0; muy = 0.0; muz = C.muz0;
-        muz2 = muz * muz;
-        path = 0.0;
-        i = 0;
-        bounce = false;
-        cond = 0;
-        return false;
-    }
-
-    __device__ bool step(const KParams &P)
-    {
-#pragma clang fp contract(off)
-        ++i;
-        double dtau, u_hg = 0.0, u_phi = 0.0, p_cos = 0.0, two_phi = 0.0;
-        if (i >= 2) {
-            if (pm >= 0) {
-                polar_draws(P, p_cos, two_phi);
-            } else {
-                u_hg = draw(P);
-                u_phi = draw(P);
-            }
-            dtau = -log(draw(P));
-            u_ssa = draw(P);
-            u_ext = draw(P);
-        } else {
-            dtau = surf ? 0.0 : dtau0;   // mc3D:1228-1229
-        }
-        double mxn = mux, myn = muy, mzn = muz;
-        if (i > 1) {
-            double cost, sint;
-            if (bounce || surf) {
-                mux = 0.0; muy = 0.0; muz = 1.0;
-                for (;;) {
-                    const double th = 0.0 + (PI_D / 2) * draw(P);
-                    const double r1 = draw(P);
-                    if (err) { cost = 1.0; sint = 0.0; break; }
-                    if (r1 < 2.0 * sin(th) * cos(th)) {
-                        cost = cos(th);
-                        sint = sqrt(1.0 - cost * cost);
-                        break;
-                    }
-                }
-            } else if (pm >= 0) {
-                cost = p_cos;
-                sint = sqrt(1.0 - cost * cost);
-            } else if (g == 0.0) {
-                cost = 1.0 - 2.0 * u_hg;
-                sint = sqrt(1.0 - cost * cost);
-            } else {
-                const double t = (1.0 - g * g) / ((1.0 - g) + (2.0 * g) * u_hg);
-                cost = (1.0 / (2.0 * g)) * ((1.0 + g * g) - t * t);
-                sint = sqrt(1.0 - cost * cost);
-            }
-            const double ph = (pm >= 0) ? two_phi / 2.0 : u_phi * (2.0 * PI_D);
-            const double cosp = cos(ph), sinp = sin(ph);
-            if (muz == 1.0) {
-                mxn = sint * cosp; myn = sint * sinp; mzn = cost;
-            } else if (muz == -1.0) {
-                mxn = sint * cosp; myn = -sint * sinp; mzn = -cost;
-            } else {
-                const double sq = sqrt(1.0 - muz2);
-                mxn = (sint * (mux * muz * cosp - muy * sinp)) / sq + mux * cost;
-                myn = (sint * (muy * muz * cosp + mux * sinp)) / sq + muy * cost;
-                mzn = -sq * sint * cosp + muz * cost;
-            }
-            if (bounce) {                        // mc3D:1283-1285
-                sI = 1.0; sQ = 0.0; sU = 0.0; sV = 0.0;
-            } else if (pm >= 0) {                // mc3D:1287-1342
-                stokes_update(P, cost, cosp, mzn, muz);
-            }
-            bounce = false;
-        }
-        const double z_prev = z;
-        const double z_i = z_prev + dtau * mzn;
-        z = z_i;
-        if (i > 1) {
-            mux = mxn; muy = myn; muz = mzn;
-            muz2 = muz * muz;
-        }
-        path += dtau / ext_cff;
-        const bool ice = u_ext > p_ext_imp;
-        const double ssa_event = surf ? R : (ice ? ssa_ice : ssa_imp);   // mc3D:1374-1387
-        if (z_i > 0.0) {
-            path += -((z_i * dtau) / ((z_i - z_prev) * ext_cff));
-            cond = 1;
-            return true;
-        }
-        if (z_i < -tau_tot) {
-            path += -(((z_i + tau_tot) * dtau) / ((z_i - z_prev) * ext_cff));
-            const double dcorr = -(((z_i + tau_tot) / (z_prev - z_i)) * dtau);
-            z = z_i - (mzn * dcorr);
-            const int exit_cond = (i == 1) ? 3 : 2;
-            if (lamb) {
-                if (draw(P) <= R) {
-                    bounce = true;
-                    return err;
-                }
-            }
-            cond = exit_cond;
-            return true;
-        }
-        if (u_ssa >= ssa_event) {
-            cond = ice ? 4 : 5;
-            return true;
-        }
-        return err;
-    }
-
-    // populate_pdfs for an aspherical particle (mc3D:898-1004): area, cos
-    // theta and phi, then the rejection loop (r3; after a rejection: area,
-    // cos theta, 2 phi, r3) against I S11 + S12 (Q cos 2phi + U sin 2phi).
-    // The reference's operation order and float64 functions.
-    __device__ void polar_draws(const KParams &P, double &ct, double &two_phi)
-    {
-#pragma clang fp contract(off)
-        const DevCase &C = P.cases[case_id];
-        const DevBand &B = P.bands[C.band_begin + band];
-        const int n = P.pm_n;
-        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
-        const double I = sI, Q = sQ, U = sU;
-        const double two_beta = (Q == 0.0) ? atan(U / 1e-15) : atan(U / Q);
-        double A = fabs(I) * B.pA1;
-        double Bm = B.pB1;
-        const double Cc = fabs(Q * cos(two_beta) + U * sin(two_beta));
-        const double mv1 = A + Bm * Cc;
-        A = fabs(I) * B.pA2;
-        Bm = B.pB2;
-        const double mv2 = A + Bm * Cc;
-        const double a1 = (C.cmax - C.ccut) * mv1;
-        const double a2 = (C.ccut - C.cmin) * mv2;
-        const double pct = a1 / (a1 + a2);
-        double ar = draw(P);
-        ct = draw(P);
-        two_phi = 2.0 * (draw(P) * (2.0 * PI_D));
-        int area;
-        if (ar < pct) {
-            area = 1;
-            ct = ct * (C.cmax - C.ccut) + C.ccut;
-        } else {
-            area = 2;
-            ct = -ct * (C.ccut - C.cmin) + C.ccut;
-        }
-        double r3 = 1.0, val = 0.0;
-        for (int k = 0; r3 > val; ++k) {
-            if (k > 0) {
-                ar = draw(P);
-                if (ar < pct) {
-                    area = 1;
-                    ct = draw(P) * (C.cmax - C.ccut) + C.ccut;
-                } else {
-                    area = 2;
-                    ct = -draw(P) * (C.ccut - C.cmin) + C.ccut;
-                }
-                two_phi = draw(P) * (4.0 * PI_D);
-            }
-            r3 = 1.000001 * draw(P);
-            r3 = (area == 1) ? r3 * mv1 : r3 * mv2;
-            const int j = grid_index(x, n, ct);
-            const double S11 = interp_at(x, x + n, n, j, ct);
-            const double S12 = interp_at(x, x + 2 * n, n, j, ct);
-            val = I * S11 + S12 * (Q * cos(two_phi) + U * sin(two_phi));
-            if (err) break;
-        }
-    }
-
-    // mc3D:1287-1342 and rotate_stokes_vector (mc3D:1090-1109), the
-    // reference's order and float64 functions.
-    __device__ void stokes_update(const KParams &P, double costheta, double cosphi, double muz_n, double muz_0)
-    {
-#pragma clang fp contract(off)
-        const int n = P.pm_n;
-        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
-        const double cos2theta = costheta * costheta;
-        const double phi_sca = acos(cosphi);
-        double da = 2.0 * phi_sca;
-        double c = cos(da), s = sin(da);
-        const double Isp = sI, Qsp = sQ * c + sU * s, Usp = -sQ * s + sU * c, Vsp = sV;
-        const int j = grid_index(x, n, costheta);
-        const double P11 = interp_at(x, x + n, n, j, costheta);
-        const double P12 = interp_at(x, x + 2 * n, n, j, costheta);
-        const double P22 = interp_at(x, x + 3 * n, n, j, costheta);
-        const double P33 = interp_at(x, x + 4 * n, n, j, costheta);
-        const double P43 = interp_at(x, x + 5 * n, n, j, costheta);
-        const double P44 = interp_at(x, x + 6 * n, n, j, costheta);
-        const double Ic = Isp * P11 + Qsp * P12;
-        const double Qc = Isp * P12 + Qsp * P22;
-        const double Uc = Usp * P33 - Vsp * P43;
-        const double Vc = Usp * P43 + Vsp * P44;
-        const double den = sqrt((1.0 - cos2theta) * (1.0 - muz_n * muz_n));
-        double ci;
-        if (den == 0.0) {
-            ci = 0.0;
-        } else {
-            const double num = muz_n * costheta - muz_0;
-            const double sg = (phi_sca >= PI_D) ? 1.0 : -1.0;
-            ci = num / (sg * den);
-            if (ci < -1.0) ci = -1.0;
-            else if (ci > 1.0) ci = 1.0;
-        }
-        const double i2 = acos(ci);
-        da = 2.0 * (-i2);
-        c = cos(da);
-        s = sin(da);
-        const double Qm = Qc * c + Uc * s, Um = -Qc * s + Uc * c;
-        sI = Ic / Ic;
-        sQ = Qm / Ic;
-        sU = Um / Ic;
-        sV = Vc / Ic;
-    }
-
-    __device__ void result(const KParams &, int, double &theta, double &phi, double &pth, bool &axis) const
-    {
-#pragma clang fp contract(off)
-        theta = acos(muz);
-        pth = path;
-        axis = false;
-        if (i == 1) phi = 0.0;
-        else if (mux > 0 && muy > 0) phi = atan(muy / mux);
-        else if (mux < 0 && muy > 0) phi = atan(muy / mux) + PI_D;
-        else if (mux < 0 && muy < 0) phi = atan(muy / mux) + PI_D;
-        else if (mux > 0 && muy < 0) phi = atan(muy / mux) + 2.0 * PI_D;
-        else { phi = __builtin_nan(""); axis = true; }
-    }
-};
-
-
-// Lanes are EMPTY (no photon), LIVE or DONE (finished, record pending).
-enum { EMPTY = 0, LIVE = 1, DONE = 2 };
-
-// ------------------------------------------------- wave-cooperative tail mode
-// Once a block's queue is empty, a wave's last photons would walk one step
-// per wave instruction stream (~1,100 cycles per step for a lone wave), and
-// the longest walks of a launch (~1e4 steps) set its duration.  Here the
-// G = 64 / L lanes of a group walk 2G consecutive steps of one of the L live
-// photons at once:
-//  * lane j of the group draws the Philox blocks of steps i + 1 + 2j and
-//    i + 2 + 2j -- the counters the one-lane walk uses, so every step keeps
-//    its uniforms;
-//  * a step is "scatter by (theta_k, phi_k) about the current direction, then
-//    move dtau_k": with the direction carried as the third column of a frame
-//    quaternion F, F_k = F_{k-1} (x) L_k (L_k = Rz(phi_k) Ry(theta_k)).  Each
-//    lane multiplies its two steps' L, one inclusive product scan over the
-//    group (DPP row shifts + row broadcasts, lower lanes on the left) gives
-//    the frame after each lane's second step, and the frame after its first
-//    is that one times the conjugate of its second L; depth and optical path
-//    are one prefix sum of both steps' dtau_k * uz_k and dtau_k;
-//  * the first step (in walk order) that exits or absorbs (or reaches
-//    max_steps) ends the segment; the photon's lane takes that step's state,
-//    otherwise the last lane's, and continues.
-// Azimuths are measured in the transported frame instead of the lab-axis
-// frame of mc3D:1255-1281.  phi_k is uniform and independent of the history,
-// so the walk has the same law; events use the uniforms of their own step,
-// so no step is re-drawn.
-
-struct Quat {
-    float w, x, y, z;
-};
-
-__device__ __forceinline__ Quat qmul(const Quat &A, const Quat &B)
-{
-    return Quat{fmaf(A.w, B.w, -fmaf(A.x, B.x, fmaf(A.y, B.y, A.z * B.z))),
-                fmaf(A.w, B.x, fmaf(A.x, B.w, fmaf(A.y, B.z, -A.z * B.y))),
-                fmaf(A.w, B.y, fmaf(-A.x, B.z, fmaf(A.y, B.w, A.z * B.x))),
-                fmaf(A.w, B.z, fmaf(A.x, B.y, fmaf(-A.y, B.x, A.z * B.w)))};
-}
-
-// DPP lane move: CTRL 0x111..0x118 row_shr:1..8, 0x142 row_bcast:15,
-// 0x143 row_bcast:31; rows outside ROWS and lanes without a source get `old`.
-template <int CTRL, int ROWS>
-__device__ __forceinline__ float dppf(float old, float v)
-{
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL, ROWS,
-                                                      0xF, false));
-}
-
-// One scan level.  Lanes without a source (outside the row, or rows not in
-// ROWS) read the identity, so they keep q; `take` is needed only where the
-// source lane is in the row but in another group (groups under 16 lanes).
-template <int CTRL, int ROWS, bool SEL>
-__device__ __forceinline__ void qlevel(Quat &q, bool take)
-{
-    const Quat y{dppf<CTRL, ROWS>(1.f, q.w), dppf<CTRL, ROWS>(0.f, q.x), dppf<CTRL, ROWS>(0.f, q.y),
-                 dppf<CTRL, ROWS>(0.f, q.z)};
-    const Quat m = qmul(y, q);
-    if constexpr (SEL) {
-        q.w = take ? m.w : q.w;
-        q.x = take ? m.x : q.x;
-        q.y = take ? m.y : q.y;
-        q.z = take ? m.z : q.z;
-    } else {
-        q = m;
-    }
-}
-
-template <int CTRL, int ROWS, bool SEL>
-__device__ __forceinline__ void slevel(float &s0, float &s1, bool take)
-{
-    const float t0 = dppf<CTRL, ROWS>(0.f, s0), t1 = dppf<CTRL, ROWS>(0.f, s1);
-    if constexpr (SEL) {
-        s0 = take ? s0 + t0 : s0;
-        s1 = take ? s1 + t1 : s1;
-    } else {
-        s0 += t0;
-        s1 += t1;
-    }
-}
-
-// Inclusive scans within groups of G = 2^lg lanes (jj = lane within group):
-// q <- q_0 (x) q_1 (x) ... (x) q_jj.  Levels 1..8 stay inside 16-lane rows;
-// row_bcast:15 then joins row pairs (rows 1, 3) and row_bcast:31 the halves.
-__device__ __forceinline__ void group_scan_q(Quat &q, int lg, int jj)
-{
-    if (lg < 4) {
-        if (lg >= 1) qlevel<0x111, 0xF, true>(q, jj >= 1);
-        if (lg >= 2) qlevel<0x112, 0xF, true>(q, jj >= 2);
-        if (lg >= 3) qlevel<0x114, 0xF, true>(q, jj >= 4);
-    } else {
-        // groups of whole rows: row scans, then row pairs and halves joined
-        qlevel<0x111, 0xF, false>(q, true);
-        qlevel<0x112, 0xF, false>(q, true);
-        qlevel<0x114, 0xF, false>(q, true);
-        qlevel<0x118, 0xF, false>(q, true);
-        if (lg >= 5) qlevel<0x142, 0xA, false>(q, true);
-        if (lg >= 6) qlevel<0x143, 0xC, false>(q, true);
-    }
-}
-
-// the same for two prefix sums
-__device__ __forceinline__ void group_scan_s(float &s0, float &s1, int lg, int jj)
-{
-    if (lg < 4) {
-        if (lg >= 1) slevel<0x111, 0xF, true>(s0, s1, jj >= 1);
-        if (lg >= 2) slevel<0x112, 0xF, true>(s0, s1, jj >= 2);
-        if (lg >= 3) slevel<0x114, 0xF, true>(s0, s1, jj >= 4);
-    } else {
-        slevel<0x111, 0xF, false>(s0, s1, true);
-        slevel<0x112, 0xF, false>(s0, s1, true);
-        slevel<0x114, 0xF, false>(s0, s1, true);
-        slevel<0x118, 0xF, false>(s0, s1, true);
-        if (lg >= 5) slevel<0x142, 0xA, false>(s0, s1, true);
-        if (lg >= 6) slevel<0x143, 0xC, false>(s0, s1, true);
-    }
-}
-
-__device__ __forceinline__ float bperm(int src_lane, float v)
-{
-    return __int_as_float(__builtin_amdgcn_ds_bpermute(src_lane << 2, __float_as_int(v)));
-}
-__device__ __forceinline__ uint32_t bperm(int src_lane, uint32_t v)
-{
-    return (uint32_t)__builtin_amdgcn_ds_bpermute(src_lane << 2, (int)v);
-}
-
-// Per-wave state of the cooperative mode.  On entry every live photon of the
-// wave writes its constants once into its own entry of the wave's LDS table
-// (index = its rank at entry), so the walker's constants and its pipelined
-// Philox block need no registers during the phase.  Groups are sticky: while
-// no photon of the wave exits, is absorbed or joins, each group keeps its
-// photon's state in its own lanes and carries it from its last lane to the
-// next segment; the photon's lane is updated (and groups rebuilt) only after a
-// round with an event.
-struct CoopEntry {
-    uint32_t ck0, h0, l0, t_abs;     // Philox words, absorption threshold
-    float a, b, cc, d;               // HG constants
-    float e, tau_tot, z, tau;        // HG constant, layer depth; state at grouping
-    float dx, dy, dz;
-    uint32_t i;
-};
-
-struct CoopGroups {
-    unsigned long long em;   // photon lanes of the current grouping (0: rebuild)
-    int lg;                  // G = 2^lg lanes per photon
-    int ent;                 // table entry of this lane's group's photon
-    PhiloxPre pre;           // that photon's Philox words
-    float z0, tau0;          // its state before the next segment: depth, path,
-    Quat f;                  // direction frame (third column = direction),
-    uint32_t i0;             // steps taken
-};
-
-// direction = third column of the rotation of unit quaternion q
-__device__ __forceinline__ void frame_dir(const Quat &q, float &x, float &y, float &z)
-{
-    x = 2.f * fmaf(q.x, q.z, q.w * q.y);
-    y = 2.f * fmaf(q.y, q.z, -q.w * q.x);
-    z = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
-}
-
-// a frame whose third column is the unit vector d: the rotation taking e_z to d
-__device__ __forceinline__ Quat frame_of(float dx, float dy, float dz)
-{
-    const bool up = dz >= 0.f;
-    const Quat f = up ? Quat{1.f + dz, -dy, dx, 0.f} : Quat{-dy, 1.f - dz, 0.f, dx};
-    const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
-    return Quat{f.w * n, f.x * n, f.y * n, f.z * n};
-}
-
-__device__ __forceinline__ float rdlane63(float v)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-
-// Groups carrying a photon's state (no event in their last rounds) hand it
-// back to the photon's lane.
-__device__ __forceinline__ void coop_handback(NativeWalker &w, const CoopGroups &cg, unsigned long long *coop_steps)
-{
-    const int lane = lane_id();
-    const bool was = (cg.em >> lane) & 1ull;
-    const int src = was ? (lane_rank(cg.em) << cg.lg) : lane;
-    const float z = bperm(src, cg.z0), t = bperm(src, cg.tau0);
-    float x, y, u;
-    frame_dir(cg.f, x, y, u);
-    x = bperm(src, x); y = bperm(src, y); u = bperm(src, u);
-    const uint32_t i = bperm(src, cg.i0);
-    if (was) {
-        atomicAdd(coop_steps, (unsigned long long)(i - w.i));
-        w.i = i; w.z = z; w.tau_acc = t; w.mux = x; w.muy = y; w.muz = u;
-    }
-}
-
-// A step's free path and local rotation L = Rz(phi) Ry(theta) (from the half
-// angles) from its Philox block, with the HG constants of the photon (the
-// same draws as NativeWalker::step).
-__device__ __forceinline__ void coop_local(const u32x4 &cur, float ca, float cb, float ccc, float cd, float ce,
-                                           float &dtau, Quat &L)
-{
-    dtau = -LN2_F * __builtin_amdgcn_logf(u01f(cur.z));
-    const float xf = (float)cur.x;
-    const float t = ccc * __builtin_amdgcn_rcpf(fmaf(ce, xf, cd));
-    // isotropic bands carry e = 0
-    const float cost = (ce == 0.f) ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(ca * fmaf(-t, t, cb), -1.f), 1.f);
-    const float hc = __builtin_amdgcn_sqrtf(fmaf(0.5f, cost, 0.5f));    // cos(theta/2); |cost| <= 1
-    const float hs = __builtin_amdgcn_sqrtf(fmaf(-0.5f, cost, 0.5f));   // sin(theta/2)
-    const float hrev = 0.5f * u01f(cur.y);                                          // phi/2 in revolutions
-    const float ha = __builtin_amdgcn_cosf(hrev), hb = __builtin_amdgcn_sinf(hrev);
-    L = Quat{ha * hc, -hb * hs, ha * hs, hb * hc};
-}
-
-// One cooperative round of a wave (all 64 lanes call; no divergence around
-// the lane permutes).  `tab`: the wave's entries; `map`: group -> entry;
-// `me`: this lane's photon's entry.
-__device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab,
-                                           int *map, int me, CoopGroups &cg, unsigned long long *coop_steps)
-{
-    const int lane = lane_id();
-    const bool elig = state == LIVE;
-    const unsigned long long em = __ballot(elig);
-    if (em != cg.em) {
-        // a photon joined: hand the carried state back before regrouping
-        if (cg.em) coop_handback(w, cg, coop_steps);
-        cg.em = em;
-        if (em == 0ull) return;
-        const int L = __popcll(em);
-        cg.lg = L <= 1 ? 6 : 6 - (32 - __clz(L - 1));
-        // group g walks the live photon of rank g: it posts its entry and state
-        if (elig) {
-            CoopEntry &E = tab[me];
-            E.z = w.z; E.tau = w.tau_acc; E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
-            __atomic_store_n(&map[lane_rank(em)], me, __ATOMIC_RELAXED);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        cg.ent = __atomic_load_n(&map[min(lane >> cg.lg, L - 1)], __ATOMIC_RELAXED);
-        const CoopEntry &E = tab[cg.ent];
-        cg.pre = PhiloxPre{E.ck0, E.h0, E.l0};
-        cg.z0 = E.z; cg.tau0 = E.tau; cg.i0 = E.i;
-        cg.f = frame_of(E.dx, E.dy, E.dz);
-    }
-    const int lg = cg.lg;
-    const int G = 1 << lg;
-    const int jj = lane & (G - 1);
-    const bool member = (lane >> lg) < __popcll(em);
-    const CoopEntry &E = tab[cg.ent];
-    const float ca = E.a, cb = E.b, ccc = E.cc, cd = E.d, ce = E.e, ctau = E.tau_tot;
-    const uint32_t ct_abs = E.t_abs;
-    // lane jj walks steps ka = i0 + 1 + 2 jj and kb = ka + 1 (the same Philox
-    // blocks the one-lane walk draws for them)
-    const uint32_t ka = cg.i0 + 1u + 2u * (uint32_t)jj, kb = ka + 1u;
-    const u32x4 ua = philox_step(cg.pre, ka, P.key0, P.key1);
-    const u32x4 ub = philox_step(cg.pre, kb, P.key0, P.key1);
-    float dta, dtb;
-    Quat La, Lb;
-    coop_local(ua, ca, cb, ccc, cd, ce, dta, La);
-    coop_local(ub, ca, cb, ccc, cd, ce, dtb, Lb);
-    Quat q = qmul(La, Lb);
-    if (jj == 0) q = qmul(cg.f, q);   // the group's carried frame on the left
-    // frames first: the depth changes need the directions
-    group_scan_q(q, lg, jj);
-    // frame after step ka: F_b (x) conj(L_b)
-    const Quat qa = qmul(q, Quat{Lb.w, -Lb.x, -Lb.y, -Lb.z});
-    const float uza = fmaf(-2.f, fmaf(qa.x, qa.x, qa.y * qa.y), 1.f);
-    const float uzb = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
-    const float dzb = dtb * uzb;
-    float sz = fmaf(dta, uza, dzb), sp = dta + dtb;
-    group_scan_s(sz, sp, lg, jj);
-    const float zb = cg.z0 + sz, pb = cg.tau0 + sp;
-    const float za = zb - dzb, pa = pb - dtb;
-    const bool top_a = za > 0.f && uza > 0.f, bot_a = za < -ctau && uza < 0.f;
-    const bool top_b = zb > 0.f && uzb > 0.f, bot_b = zb < -ctau && uzb < 0.f;
-    const bool eva = member && (top_a || bot_a || ua.w < ct_abs || ka >= P.max_steps);
-    const bool evb = member && (top_b || bot_b || ub.w < ct_abs || kb >= P.max_steps);
-    const unsigned long long ev = __ballot(eva || evb);
-    if (ev == 0ull) {
-        // no event: every group walked 2G steps; carry its last lane's state
-        // (the frame renormalised against drift)
-        Quat f;
-        if (lg == 6) {
-            cg.z0 = rdlane63(zb); cg.tau0 = rdlane63(pb);
-            f = Quat{rdlane63(q.w), rdlane63(q.x), rdlane63(q.y), rdlane63(q.z)};
-        } else {
-            const int src = lane | (G - 1);
-            cg.z0 = bperm(src, zb); cg.tau0 = bperm(src, pb);
-            f = Quat{bperm(src, q.w), bperm(src, q.x), bperm(src, q.y), bperm(src, q.z)};
-        }
-        const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
-        cg.f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
-        cg.i0 += 2u * (uint32_t)G;
-        return;
-    }
-    // this lane's outcome, should one of its steps end the segment: its first
-    // step if that one has an event, else its second
-    const bool A = eva;
-    const bool top = A ? top_a : top_b, bot = A ? bot_a : bot_b;
-    const uint32_t wv = A ? ua.w : ub.w, k = A ? ka : kb;
-    const bool ab = wv < ct_abs, cap = k >= P.max_steps;
-    const float zk = A ? za : zb, pk = A ? pa : pb, dtau = A ? dta : dtb, uz = A ? uza : uzb;
-    const Quat qe{A ? qa.w : q.w, A ? qa.x : q.x, A ? qa.y : q.y, A ? qa.z : q.z};
-    const float ox = 2.f * fmaf(qe.x, qe.z, qe.w * qe.y);
-    const float oy = 2.f * fmaf(qe.y, qe.z, -qe.w * qe.x);
-    const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
-    const float corr = top ? zk : zk + ctau;
-    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
-    const float zout = bot ? -ctau : zk;
-    // each photon's lane reads the first event lane of its group (or the last)
-    int src = lane;
-    if (elig) {
-        const int base = lane_rank(em) << lg;
-        const unsigned long long gm = (ev >> base) & (lg == 6 ? ~0ull : ((1ull << G) - 1ull));
-        src = base + (gm ? (__ffsll((long long)gm) - 1) : G - 1);
-    }
-    const int r_type = (int)bperm(src, (uint32_t)type);
-    const uint32_t r_k = bperm(src, k);
-    const uint32_t r_w = bperm(src, wv);
-    const float r_z = bperm(src, zout), r_p = bperm(src, pout);
-    const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uz);
-    cg.em = 0ull;   // photons' lanes hold the state again: regroup next round
-    if (elig) {
-        atomicAdd(coop_steps, (unsigned long long)(r_k - w.i));
-        w.i = r_k;
-        w.z = r_z;
-        w.tau_acc = r_p;
-        w.mux = r_x; w.muy = r_y; w.muz = r_uz;
-        if (r_type == 1) {
-            w.cond = 1;
-            state = DONE;
-        } else if (r_type == 2) {
-            if (P.cases[w.case_id(P)].lamb && w.reflects(P)) {
-                // re-emitted from the bottom: the next step, taken here
-                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);
-                if (w.reemit(P)) {
-                    state = DONE;
-                } else if (w.i >= P.max_steps) {
-                    w.cond = 0;
-                    state = DONE;
-                }
-            } else {
-                w.cond = 2;
-                state = DONE;
-            }
-        } else if (r_type == 3) {
-            const uint32_t cs = w.case_id(P);
-            w.cond = (r_w < P.bands[P.cases[cs].band_begin + w.band].t_abs5) ? 5 : 4;
-            state = DONE;
-        } else if (r_type == 4) {
-            w.cond = 0;
-            state = DONE;
-        }
-    }
-}
-
-// Hand-over within the block.  A wave that has finished its photons sets its
-// bit in `idle_mask` and polls its own inbox; a wave in the cooperative phase
-// with L >= 2 photons claims a waiting wave (clears its bit), hands its
-// carried group state back to the photon lanes, writes the upper half of its
-// photons' walk state into their table entries and posts entry mask and wave
-// to the claimed wave's inbox; those lanes are emptied.  Each wave has its own
-// inbox, so hand-overs to different waves proceed at once.  The photons keep
-// their chunk slot (found again from case and photon id), so tallies, records
-// and chunk completion are unchanged; each photon still walks its own Philox
-// stream.  Spread over the block's SIMDs, the last long walks get more lanes.
-__device__ __forceinline__ void coop_donate(BlockState &bs, NativeWalker &w, int &state, CoopEntry *tab, int me,
-                                            CoopGroups &cg, unsigned long long *coop_steps, unsigned long long *moved,
-                                            int wave)
-{
-    const int lane = lane_id();
-    const unsigned long long live = __ballot(state == LIVE);
-    const int L = __popcll(live);
-    if (L < 2) return;
-    unsigned int im = lds_load(&bs.idle_mask);
-    if (im == 0u) return;
-    int v = -1;
-    if (lane == 0) {
-        while (im) {
-            const int c = __ffs(im) - 1;
-            const unsigned int old = atomicAnd(&bs.idle_mask, ~(1u << c));
-            if ((old >> c) & 1u) {
-                v = c;
-                break;
-            }
-            im = old & ~(1u << c);
-        }
-    }
-    v = bcast(v, 0);
-    if (v < 0) return;
-    if (cg.em) coop_handback(w, cg, coop_steps);
-    cg.em = 0ull;   // regroup next round
-    const int nd = L >> 1;
-    if (lane == 0) {
-        bs.ib_mask[v] = 0u;
-        bs.ib_src[v] = wave;
-        atomicAdd(&bs.out_posts[wave], 1);
-    }
-    if (state == LIVE && lane_rank(live) >= L - nd) {
-        CoopEntry &E = tab[me];
-        E.z = w.z; E.tau = w.tau_acc; E.dx = w.mux; E.dy = w.muy; E.dz = w.muz; E.i = w.i;
-        atomicOr(&bs.ib_mask[v], 1u << me);
-        atomicAdd(moved, 1ull);
-        state = EMPTY;
-    }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (lane == 0) __atomic_store_n(&bs.ib_state[v], 1, __ATOMIC_RELAXED);
-}
-
-// The cooperative phase of a wave: every photon the wave still holds is
-// finished (or, with HAND, handed over) when it returns.  `tab`: coop_live
-// entries; `map`: coop_live group slots.
-template <bool HAND>
-__device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int &state, const KParams &P,
-                                           CoopEntry *tab, int *map, unsigned long long *coop_steps,
-                                           unsigned long long *moved, int wave)
-{
-    const unsigned long long live0 = __ballot(state == LIVE);
-    const int me = lane_rank(live0);
-    if (state == LIVE) {
-        CoopEntry &E = tab[me];
-        E.ck0 = w.pre.ck0; E.h0 = w.pre.h0; E.l0 = w.pre.l0; E.t_abs = w.t_abs_all;
-        E.a = w.a; E.b = w.b; E.cc = w.cc; E.d = w.d;
-        E.e = w.e; E.tau_tot = w.tau_tot;
-    }
-    CoopGroups cg;
-    cg.em = 0ull;
-    cg.lg = 0;
-    cg.ent = 0;
-    for (uint32_t round = 1; __ballot(state == LIVE); ++round) {
-        coop_round(w, state, P, tab, map, me, cg, coop_steps);
-        if constexpr (HAND) {
-            if ((round & 1u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
-        }
-    }
-}
-
-// Wait for photons posted to this wave's inbox (all lanes EMPTY on entry).
-// True with the posted photons LIVE on the first lanes; false once no sibling
-// holds photons (then none can post) and the wave has left the idle set before
-// a donor claimed it.
-__device__ bool coop_receive(BlockState &bs, const KParams &P, NativeWalker &w, int &state, int &my_slot,
-                             uint64_t &my_gid, const CoopEntry *coop_tab, int wave)
-{
-    const int lane = lane_id();
-    if (lane == 0) {
-        atomicSub(&bs.holders, 1);
-        atomicOr(&bs.idle_mask, 1u << wave);
-    }
-    __builtin_amdgcn_s_setprio(0);
-    for (int spin = 0; spin < (1 << 24); ++spin) {
-        if (lds_load(&bs.ib_state[wave]) == 1) {
-            if (lane == 0) atomicAdd(&bs.holders, 1);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            const int dw = lds_load(&bs.ib_src[wave]);
-            const unsigned int mask = lds_load(&bs.ib_mask[wave]);
-            if (lane < __popc(mask)) {
-                unsigned int m = mask;
-                for (int k = 0; k < lane; ++k) m &= m - 1u;
-                const CoopEntry &E = coop_tab[dw * P.coop_live + (__ffs(m) - 1)];
-                const PhiloxPre pre{E.ck0, E.h0, E.l0};
-                const uint32_t cs = pre.ck0 ^ P.key0, ph = pre.l0 * 0x991A7CDBu;
-                int slot = -1;
-                for (int s = 0; s < SLOTS; ++s) {
-                    const SlotHdr &H = bs.hdr[s];
-                    if (lds_load(&H.state) == 1 && (uint32_t)H.case_id == cs &&
-                        ph - H.p0 < (uint32_t)lds_load(&H.seqcount))
-                        slot = s;
-                }
-                if (slot >= 0) {
-                    my_slot = slot;
-                    my_gid = bs.hdr[slot].gid0 + (ph - bs.hdr[slot].p0);
-                    w.resume(P, pre, my_gid, E.z, E.tau, E.dx, E.dy, E.dz, E.i);
-                    state = LIVE;
-                } else {
-                    atomicAdd(P.ctl + 1, 1ull);   // cannot happen: the chunk waits for the photon
-                }
-            }
-            // the donor's entries are read before it may rewrite its table
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            if (lane == 0) {
-                bs.ib_state[wave] = 0;
-                atomicSub(&bs.out_posts[dw], 1);
-            }
-            return true;
-        }
-        if (lds_load(&bs.holders) == 0) {
-            // nobody can post any more -- unless a donor claimed this wave just
-            // before: then its post is on the way
-            int left = 0;
-            if (lane == 0) left = (atomicAnd(&bs.idle_mask, ~(1u << wave)) >> wave) & 1u;
-            if (bcast(left, 0)) return false;
-            continue;
-        }
-        __builtin_amdgcn_s_sleep(4);
-    }
-    if (lane == 0) atomicAdd(P.ctl + 1, 1ull);   // cannot happen: siblings finish
-    return false;
-}
-
-// Before a wave rewrites its coop table for photons it received: wait until
-// the waves it posted photons to have read them (they are polling their
-// inboxes, so this is short).
-__device__ __forceinline__ void await_own_posts(BlockState &bs, const KParams &P, int wave)
-{
-    for (int spin = 0; spin < (1 << 24); ++spin) {
-        if (lds_load(&bs.out_posts[wave]) == 0) return;
-        __builtin_amdgcn_s_sleep(1);
-    }
-    if (lane_id() == 0) atomicAdd(P.ctl + 1, 1ull);   // cannot happen
-}
 
 // ------------------------------------------------------------ chunk queue
 // The block's current chunk: BlockState::cur = (seq << 32) | (slot << 28) |

@@ -1,0 +1,954 @@
+// walkers.h -- the photon walkers: NativeWalkerT (FP32, Philox; polarised for aspherical grains) and ReplayWalker (FP64, the reference's deviates)
+// (part of walk.hip's translation unit; included from there in this order)
+#pragma once
+
+namespace mcs {
+
+// ----------------------------------------------------------- native walker
+// FP32 walk with one Philox4x32-10 call per step (4 uniforms: HG / Lambertian
+// cosine, azimuth, free path, absorption).  The common path is straight-line
+// code -- exits, absorption and near-vertical rotations sit behind rarely
+// taken branches (the last one wave-uniform) -- and the next step's Philox call
+// is issued in the same basic block as the current step's arithmetic: the two
+// chains are independent, so their latencies overlap.
+// np.interp(v, x, y) on an ascending grid of n points (what the reference's
+// interp1d(cos_theta[::-1], P[::-1]) evaluates, mc3D:842-857): y at the ends
+// and at grid points, else slope (v - x_j) + y_j with x_j <= v < x_j+1.
+template <class T>
+__device__ __forceinline__ int grid_index(const T *x, int n, T v)
+{
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= v) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+template <class T>
+__device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v)
+{
+#pragma clang fp contract(off)
+    if (v <= x[0]) return y[0];
+    if (v >= x[n - 1]) return y[n - 1];
+    if (x[j] == v) return y[j];
+    const T slope = (y[j + 1] - y[j]) / (x[j + 1] - x[j]);
+    T r = slope * (v - x[j]) + y[j];
+    if (r != r) {
+        r = slope * (v - x[j + 1]) + y[j + 1];
+        if (r != r && y[j] == y[j + 1]) r = y[j];
+    }
+    return r;
+}
+
+// The state-independent part of one step of the polarised walk (exact
+// sampler): free path and absorption word from the step's main Philox block,
+// cos theta and the phase matrix there, the first 2 phi candidate and its
+// acceptance uniform.  One lane can build it for any step of any photon.
+struct PolarPacket {
+    float dtau;
+    uint32_t w;
+    float ct, r2, acc;
+    float p11, p12, p22, p33, p43, p44;
+    int seg;
+};
+
+template <bool POLAR>
+struct NativeWalkerT {
+    using Real = float;
+    static constexpr bool COOP = !POLAR;      // has the wave-cooperative tail mode below
+    static constexpr bool STOKES = POLAR;     // carries a Stokes vector (aspherical particles)
+    // the polarised walk is built for 3 waves per SIMD (168 VGPRs), which
+    // caps its blocks at 512 threads
+    static constexpr int MAX_BLOCK = POLAR ? 512 : 1024;
+    // k_walk is built for 5 (96 VGPRs) and 6 (80 VGPRs) waves per SIMD
+    float z, mux, muy, muz, tau_acc;
+    uint32_t i;
+    int band;
+    u32x4 r;                     // uniforms of the next step
+    // per-photon parameters (the photon and case ids live in `pre`; the
+    // bottom reflectance and 1 / ext_cff are read from the tables when needed)
+    float a, b, cc, d, e, tau_tot;
+    uint32_t t_abs5, t_abs_all;  // absorbed as condition 5 iff bits < t_abs5, else 4 iff < t_abs_all
+    bool iso, lamb;
+    int cond;
+    PhiloxPre pre;
+    // aspherical particles (POLAR): Stokes vector normalised to I = 1 and the
+    // band's phase-matrix table (-1: Henyey-Greenstein)
+    float sI, sQ, sU, sV;
+    int pm;
+    bool pm_exact;               // the band's exact sampler applies (see DevBand)
+    float pm_mass;               // integral of S11 over the grid
+
+    // Set up photon `photon` of case `case_id` and take its first step (or
+    // its whole walk in the Lambertian surface mode); true when it is done.
+    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    {
+        const DevCase &C = P.cases[case_id];
+        pre = philox_pre(photon, (uint32_t)case_id, 0u, P.key0, P.key1);
+        r = philox_step(pre, 1u, P.key0, P.key1);
+        const int bnd = select_band(P, C, gid, r.x);
+        band = bnd;
+        load_params(P, C, bnd);
+        z = 0.f; mux = C.mux0_f; muy = 0.f; muz = C.muz0_f; tau_acc = 0.f;
+        i = 0;
+        cond = 0;
+        if constexpr (POLAR) {
+            const DevBand &PB = P.bands[C.band_begin + bnd];
+            pm = C.polar ? PB.pm : -1;
+            pm_exact = PB.pm_exact != 0;   // per-photon constants stay in registers:
+            pm_mass = PB.fM;               // a step reads only the tables
+            sI = C.s0f[0]; sQ = C.s0f[1]; sU = C.s0f[2]; sV = C.s0f[3];
+        }
+        if (C.surf) return surface_walk(P, C);
+        // first step: along the incident direction (mc3D:1344-1347)
+        ++i;
+        const u32x4 cur = r;
+        r = philox_step(pre, 2u, P.key0, P.key1);
+        const float dtau = free_path(cur);
+        z = dtau * muz;
+        tau_acc = dtau;
+        bool done = false;
+        if (z < -tau_tot) {                          // unscattered through the layer
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
+            z = -tau_tot;
+            if (lamb && reflects(P)) {
+                done = reemit(P);
+            } else {
+                cond = 3;
+                done = true;
+            }
+        } else if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            done = true;
+        }
+        if (!done && i >= P.max_steps) {             // capped, as after any step
+            cond = 0;
+            done = true;
+        }
+        return done;
+    }
+
+    // the photon's wavelength bin: given per photon, or drawn from the case's
+    // band distribution with the first step's polar word x (the first step
+    // goes along the incident direction and draws no angle): the high bits of
+    // x n_bands pick an alias-table column, the low bits accept it or take its
+    // alias -- one table read instead of a search
+    __device__ static int select_band(const KParams &P, const DevCase &C, uint64_t gid, uint32_t x)
+    {
+        int bnd = 0;
+        if (P.photon_band) {
+            bnd = P.photon_band[gid];
+        } else if (C.n_bands > 1) {
+            const uint64_t m = (uint64_t)x * (uint32_t)C.n_bands;
+            const int j = (int)(m >> 32);
+            const DevBand &B = P.bands[C.band_begin + j];
+            bnd = (uint32_t)m < B.alias_t ? j : B.alias_k;
+        }
+        return bnd;
+    }
+
+    // Continue a photon handed over by another wave of the block: its Philox
+    // words and walk state; band and constants are derived again.
+    __device__ void resume(const KParams &P, const PhiloxPre &p, uint64_t gid, float z_, float tau_,
+                           float dx, float dy, float dz, uint32_t i_)
+    {
+        pre = p;
+        const uint32_t cs = case_id(P);
+        const DevCase &C = P.cases[cs];
+        band = select_band(P, C, gid, philox_step(pre, 1u, P.key0, P.key1).x);
+        load_params(P, C, band);
+        z = z_; tau_acc = tau_; mux = dx; muy = dy; muz = dz; i = i_;
+        cond = 0;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
+    }
+
+    // per-photon constants of (case, band)
+    __device__ __forceinline__ void load_params(const KParams &P, const DevCase &C, int bnd)
+    {
+        const DevBand &B = P.bands[C.band_begin + bnd];
+        a = B.hg_a; b = B.hg_b; cc = B.hg_c; d = B.hg_d; e = B.hg_e;
+        iso = B.iso != 0;
+        if (C.surf) {
+            // Lambertian surface: absorbed with probability 1 - R, split by species
+            const double q = 1.0 - C.r_lamb;
+            t_abs5 = prob_bits_dev(B.p_ext_imp * q);
+            t_abs_all = prob_bits_dev(q);
+        } else {
+            t_abs5 = B.t_abs5;
+            t_abs_all = B.t_abs_all;
+        }
+        tau_tot = C.tau_tot_f;
+        lamb = C.lamb != 0;
+    }
+
+    __device__ __forceinline__ static float free_path(const u32x4 &q)
+    {
+        return -LN2_F * __builtin_amdgcn_logf(u01f(q.z));
+    }
+
+    // Cosine-weighted upward direction, cos = sqrt(u): the pdf 2 sin cos of
+    // the rejection loop in mc3D:1242-1250 (Lambertian re-emission).
+    __device__ __forceinline__ void lambertian_direction(const u32x4 &q)
+    {
+        const float cost = __builtin_amdgcn_sqrtf(u01f(q.x));
+        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        const float rev = u01f(q.y);
+        mux = sint * __builtin_amdgcn_cosf(rev);
+        muy = sint * __builtin_amdgcn_sinf(rev);
+        muz = cost;
+    }
+
+    // After a Lambertian bottom reflection at step i (z = -tau_tot): the next
+    // step re-emits cosine-weighted upward and moves (mc3D:1420-1459); true
+    // when the photon is done.
+    __device__ bool reemit(const KParams &P)
+    {
+        if (i >= P.max_steps) {
+            cond = 0;
+            return true;
+        }
+        ++i;
+        const u32x4 cur = r;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
+        lambertian_direction(cur);
+        if constexpr (POLAR) {
+            // the step after a bottom reflection still draws its angles from
+            // the phase matrix (the azimuth is used), then resets the Stokes
+            // vector (mc3D:1238-1285)
+            if (pm >= 0) polar_lambertian(P, false);
+            sI = 1.f; sQ = 0.f; sU = 0.f; sV = 0.f;
+        }
+        const float dtau = free_path(cur);
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        if (z > 0.f) {
+            tau_acc -= fminf(z / muz, dtau);
+            cond = 1;
+            return true;
+        }
+        if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            return true;
+        }
+        return false;
+    }
+
+    // Lambertian surface (mc3D:1228-1238, 1385-1387): first event at z = 0,
+    // absorbed with probability 1 - R, else re-emitted cosine-weighted upward
+    // until the photon leaves.
+    __device__ bool surface_walk(const KParams &P, const DevCase &C)
+    {
+        (void)C;
+        ++i;
+        u32x4 cur = r;
+        if (cur.w < t_abs_all) {
+            cond = (cur.w < t_abs5) ? 5 : 4;
+            return true;
+        }
+        for (;;) {
+            if (i >= P.max_steps) {
+                cond = 0;
+                return true;
+            }
+            ++i;
+            cur = philox_step(pre, i, P.key0, P.key1);
+            lambertian_direction(cur);
+            if constexpr (POLAR) {
+                // Lambertian surface of aspherical grains: azimuth from the
+                // phase matrix, Stokes vector scattered at the Lambertian
+                // angle (mc3D:1238-1250, 1287-1342)
+                if (pm >= 0) polar_lambertian(P, true);
+            }
+            const float dtau = free_path(cur);
+            z = fmaf(dtau, muz, z);
+            tau_acc += dtau;
+            if (z > 0.f) {
+                tau_acc -= fminf(z / muz, dtau);
+                cond = 1;
+                return true;
+            }
+            if (cur.w < t_abs_all) {
+                cond = (cur.w < t_abs5) ? 5 : 4;
+                return true;
+            }
+        }
+    }
+
+    // one step (i >= 1: after the first); true when the photon has left or
+    // been absorbed
+    __device__ __forceinline__ bool step(const KParams &P)
+    {
+        if constexpr (POLAR) {
+            if (pm >= 0 && pm_exact) {
+                const u32x4 cur = r;
+                r = philox_step(pre, i + 2u, P.key0, P.key1);
+                const u32x4 pb = philox4x32_10(u32x4{photon_id(), case_id(P), i + 1u, 2u}, P.key0, P.key1);
+                PolarPacket k;
+                make_packet(cur, pb, pm_table(P), P.pm_n, pm_mass, k);
+                return polar_step(P, k);
+            }
+        }
+        ++i;
+        const u32x4 cur = r;
+        r = philox_step(pre, i + 1u, P.key0, P.key1);
+        const float dtau = free_path(cur);
+        // polar cosine: HG inverse CDF (mc3D:790-800) with the uniform's
+        // scale and offset folded into d and e; isotropic for g = 0
+        const float xf = (float)cur.x;
+        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
+        float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+        // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
+        float rev = u01f(cur.y);
+        float rev2 = 0.f;
+        int seg = -1;
+        if constexpr (POLAR) {
+            if (pm >= 0) {   // aspherical: phase-matrix sampler
+                polar_angles(P, cost, rev2, seg);
+                rev = 0.5f * rev2;
+            }
+        }
+        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));   // |cost| <= 1
+        const float sinp = __builtin_amdgcn_sinf(rev);
+        const float cosp = __builtin_amdgcn_cosf(rev);
+        // rotation (mc3D:1255-1281); near-vertical incoming directions take the
+        // axis form, behind a wave-uniform branch so that the common path
+        // issues none of its instructions
+        const float s2 = fmaf(-muz, muz, 1.f);
+        const float st = sint * __builtin_amdgcn_rsqf(s2);
+        const float mz0 = muz, mx0 = mux, my0 = muy;
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        muz = fmaf(-s2 * st, cosp, muz * cost);
+        mux = gx; muy = gy;
+        if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
+            if (s2 < 1e-7f) {
+                const float sg = mz0 > 0.f ? 1.f : -1.f;
+                mux = sint * cosp;
+                muy = sg * sint * sinp;
+                muz = sg * cost;
+            }
+        }
+        if constexpr (POLAR) {
+            if (pm >= 0) stokes_update(P, cost, rev2, mx0, my0, mz0, seg);
+        }
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all)) return false;
+        // ---- rare: exits and absorption.  The depth passes an empty asm so
+        // that the tests below are issued here, not hoisted into every step.
+        float ze = z;
+        asm volatile("" : "+v"(ze));
+        if (ze > 0.f) {                              // mc3D:1390-1397
+            tau_acc -= fminf(z / muz, dtau);
+            cond = 1;
+            return true;
+        }
+        if (ze < -tau_tot) {                         // mc3D:1399-1459
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
+            z = -tau_tot;
+            if (lamb && reflects(P)) return reemit(P);
+            cond = 2;
+            return true;
+        }
+        // absorbed: one uniform decides species and absorption jointly, with
+        // P(cond 5) = P_imp (1 - omega_imp), P(cond 4) = (1 - P_imp)(1 - omega_ice)
+        // (mc3D:1374-1383, 1461-1466)
+        cond = (cur.w < t_abs5) ? 5 : 4;
+        return true;
+    }
+
+    // ids from the Philox words: ck0 = case ^ k0, l0 = lo(M0 * photon) with
+    // M0 odd, so photon = l0 * M0^-1 (mod 2^32), M0^-1 = 0x991A7CDB
+    __device__ __forceinline__ uint32_t case_id(const KParams &P) const { return pre.ck0 ^ P.key0; }
+    __device__ __forceinline__ uint32_t photon_id() const { return pre.l0 * 0x991A7CDBu; }
+
+    // Lambertian bottom: reflected with probability R, from Philox block
+    // (photon, case, step, 1)
+    __device__ __forceinline__ bool reflects(const KParams &P) const
+    {
+        const uint32_t cs = case_id(P);
+        const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 1u}, P.key0, P.key1);
+        return (unsigned long long)q.y < P.cases[cs].t_refl;
+    }
+
+    // ---- aspherical particles (POLAR only)
+    __device__ __forceinline__ const float *pm_table(const KParams &P) const
+    {
+        extern __shared__ float pm_lds_tab[];
+        const size_t off = (size_t)pm * (size_t)P.pm_stride;
+        return (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + off;
+    }
+
+    // Scattering angles of mc3D:905-1004: rejection sampling of
+    // I S11(cos) + S12(cos) (Q cos 2phi + U sin 2phi) over cos theta in two
+    // areas split at the cutoff (each bounded by |I| max|P11| + max|P12|
+    // sqrt(Q^2 + U^2), the reference's A + B C) and 2 phi uniform in
+    // [0, 4 pi).  Attempt k of step i draws Philox block (photon, case, i,
+    // 2 + k): area, cos theta, 2 phi, acceptance.  Returns cos theta and
+    // 2 phi in revolutions.
+    __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg) const
+    {
+        const uint32_t cs = case_id(P);
+        const float *T = pm_table(P);
+        const int n = P.pm_n;
+        seg = -1;
+        if (pm_exact) {
+            // The same law without the two-area rejection loop (whose
+            // acceptance is ~10 % on a forward-peaked P11): cos theta from
+            // the marginal S11 by inverting its piecewise-quadratic integral
+            // (make_packet), then 2 phi from 1 + (S12 / S11) (Q cos 2phi +
+            // U sin 2phi) by rejection against 1 + |S12 / S11| sqrt(Q^2 +
+            // U^2) (polar_phi: acceptance >= 1/2, no table reads).  Philox
+            // block (photon, case, i, 2 + k), k = 0, 1, ...
+            const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 2u}, P.key0, P.key1);
+            PolarPacket k;
+            make_packet(q, q, T, n, pm_mass, k);   // its free path and absorption word go unused
+            float r2 = k.r2, sinp, cosp, c2r, s2r;
+            polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
+            cost = k.ct;
+            rev2 = r2;
+            seg = k.seg;
+            return;
+        }
+        const DevCase &C = P.cases[cs];
+        const DevBand &B = P.bands[C.band_begin + band];
+        const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        const float ai = fabsf(sI);
+        const float mv1 = fmaf(ai, B.fA1, B.fB1 * cq), mv2 = fmaf(ai, B.fA2, B.fB2 * cq);
+        const float w1 = C.fcmax - C.fccut, w2 = C.fccut - C.fcmin;
+        const float a1 = w1 * mv1, a2 = w2 * mv2;
+        const float pct = a1 / (a1 + a2);
+        const uint32_t ph = photon_id();
+        for (uint32_t k = 0;; ++k) {
+            const u32x4 q = philox4x32_10(u32x4{ph, cs, i, 2u + k}, P.key0, P.key1);
+            const bool in1 = u01f(q.x) < pct;
+            const float u = u01f(q.y);
+            const float ct = in1 ? fmaf(u, w1, C.fccut) : fmaf(-u, w2, C.fccut);
+            const float r2 = 2.f * u01f(q.z);
+            const float r3 = 1.000001f * u01f(q.w) * (in1 ? mv1 : mv2);
+            const int j = grid_index(T, n, ct);
+            const float s11 = interp_at(T, T + n, n, j, ct);
+            const float s12 = interp_at(T, T + 2 * n, n, j, ct);
+            const float val = fmaf(sI, s11, s12 * fmaf(sQ, __builtin_amdgcn_cosf(r2), sU * __builtin_amdgcn_sinf(r2)));
+            if (r3 <= val) {
+                cost = ct;
+                rev2 = r2;
+                return;
+            }
+            if (k + 1 >= POLAR_MAX_TRIES) {
+                atomicAdd(P.ctl + 2, 1ull);
+                cost = ct;
+                rev2 = r2;
+                return;
+            }
+        }
+    }
+
+    // Stokes vector after scattering by (theta, phi) from direction z-cosine
+    // mz0 to muz_n (mc3D:1287-1342): rotate into the scattering plane by
+    // phi_sca = arccos(cos phi) (phi folded into [0, pi]: the double angle's
+    // sine changes sign for phi > pi), apply the phase matrix at cos theta,
+    // rotate back by -i_2 with cos i_2 = (muz_n cos - mz0) / (sign den)
+    // (sign -1 unless phi_sca = pi, as in the reference; cos i_2 = 0 where
+    // den = 0), normalise to I = 1.  The doubled angles of the return
+    // rotation come from cos i_2 directly: cos 2i = 2c^2 - 1, sin 2i = 2c s.
+    __device__ void stokes_update(const KParams &P, float cost, float rev2, float mx0, float my0, float mz0, int seg)
+    {
+        const float *T = pm_table(P);
+        const int n = P.pm_n;
+        const int j = seg >= 0 ? seg : grid_index(T, n, cost);
+        // linear in the segment (slopes precomputed), clamped to the grid
+        const float dx = fminf(fmaxf(cost, T[0]), T[n - 1]) - T[j];
+        const float p11 = fmaf(T[8 * n + j], dx, T[n + j]);
+        const float p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
+        const float p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
+        const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
+        const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
+        const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
+        stokes_apply(cost, rev2, __builtin_amdgcn_cosf(rev2), __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12,
+                     p22, p33, p43, p44);
+    }
+
+    // (mx0, my0, mz0): the direction before the scattering; the photon's
+    // direction members hold the one after it
+    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions)
+    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float c2r, float s2r, float mx0, float my0,
+                                                 float mz0, float p11, float p12, float p22, float p33, float p43,
+                                                 float p44)
+    {
+#pragma clang fp contract(off)
+        const float c2 = c2r;
+        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * s2r;
+        const float Qs = fmaf(sQ, c2, sU * s2);
+        const float Us = fmaf(-sQ, s2, sU * c2);
+        const float Ic = fmaf(sI, p11, Qs * p12);
+        const float Qc = fmaf(sI, p12, Qs * p22);
+        const float Uc = fmaf(Us, p33, -sV * p43);
+        const float Vc = fmaf(Us, p43, sV * p44);
+        const float muz_n = muz;
+        const float den = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f) *
+                                                 fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
+        // i_2 in [0, pi]: cos i_2 as the reference forms it; sin i_2 >= 0 from
+        // its exact identity (den sin i_2)^2 = (e_z . (u_0 x u_n))^2 -- the
+        // Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2 - cos^2 --
+        // instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1 (scattering
+        // plane close to the meridian plane) would turn an FP32 rounding of
+        // cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 = 0.
+        float ci = 0.f, si = 1.f;
+        if (den != 0.f) {
+            const float rd = __builtin_amdgcn_rcpf(den);
+            const float sg = (rev2 == 1.f) ? 1.f : -1.f;
+            ci = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
+            si = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
+        }
+        const float cr = fmaf(-2.f * si, si, 1.f);     // cos(-2 i_2) = 1 - 2 sin^2 i_2
+        const float sr = -2.f * ci * si;               // sin(-2 i_2)
+        const float inv = __builtin_amdgcn_rcpf(Ic);
+        const float Qm = fmaf(Qc, cr, Uc * sr);
+        const float Um = fmaf(-Qc, sr, Uc * cr);
+        sI = 1.f;
+        sQ = Qm * inv;
+        sU = Um * inv;
+        sV = Vc * inv;
+    }
+
+    // Packet of one step (see PolarPacket) from its main Philox block and its
+    // block (photon, case, step, 2), for a band with the exact sampler: cos
+    // theta by inverting the integral of the linear S11 (guided search), the
+    // six elements there.
+    __device__ static void make_packet(const u32x4 &mainb, const u32x4 &pb, const float *T, int n, float mass,
+                                       PolarPacket &o)
+    {
+        // no implicit contraction in the packet or the step that uses it: a
+        // packet built in another lane and read from LDS must give the same
+        // roundings as one built in place (explicit fmaf only)
+#pragma clang fp contract(off)
+        o.dtau = free_path(mainb);
+        o.w = mainb.w;
+        const float *cum = T + 7 * n;
+        const float u = u01f(pb.x);
+        const float t = u * mass;
+        const int *guide = reinterpret_cast<const int *>(T + 14 * n);
+        const int gk = min((int)(u * (float)PM_GUIDE), PM_GUIDE - 1);
+        int lo = guide[gk], hi = guide[gk + 1] + 1;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (cum[mid] <= t) lo = mid; else hi = mid;
+        }
+        const int j = lo;
+        const float x0 = T[j], h = T[j + 1] - T[j];
+        const float y0 = T[n + j];
+        const float slope = T[8 * n + j];
+        const float dm = fmaxf(t - cum[j], 0.f);
+        const float s = 2.f * dm *
+                        __builtin_amdgcn_rcpf(y0 + __builtin_amdgcn_sqrtf(fmaxf(fmaf(2.f * slope, dm, y0 * y0), 0.f)));
+        const float ct = fminf(x0 + fminf(fmaxf(s, 0.f), h), T[n - 1]);
+        const float dx = ct - x0;
+        o.ct = ct;
+        o.seg = j;
+        o.p11 = fmaf(slope, dx, y0);
+        o.p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
+        o.p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
+        o.p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
+        o.p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
+        o.p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
+        o.r2 = 2.f * u01f(pb.y);
+        o.acc = u01f(pb.z);
+    }
+
+    // 2 phi given cos theta: the first candidate r2 (revolutions) and its
+    // acceptance uniform come with the packet, retries draw blocks (photon,
+    // case, i, 2 + k); target I S11 + S12 (Q cos 2phi + U sin 2phi), bound
+    // S11 + |S12| sqrt(Q^2 + U^2).  Returns the accepted r2 with the sine and
+    // cosine of the azimuth (r2 / 2) and of 2 phi (one transcendental pair).
+    __device__ __forceinline__ void polar_phi(const KParams &P, float s11, float s12, float acc, float &r2, float &sinp,
+                                              float &cosp, float &c2r, float &s2r) const
+    {
+#pragma clang fp contract(off)
+        const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        for (uint32_t kk = 1;; ++kk) {
+            sinp = __builtin_amdgcn_sinf(0.5f * r2);
+            cosp = __builtin_amdgcn_cosf(0.5f * r2);
+            c2r = fmaf(cosp, cosp, -sinp * sinp);
+            s2r = 2.f * sinp * cosp;
+            const float val = fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r));
+            if (acc * bound <= val) break;
+            if (kk >= POLAR_MAX_TRIES) {
+                atomicAdd(P.ctl + 2, 1ull);
+                break;
+            }
+            const u32x4 q = philox4x32_10(u32x4{photon_id(), case_id(P), i, 2u + kk}, P.key0, P.key1);
+            r2 = 2.f * u01f(q.x);
+            acc = u01f(q.y);
+        }
+    }
+
+    // One step of an exact-sampler photon given its packet: 2 phi by
+    // rejection against the current Stokes vector (retries draw blocks
+    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
+    // one-lane walk and the tail mode below both step through here, so a
+    // photon's record does not depend on which of them walked it.
+    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        float r2 = k.r2, sinp, cosp, c2r, s2r;
+        polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
+        const float cost = k.ct;
+        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
+        const float s2 = fmaf(-muz, muz, 1.f);
+        const float st = sint * __builtin_amdgcn_rsqf(s2);
+        const float mz0 = muz, mx0 = mux, my0 = muy;
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        muz = fmaf(-s2 * st, cosp, muz * cost);
+        mux = gx; muy = gy;
+        if (s2 < 1e-7f) {
+            const float sg = mz0 > 0.f ? 1.f : -1.f;
+            mux = sint * cosp;
+            muy = sg * sint * sinp;
+            muz = sg * cost;
+        }
+        stokes_apply(cost, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        const float dtau = k.dtau;
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        if (!(z > 0.f || z < -tau_tot || k.w < t_abs_all)) return false;
+        if (z > 0.f) {                               // mc3D:1390-1397
+            tau_acc -= fminf(z / muz, dtau);
+            cond = 1;
+            return true;
+        }
+        if (z < -tau_tot) {                          // mc3D:1399-1459
+            tau_acc -= fminf((z + tau_tot) / muz, dtau);
+            z = -tau_tot;
+            if (lamb && reflects(P)) {
+                r = philox_step(pre, i + 1u, P.key0, P.key1);   // the re-emission step's block
+                return reemit(P);
+            }
+            cond = 2;
+            return true;
+        }
+        cond = (k.w < t_abs5) ? 5 : 4;
+        return true;
+    }
+
+    // Lambertian re-emission of an aspherical-grain photon (cos theta already
+    // set by lambertian_direction): the azimuth is the phase-matrix
+    // sampler's (the reference keeps phi_rand there); with `update` the
+    // Stokes vector scatters at the Lambertian angle from mz0 = 1.
+    __device__ void polar_lambertian(const KParams &P, bool update)
+    {
+        float ct, rev2;
+        int seg;
+        polar_angles(P, ct, rev2, seg);
+        const float cost = muz;
+        const float sint = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f));
+        const float rv = 0.5f * rev2;
+        mux = sint * __builtin_amdgcn_cosf(rv);
+        muy = sint * __builtin_amdgcn_sinf(rv);
+        if (update) stokes_update(P, cost, rev2, 0.f, 0.f, 1.f, -1);
+    }
+
+    __device__ void result(const KParams &P, int case_id, float &theta, float &phi, float &path, bool &axis) const
+    {
+        theta = acosf(fminf(fmaxf(muz, -1.f), 1.f));
+        path = tau_acc * P.bands[P.cases[case_id].band_begin + band].inv_ext;
+        axis = false;
+        if (i == 1) {
+            phi = 0.f;
+        } else {
+            float ph = atan2f(muy, mux);
+            if (ph < 0.f) ph += TWO_PI_F;
+            phi = ph;
+            axis = (mux == 0.f) || (muy == 0.f);
+        }
+    }
+};
+
+using NativeWalker = NativeWalkerT<false>;
+using PolarWalker = NativeWalkerT<true>;
+
+// ----------------------------------------------------------- replay walker
+// FP64, the reference's operation order; contraction off so that every
+// product and sum rounds as numpy's float64 scalars do.
+struct ReplayWalker {
+    using Real = double;
+    static constexpr bool COOP = false;
+    static constexpr bool STOKES = true;
+    static constexpr int MAX_BLOCK = 256;
+    static constexpr int WAVES_PER_EU = 2;
+    double z, mux, muy, muz, muz2, path;
+    double u_ssa, u_ext, dtau0;
+    double g, ssa_ice, ssa_imp, ext_cff, p_ext_imp, tau_tot, R;
+    int64_t pos, end;
+    uint32_t i;
+    int band;
+    bool bounce, lamb, err, surf;
+    int cond;
+    // aspherical particles: Stokes vector and the band's phase-matrix table
+    double sI, sQ, sU, sV;
+    int pm;
+    int case_id;
+
+    __device__ double draw(const KParams &P)
+    {
+        if (pos >= end) {
+            err = true;
+            return 0.5;
+        }
+        return P.rdev[pos++];
+    }
+
+    // set up the photon; its steps all run in step() (returns false)
+    __device__ bool start(const KParams &P, int case_id, uint32_t photon, uint64_t gid)
+    {
+        (void)photon;
+        const DevCase &C = P.cases[case_id];
+        band = P.photon_band ? P.photon_band[gid] : 0;
+        const DevBand &B = P.bands[C.band_begin + band];
+        g = B.g; ssa_ice = B.ssa_ice; ssa_imp = B.ssa_imp; ext_cff = B.ext_cff;
+        p_ext_imp = B.p_ext_imp;
+        tau_tot = C.tau_tot; R = C.r_lamb; lamb = C.lamb != 0;
+        surf = C.surf != 0;
+        pos = P.roff[gid];
+        end = P.roff[gid + 1];
+        err = false;
+        this->case_id = case_id;
+        pm = C.polar ? B.pm : -1;
+        sI = C.s0[0]; sQ = C.s0[1]; sU = C.s0[2]; sV = C.s0[3];
+        dtau0 = -log(draw(P));
+        u_ssa = draw(P);
+        u_ext = draw(P);
+        z = 0.0; mux = C.mux0; muy = 0.0; muz = C.muz0;
+        muz2 = muz * muz;
+        path = 0.0;
+        i = 0;
+        bounce = false;
+        cond = 0;
+        return false;
+    }
+
+    __device__ bool step(const KParams &P)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        double dtau, u_hg = 0.0, u_phi = 0.0, p_cos = 0.0, two_phi = 0.0;
+        if (i >= 2) {
+            if (pm >= 0) {
+                polar_draws(P, p_cos, two_phi);
+            } else {
+                u_hg = draw(P);
+                u_phi = draw(P);
+            }
+            dtau = -log(draw(P));
+            u_ssa = draw(P);
+            u_ext = draw(P);
+        } else {
+            dtau = surf ? 0.0 : dtau0;   // mc3D:1228-1229
+        }
+        double mxn = mux, myn = muy, mzn = muz;
+        if (i > 1) {
+            double cost, sint;
+            if (bounce || surf) {
+                mux = 0.0; muy = 0.0; muz = 1.0;
+                for (;;) {
+                    const double th = 0.0 + (PI_D / 2) * draw(P);
+                    const double r1 = draw(P);
+                    if (err) { cost = 1.0; sint = 0.0; break; }
+                    if (r1 < 2.0 * sin(th) * cos(th)) {
+                        cost = cos(th);
+                        sint = sqrt(1.0 - cost * cost);
+                        break;
+                    }
+                }
+            } else if (pm >= 0) {
+                cost = p_cos;
+                sint = sqrt(1.0 - cost * cost);
+            } else if (g == 0.0) {
+                cost = 1.0 - 2.0 * u_hg;
+                sint = sqrt(1.0 - cost * cost);
+            } else {
+                const double t = (1.0 - g * g) / ((1.0 - g) + (2.0 * g) * u_hg);
+                cost = (1.0 / (2.0 * g)) * ((1.0 + g * g) - t * t);
+                sint = sqrt(1.0 - cost * cost);
+            }
+            const double ph = (pm >= 0) ? two_phi / 2.0 : u_phi * (2.0 * PI_D);
+            const double cosp = cos(ph), sinp = sin(ph);
+            if (muz == 1.0) {
+                mxn = sint * cosp; myn = sint * sinp; mzn = cost;
+            } else if (muz == -1.0) {
+                mxn = sint * cosp; myn = -sint * sinp; mzn = -cost;
+            } else {
+                const double sq = sqrt(1.0 - muz2);
+                mxn = (sint * (mux * muz * cosp - muy * sinp)) / sq + mux * cost;
+                myn = (sint * (muy * muz * cosp + mux * sinp)) / sq + muy * cost;
+                mzn = -sq * sint * cosp + muz * cost;
+            }
+            if (bounce) {                        // mc3D:1283-1285
+                sI = 1.0; sQ = 0.0; sU = 0.0; sV = 0.0;
+            } else if (pm >= 0) {                // mc3D:1287-1342
+                stokes_update(P, cost, cosp, mzn, muz);
+            }
+            bounce = false;
+        }
+        const double z_prev = z;
+        const double z_i = z_prev + dtau * mzn;
+        z = z_i;
+        if (i > 1) {
+            mux = mxn; muy = myn; muz = mzn;
+            muz2 = muz * muz;
+        }
+        path += dtau / ext_cff;
+        const bool ice = u_ext > p_ext_imp;
+        const double ssa_event = surf ? R : (ice ? ssa_ice : ssa_imp);   // mc3D:1374-1387
+        if (z_i > 0.0) {
+            path += -((z_i * dtau) / ((z_i - z_prev) * ext_cff));
+            cond = 1;
+            return true;
+        }
+        if (z_i < -tau_tot) {
+            path += -(((z_i + tau_tot) * dtau) / ((z_i - z_prev) * ext_cff));
+            const double dcorr = -(((z_i + tau_tot) / (z_prev - z_i)) * dtau);
+            z = z_i - (mzn * dcorr);
+            const int exit_cond = (i == 1) ? 3 : 2;
+            if (lamb) {
+                if (draw(P) <= R) {
+                    bounce = true;
+                    return err;
+                }
+            }
+            cond = exit_cond;
+            return true;
+        }
+        if (u_ssa >= ssa_event) {
+            cond = ice ? 4 : 5;
+            return true;
+        }
+        return err;
+    }
+
+    // populate_pdfs for an aspherical particle (mc3D:898-1004): area, cos
+    // theta and phi, then the rejection loop (r3; after a rejection: area,
+    // cos theta, 2 phi, r3) against I S11 + S12 (Q cos 2phi + U sin 2phi).
+    // The reference's operation order and float64 functions.
+    __device__ void polar_draws(const KParams &P, double &ct, double &two_phi)
+    {
+#pragma clang fp contract(off)
+        const DevCase &C = P.cases[case_id];
+        const DevBand &B = P.bands[C.band_begin + band];
+        const int n = P.pm_n;
+        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
+        const double I = sI, Q = sQ, U = sU;
+        const double two_beta = (Q == 0.0) ? atan(U / 1e-15) : atan(U / Q);
+        double A = fabs(I) * B.pA1;
+        double Bm = B.pB1;
+        const double Cc = fabs(Q * cos(two_beta) + U * sin(two_beta));
+        const double mv1 = A + Bm * Cc;
+        A = fabs(I) * B.pA2;
+        Bm = B.pB2;
+        const double mv2 = A + Bm * Cc;
+        const double a1 = (C.cmax - C.ccut) * mv1;
+        const double a2 = (C.ccut - C.cmin) * mv2;
+        const double pct = a1 / (a1 + a2);
+        double ar = draw(P);
+        ct = draw(P);
+        two_phi = 2.0 * (draw(P) * (2.0 * PI_D));
+        int area;
+        if (ar < pct) {
+            area = 1;
+            ct = ct * (C.cmax - C.ccut) + C.ccut;
+        } else {
+            area = 2;
+            ct = -ct * (C.ccut - C.cmin) + C.ccut;
+        }
+        double r3 = 1.0, val = 0.0;
+        for (int k = 0; r3 > val; ++k) {
+            if (k > 0) {
+                ar = draw(P);
+                if (ar < pct) {
+                    area = 1;
+                    ct = draw(P) * (C.cmax - C.ccut) + C.ccut;
+                } else {
+                    area = 2;
+                    ct = -draw(P) * (C.ccut - C.cmin) + C.ccut;
+                }
+                two_phi = draw(P) * (4.0 * PI_D);
+            }
+            r3 = 1.000001 * draw(P);
+            r3 = (area == 1) ? r3 * mv1 : r3 * mv2;
+            const int j = grid_index(x, n, ct);
+            const double S11 = interp_at(x, x + n, n, j, ct);
+            const double S12 = interp_at(x, x + 2 * n, n, j, ct);
+            val = I * S11 + S12 * (Q * cos(two_phi) + U * sin(two_phi));
+            if (err) break;
+        }
+    }
+
+    // mc3D:1287-1342 and rotate_stokes_vector (mc3D:1090-1109), the
+    // reference's order and float64 functions.
+    __device__ void stokes_update(const KParams &P, double costheta, double cosphi, double muz_n, double muz_0)
+    {
+#pragma clang fp contract(off)
+        const int n = P.pm_n;
+        const double *x = P.pm + (size_t)pm * 7 * (size_t)n;
+        const double cos2theta = costheta * costheta;
+        const double phi_sca = acos(cosphi);
+        double da = 2.0 * phi_sca;
+        double c = cos(da), s = sin(da);
+        const double Isp = sI, Qsp = sQ * c + sU * s, Usp = -sQ * s + sU * c, Vsp = sV;
+        const int j = grid_index(x, n, costheta);
+        const double P11 = interp_at(x, x + n, n, j, costheta);
+        const double P12 = interp_at(x, x + 2 * n, n, j, costheta);
+        const double P22 = interp_at(x, x + 3 * n, n, j, costheta);
+        const double P33 = interp_at(x, x + 4 * n, n, j, costheta);
+        const double P43 = interp_at(x, x + 5 * n, n, j, costheta);
+        const double P44 = interp_at(x, x + 6 * n, n, j, costheta);
+        const double Ic = Isp * P11 + Qsp * P12;
+        const double Qc = Isp * P12 + Qsp * P22;
+        const double Uc = Usp * P33 - Vsp * P43;
+        const double Vc = Usp * P43 + Vsp * P44;
+        const double den = sqrt((1.0 - cos2theta) * (1.0 - muz_n * muz_n));
+        double ci;
+        if (den == 0.0) {
+            ci = 0.0;
+        } else {
+            const double num = muz_n * costheta - muz_0;
+            const double sg = (phi_sca >= PI_D) ? 1.0 : -1.0;
+            ci = num / (sg * den);
+            if (ci < -1.0) ci = -1.0;
+            else if (ci > 1.0) ci = 1.0;
+        }
+        const double i2 = acos(ci);
+        da = 2.0 * (-i2);
+        c = cos(da);
+        s = sin(da);
+        const double Qm = Qc * c + Uc * s, Um = -Qc * s + Uc * c;
+        sI = Ic / Ic;
+        sQ = Qm / Ic;
+        sU = Um / Ic;
+        sV = Vc / Ic;
+    }
+
+    __device__ void result(const KParams &, int, double &theta, double &phi, double &pth, bool &axis) const
+    {
+#pragma clang fp contract(off)
+        theta = acos(muz);
+        pth = path;
+        axis = false;
+        if (i == 1) phi = 0.0;
+        else if (mux > 0 && muy > 0) phi = atan(muy / mux);
+        else if (mux < 0 && muy > 0) phi = atan(muy / mux) + PI_D;
+        else if (mux < 0 && muy < 0) phi = atan(muy / mux) + PI_D;
+        else if (mux > 0 && muy < 0) phi = atan(muy / mux) + 2.0 * PI_D;
+        else { phi = __builtin_nan(""); axis = true; }
+    }
+};
+
+
+// Lanes are EMPTY (no photon), LIVE or DONE (finished, record pending).
+enum { EMPTY = 0, LIVE = 1, DONE = 2 };
+
+
+}  // namespace mcs
