@@ -229,6 +229,17 @@ def polar_block(w, no_cpu):
                                'sample': '%d photons of the same case walked by oracle/polar.py (the reference\'s '
                                          'per-step numpy structure, one core): %.1f s' % (m, dt)}
         out['gpu_over_cpu'] = round(out['photons_per_s'] / out['cpu_baseline']['value'], 1)
+        # the whole 1e6-photon case walked by the compiled C restatement (FP64)
+        from oracle import cpu_baseline
+        cc = cpu_baseline.run_c_polar(optics.aspherical_dir(d, p['shape'], p['roughness'], p['wvl']), p,
+                                      case.tau_tot, n_photon=n, seed=1, rho_snw=RHO_SNW, rho_ice=RHO_ICE)
+        out['cpu_baseline']['c_restatement'] = {
+            'value': round(cc['photons_per_s'], 1), 'unit': 'photons/s', 'cores': cc['threads'],
+            'sample': '%d photons of the same case, %d host threads each walking its share with '
+                      'oracle/mcoracle.c (phase-matrix rejection draws and Stokes update, FP64, gcc -O2); '
+                      'walk phase %.2f s, %.3g photon-steps/s'
+                      % (cc['photons'], cc['threads'], cc['walk_s'], cc['steps_per_s'])}
+        out['gpu_over_cpu_c'] = round(out['photons_per_s'] / cc['photons_per_s'], 1)
     return out
 
 

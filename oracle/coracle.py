@@ -27,6 +27,21 @@ class CaseC(ctypes.Structure):
                 ('r_lambertian', ctypes.c_double), ('max_steps', ctypes.c_int64)]
 
 
+class PmC(ctypes.Structure):
+    _fields_ = [('n', ctypes.c_int32), ('pad', ctypes.c_int32), ('x', ctypes.c_void_p),
+                ('P', ctypes.c_void_p), ('A1', ctypes.c_double), ('B1', ctypes.c_double),
+                ('A2', ctypes.c_double), ('B2', ctypes.c_double)]
+
+
+class PolarC(ctypes.Structure):
+    _fields_ = [('cmax', ctypes.c_double), ('ccut', ctypes.c_double), ('cmin', ctypes.c_double),
+                ('stokes0', ctypes.c_double * 4), ('pm', ctypes.c_void_p), ('q', ctypes.c_void_p),
+                ('u', ctypes.c_void_p), ('v', ctypes.c_void_p)]
+
+
+ELEMENTS = ('P11', 'P12', 'P22', 'P33', 'P43', 'P44')
+
+
 def lib():
     global _LIB
     if _LIB is None:
@@ -38,7 +53,7 @@ def lib():
         P = ctypes.c_void_p
         L.mco_walk.argtypes = [ctypes.POINTER(CaseC), ctypes.c_int64, P, P, P, P, P,
                                ctypes.POINTER(MTState), P, P, P, ctypes.c_int64, P,
-                               P, P, P, P, P, P]
+                               P, P, P, P, P, P, P]
         L.mco_walk.restype = ctypes.c_int
         L.mco_mt_doubles.argtypes = [ctypes.POINTER(MTState), P, ctypes.c_int64]
         L.mco_mt_doubles.restype = ctypes.c_int
@@ -75,10 +90,13 @@ ERRORS = {-1: 'bad arguments', -2: 'record capacity exceeded', -3: 'replay slice
 
 def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
          mt: MTState | None = None, replay=None, record=False, max_steps=0,
-         allow_unbound_phi=False, lambertian_surface=False):
+         allow_unbound_phi=False, lambertian_surface=False, polar=None):
     """Walk len(par['g']) photons.  ``replay`` = (values, offsets) CSR slices.
-    Returns dict of record arrays (+ 'steps', and 'stream' = (values, offsets)
-    when ``record``)."""
+    ``polar`` = (case, bands, keys, stokes0) walks aspherical grains: ``case``
+    and ``bands`` as oracle.polar.polar_tables returns them, ``keys`` each
+    photon's wavelength (its key in ``bands``).
+    Returns dict of record arrays (+ 'steps', 'Q_n', 'U_n', 'V_n' when
+    ``polar``, and 'stream' = (values, offsets) when ``record``)."""
     theta_0 = (np.pi * theta_0_deg) / 180.
     c = CaseC(float(tau_tot), float(rho_snw), float(np.sin(theta_0)), float(-np.cos(theta_0)),
               int(bool(lambertian_bottom)), int(bool(lambertian_surface)), float(r_lambertian),
@@ -97,6 +115,28 @@ def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
         rep_v = np.ascontiguousarray(replay[0], dtype=np.float64)
         rep_o = np.ascontiguousarray(replay[1], dtype=np.int64)
         assert rep_o.size == n + 1
+    pol = None
+    if polar is not None:
+        pcase, bands, keys, stokes0 = polar
+        x = np.ascontiguousarray(pcase['x'], dtype=np.float64)
+        keep = [x]
+        tabs = {}
+        for key, b in bands.items():
+            P = np.ascontiguousarray(np.stack([b['Y'][e] for e in ELEMENTS]), dtype=np.float64)
+            keep.append(P)
+            tabs[key] = PmC(x.size, 0, x.ctypes.data, P.ctypes.data,
+                            b['A1'], b['B1'], b['A2'], b['B2'])
+        pms = list(tabs.values())
+        arr = (PmC * len(pms))(*pms)
+        slot = {key: j for j, key in enumerate(tabs)}
+        ptrs = (ctypes.c_void_p * max(n, 1))()
+        base = ctypes.addressof(arr)
+        for k, key in enumerate(np.asarray(keys)):
+            ptrs[k] = base + slot[key] * ctypes.sizeof(PmC)
+        qo, uo, vo = np.empty(n), np.empty(n), np.empty(n)
+        pol = PolarC(pcase['cmax'], pcase['ccut'], pcase['cmin'],
+                     (ctypes.c_double * 4)(*[float(v) for v in stokes0]),
+                     ctypes.addressof(ptrs), qo.ctypes.data, uo.ctypes.data, vo.ctypes.data)
     cap = 0
     rec = rec_off = None
     saved = None
@@ -113,7 +153,7 @@ def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
                              None if mt is None else ctypes.byref(mt), _p(rep_v), _p(rep_o),
                              _p(rec), cap, _p(rec_off), cond.ctypes.data, nscat.ctypes.data,
                              theta.ctypes.data, phi.ctypes.data, path.ctypes.data,
-                             ctypes.byref(steps))
+                             ctypes.byref(steps), None if pol is None else ctypes.byref(pol))
         if ret == -2 and record:
             cap = int(rec_off[n]) + 16
             if saved is not None:
@@ -124,6 +164,8 @@ def walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian,
         raise RuntimeError('mco_walk: %s (%d)' % (ERRORS.get(ret, '?'), ret))
     out = dict(condition=cond.astype(np.int64), theta_n=theta, phi_n=phi,
                n_scat=nscat, path_length=path, steps=int(steps.value))
+    if pol is not None:
+        out.update(Q_n=qo, U_n=uo, V_n=vo)
     if record:
         out['stream'] = (rec[:rec_off[n]].copy(), rec_off)
     return out
@@ -146,6 +188,31 @@ def run_rank(seed, n_photon, wvl0, half_width, ssp_fn, imp_cnc, tau_tot, rho_snw
     out['wvn'] = 1. / wvls
     out['snow_depth'] = par['snow_depth']
     return out, par
+
+
+def polar_run_rank(seed, n_photon, wvl0, half_width, rds_snw, directory, stokes0, imp_fn, imp_cnc,
+                   tau_tot, rho_snw, rho_ice, theta_0_deg, lambertian_bottom, r_lambertian,
+                   lambertian_surface=False, record=False, max_steps=0):
+    """Single-rank run() of an aspherical case (HG = False) with
+    ``RandomState(seed)``: wavelengths, SSPs and the phase-matrix tables as in
+    oracle.polar.polar_run, then the initial triples and walks in C continuing
+    the same MT19937 stream.  Returns (records dict incl. wvn/snow_depth and
+    Q_n/U_n/V_n, params, case tables, band tables)."""
+    from oracle import polar as pypolar
+    from oracle import walk as pywalk
+    rs = np.random.RandomState(seed)
+    wvls = pywalk.draw_wavelengths(rs, n_photon, wvl0, half_width)
+    wvls, ssa, ext, g, reff, theta, P = pypolar.aspherical_ssps(directory, wvl0, rds_snw, wvls, False,
+                                                                rho_ice)
+    par = pywalk.photon_params(wvls, dict(ssa=ssa, ext=ext, g=g), imp_fn(wvls), imp_cnc, tau_tot, rho_snw)
+    case, bands = pypolar.polar_tables(theta, P, wvls, wvl0)
+    mt = mt_from_numpy(rs)
+    out = walk(par, tau_tot, rho_snw, theta_0_deg, lambertian_bottom, r_lambertian, mt=mt,
+               record=record, max_steps=max_steps, lambertian_surface=lambertian_surface,
+               polar=(case, bands, wvls, stokes0))
+    out['wvn'] = 1. / wvls
+    out['snow_depth'] = par['snow_depth']
+    return out, par, case, bands
 
 
 def run_ranks(seed, n_ranks, n_per_rank, par_fn, tau_tot, rho_snw, theta_0_deg,

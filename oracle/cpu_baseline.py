@@ -108,3 +108,38 @@ def cpu_model():
     except OSError:
         pass
     return 'unknown'
+
+
+def run_c_polar(directory, p, tau_tot, n_photon=200_000, threads=None, seed=0, rho_snw=300.0,
+                rho_ice=917.0):
+    """An aspherical case (``p`` as monte_carlompi_amd.workloads.SNOW_POLAR,
+    phase-matrix tables under ``directory``) walked by the C restatement
+    (oracle/mcoracle.c with its phase-matrix rejection draws and Stokes update,
+    FP64): ``threads`` ranks of n_photon / threads photons, one host thread per
+    rank.  Only the walk is timed; the tables are read once."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import coracle
+    T = threads or min(16, len(os.sched_getaffinity(0)))
+    per = -(-int(n_photon) // T)
+    zero = lambda w: dict(ssa=np.zeros(w.shape), ext=np.zeros(w.shape))
+    _, par1, case, bands = coracle.polar_run_rank(
+        seed, 1, p['wvl'], 1e-12, p['r_eff'], directory, p['stokes0'], zero, 0.0, tau_tot, rho_snw,
+        rho_ice, p['theta0'], p['lambertian_bottom'], p['r_lambertian'])
+    par = {k: np.full(per, v[0]) for k, v in par1.items()}
+    keys = np.full(per, par1['wvl'][0])
+
+    def one(r):
+        rs = np.random.RandomState(seed + r)
+        walk.draw_wavelengths(rs, per, p['wvl'], 1e-12)
+        mt = coracle.mt_from_numpy(rs)
+        t0 = time.perf_counter()
+        out = coracle.walk(par, tau_tot, rho_snw, p['theta0'], p['lambertian_bottom'], p['r_lambertian'],
+                           mt=mt, polar=(case, bands, keys, p['stokes0']))
+        return time.perf_counter() - t0, int(out['steps'])
+
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        parts = list(ex.map(one, range(T)))
+    walk_s = max(t for t, _ in parts)
+    steps = sum(k for _, k in parts)
+    return dict(threads=T, photons=per * T, steps=steps, walk_s=walk_s,
+                photons_per_s=per * T / walk_s, steps_per_s=steps / walk_s)

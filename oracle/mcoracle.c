@@ -9,6 +9,11 @@
  *   populate_pdfs (HG)      mc3D:911-921, 1010-1023
  *   Henyey_Greenstein2      mc3D:790-800
  *   monte_carlo3D           mc3D:1111-1490
+ * and, for aspherical grains (`pol` given), the phase-matrix rejection
+ * sampler and Stokes update of oracle/polar.py:
+ *   populate_pdfs (aspherical) mc3D:898-1004
+ *   rotate_stokes_vector       mc3D:1090-1109
+ *   Stokes update              mc3D:1283-1342
  * The uniform stream is numpy's legacy RandomState MT19937 (state layout of
  * RandomState.get_state(): 624 words + position), so that a walk started from
  * a numpy state consumes exactly the deviates the reference would draw.
@@ -42,6 +47,24 @@ typedef struct {
     double r_lambertian;       /* its reflectance */
     int64_t max_steps;         /* 0: unlimited (the reference has no cap) */
 } mco_case;
+
+/* Aspherical grains: one phase-matrix table per photon wavelength (the
+ * arrays of interp1d(cos_theta[::-1], P[::-1]), mc3D:842-857) and the
+ * rejection sampler's constants (mc3D:858-883, 926-941). */
+typedef struct {
+    int32_t n;                 /* grid points */
+    int32_t pad;
+    const double *x;           /* ascending cos theta [n] */
+    const double *P;           /* P11, P12, P22, P33, P43, P44 on x [6 n] */
+    double A1, B1, A2, B2;     /* max |P11|, |P12| over theta <= / >= cutoff */
+} mco_pm;
+
+typedef struct {
+    double cmax, ccut, cmin;   /* cos theta range and the cutoff */
+    double stokes0[4];         /* initial Stokes vector */
+    const mco_pm *const *pm;   /* per photon: its wavelength's table */
+    double *q, *u, *v;         /* out: final Q, U, V per photon */
+} mco_polar;
 
 static void mt_regen(mco_mt *s)
 {
@@ -120,6 +143,111 @@ static inline double hg2(double g, double u)
     return (1.0 / (2.0 * g)) * ((1.0 + g * g) - t * t);
 }
 
+/* np.interp(v, x, y) (numpy's compiled_base.c rules): the end values
+ * outside the grid and at its last point, y_j at grid points, else
+ * slope (v - x_j) + y_j with x_j <= v < x_j+1 */
+static double interp_np(double v, const double *x, const double *y, int n)
+{
+    if (v > x[n - 1]) return y[n - 1];
+    if (v < x[0]) return y[0];
+    int lo = 0, hi = n - 1;
+    if (v == x[n - 1]) return y[n - 1];
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (x[mid] <= v) lo = mid; else hi = mid;
+    }
+    if (x[lo] == v) return y[lo];
+    const double slope = (y[lo + 1] - y[lo]) / (x[lo + 1] - x[lo]);
+    double r = slope * (v - x[lo]) + y[lo];
+    if (r != r) {
+        r = slope * (v - x[lo + 1]) + y[lo + 1];
+        if (r != r && y[lo] == y[lo + 1]) r = y[lo];
+    }
+    return r;
+}
+
+/* populate_pdfs for an aspherical grain (mc3D:898-1004): area, cos theta,
+ * phi, then r3 and, after each rejection, area, cos theta, 2 phi, r3. */
+static void polar_draws(src_t *s, const mco_polar *pl, const mco_pm *t, const double st[4],
+                        double *ct_out, double *phi_out)
+{
+    const double two_pie = 2.0 * M_PI, four_pie = 4.0 * M_PI;
+    const double I = st[0], Q = st[1], U = st[2];
+    const double two_beta = (Q == 0.0) ? atan(U / 1e-15) : atan(U / Q);
+    double A = fabs(I) * t->A1;
+    double B = t->B1;
+    const double C = fabs(Q * cos(two_beta) + U * sin(two_beta));
+    const double mv1 = A + B * C;
+    A = fabs(I) * t->A2;
+    B = t->B2;
+    const double mv2 = A + B * C;
+    const double a1 = (pl->cmax - pl->ccut) * mv1;
+    const double a2 = (pl->ccut - pl->cmin) * mv2;
+    const double pct = a1 / (a1 + a2);
+    double ar = draw(s);
+    double ct = draw(s);
+    double two_phi = 2.0 * (draw(s) * two_pie);
+    int area;
+    if (ar < pct) { area = 1; ct = ct * (pl->cmax - pl->ccut) + pl->ccut; }
+    else { area = 2; ct = -ct * (pl->ccut - pl->cmin) + pl->ccut; }
+    double r3 = 1.0, val = 0.0;
+    for (int k = 0; r3 > val; ++k) {
+        if (k > 0) {
+            ar = draw(s);
+            if (ar < pct) { area = 1; ct = draw(s) * (pl->cmax - pl->ccut) + pl->ccut; }
+            else { area = 2; ct = -draw(s) * (pl->ccut - pl->cmin) + pl->ccut; }
+            two_phi = draw(s) * four_pie;
+        }
+        r3 = 1.000001 * draw(s);
+        r3 = (area == 1) ? r3 * mv1 : r3 * mv2;
+        const double S11 = interp_np(ct, t->x, t->P, t->n);
+        const double S12 = interp_np(ct, t->x, t->P + t->n, t->n);
+        val = I * S11 + S12 * (Q * cos(two_phi) + U * sin(two_phi));
+        if (s->err) break;
+    }
+    *ct_out = ct;
+    *phi_out = two_phi / 2.0;
+}
+
+/* mc3D:1287-1342 with rotate_stokes_vector (mc3D:1090-1109). */
+static void stokes_update(const mco_pm *t, double st[4], double costheta, double cosphi, double muz_n,
+                          double muz_0)
+{
+    const int n = t->n;
+    const double cos2theta = costheta * costheta;
+    const double phi_sca = acos(cosphi);
+    double da = 2.0 * phi_sca;
+    double c = cos(da), sn = sin(da);
+    const double Isp = st[0], Qsp = st[1] * c + st[2] * sn, Usp = -st[1] * sn + st[2] * c, Vsp = st[3];
+    const double P11 = interp_np(costheta, t->x, t->P, n), P12 = interp_np(costheta, t->x, t->P + n, n);
+    const double P22 = interp_np(costheta, t->x, t->P + 2 * n, n), P33 = interp_np(costheta, t->x, t->P + 3 * n, n);
+    const double P43 = interp_np(costheta, t->x, t->P + 4 * n, n), P44 = interp_np(costheta, t->x, t->P + 5 * n, n);
+    const double Ic = Isp * P11 + Qsp * P12;
+    const double Qc = Isp * P12 + Qsp * P22;
+    const double Uc = Usp * P33 - Vsp * P43;
+    const double Vc = Usp * P43 + Vsp * P44;
+    const double den = sqrt((1.0 - cos2theta) * (1.0 - muz_n * muz_n));
+    double ci;
+    if (den == 0.0) {
+        ci = 0.0;
+    } else {
+        const double num = muz_n * costheta - muz_0;
+        const double sg = (phi_sca >= M_PI) ? 1.0 : -1.0;
+        ci = num / (sg * den);
+        if (ci < -1.0) ci = -1.0;
+        else if (ci > 1.0) ci = 1.0;
+    }
+    const double i2 = acos(ci);
+    da = 2.0 * (-i2);
+    c = cos(da);
+    sn = sin(da);
+    const double Qm = Qc * c + Uc * sn, Um = -Qc * sn + Uc * c;
+    st[0] = Ic / Ic;
+    st[1] = Qm / Ic;
+    st[2] = Um / Ic;
+    st[3] = Vc / Ic;
+}
+
 int mco_mt_doubles(mco_mt *mt, double *out, int64_t n)
 {
     for (int64_t i = 0; i < n; i++)
@@ -143,7 +271,7 @@ int mco_walk(const mco_case *c, int64_t n,
              mco_mt *mt, const double *replay, const int64_t *replay_off,
              double *rec, int64_t rec_cap, int64_t *rec_off,
              int8_t *cond, int64_t *nscat, double *theta, double *phi, double *path,
-             int64_t *steps_total)
+             int64_t *steps_total, const mco_polar *pol)
 {
     if (!c || n < 0 || !g || !ssa_ice || !ssa_imp || !ext_cff_mss || !p_ext_imp ||
         !cond || !nscat || !theta || !phi || !path)
@@ -198,7 +326,10 @@ int mco_walk(const mco_case *c, int64_t n,
         double muz2 = muz0 * muz0;
         double z = 0.0, path_length = 0.0;
         double dtau = c->lambertian_surface ? 0.0 : -log(u_tau0), u_ssa = u_ssa0, u_ext = u_ext0;
-        double u_hg = 0.0, u_phi = 0.0;
+        double u_hg = 0.0, u_phi = 0.0, p_cos = 0.0, p_phi = 0.0;
+        const mco_pm *tab = pol ? pol->pm[k] : NULL;
+        double st[4] = {1.0, 0.0, 0.0, 0.0};
+        if (pol) memcpy(st, pol->stokes0, sizeof st);
         int bottom_reflection = 0, condition = 0;
         int64_t i = 0;
         while (condition == 0) {
@@ -208,8 +339,12 @@ int mco_walk(const mco_case *c, int64_t n,
                 break;
             }
             if (i >= 2) {
-                u_hg = draw(&s);
-                u_phi = draw(&s);
+                if (tab) {
+                    polar_draws(&s, pol, tab, st, &p_cos, &p_phi);
+                } else {
+                    u_hg = draw(&s);
+                    u_phi = draw(&s);
+                }
                 dtau = -log(draw(&s));
                 u_ssa = draw(&s);
                 u_ext = draw(&s);
@@ -232,11 +367,14 @@ int mco_walk(const mco_case *c, int64_t n,
                         }
                     }
                     if (s.err) { costheta = 1.0; sintheta = 0.0; }
+                } else if (tab) {
+                    costheta = p_cos;
+                    sintheta = sqrt(1.0 - costheta * costheta);
                 } else {
                     costheta = hg2(gk, u_hg);
                     sintheta = sqrt(1.0 - costheta * costheta);
                 }
-                double ph = u_phi * two_pie;
+                double ph = tab ? p_phi : u_phi * two_pie;
                 double cosphi = cos(ph), sinphi = sin(ph);
                 if (muz0 == 1.0) {
                     mux_n = sintheta * cosphi;
@@ -251,6 +389,11 @@ int mco_walk(const mco_case *c, int64_t n,
                     mux_n = (sintheta * (mux0 * muz0 * cosphi - muy0 * sinphi)) / sq + mux0 * costheta;
                     muy_n = (sintheta * (muy0 * muz0 * cosphi + mux0 * sinphi)) / sq + muy0 * costheta;
                     muz_n = -sq * sintheta * cosphi + muz0 * costheta;
+                }
+                if (bottom_reflection) {             /* mc3D:1283-1285 */
+                    st[0] = 1.0; st[1] = 0.0; st[2] = 0.0; st[3] = 0.0;
+                } else if (tab) {                    /* mc3D:1287-1342 */
+                    stokes_update(tab, st, costheta, cosphi, muz_n, muz0);
                 }
                 bottom_reflection = 0;
             }
@@ -306,6 +449,11 @@ int mco_walk(const mco_case *c, int64_t n,
         else if (mux0 > 0 && muy0 < 0) p = atan(muy0 / mux0) + two_pie;
         else { p = NAN; if (ret == 0) ret = -4; }
         phi[k] = p;
+        if (pol) {
+            pol->q[k] = st[1];
+            pol->u[k] = st[2];
+            pol->v[k] = st[3];
+        }
         if (ret == -5) break;
     }
     if (s.rec) rec_off[n] = s.rec_len;

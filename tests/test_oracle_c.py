@@ -57,3 +57,66 @@ def test_c_oracle_replay_consumes_slices_exactly():
     with pytest.raises(RuntimeError, match='replay'):
         coracle.walk(par, case['tau'], META['rho_snw'], case['theta0'], case['lb'],
                      case['R'], replay=short)
+
+
+def _polar_cases():
+    from tests import golden_polar
+    meta, cases = golden_polar.load()
+    return meta, [c for c in cases if not c['HG']]
+
+
+PMETA, PCASES = _polar_cases()
+
+
+@pytest.mark.parametrize('case', PCASES, ids=[c['name'] for c in PCASES])
+def test_c_oracle_polar_matches_reference_records(case, tmp_path):
+    """The aspherical walk in C (phase-matrix rejection draws, Stokes update)
+    follows the reference's records: the same conditions and scattering
+    counts, the same uniform stream draw for draw as the Python oracle, and
+    angles, paths and Stokes vectors to libm-vs-numpy rounding."""
+    from tests import golden_polar
+    from tests.test_oracle_polar import run_case as py_polar_case
+    d = golden_polar.table_dir(str(tmp_path), case)
+    imp_fn = lambda w: dict(ssa=np.full(w.shape, 0.3), ext=np.full(w.shape, 12000.0))
+    out, par, pc, bands = coracle.polar_run_rank(
+        case['seed'], case['n'], case['wvl0'], case['fwhm'], case['rds'], d, case['stokes'], imp_fn,
+        case['imp_cnc'], case['tau'], PMETA['rho_snw'], PMETA['rho_ice'], case['theta0'], case['lb'],
+        case['R'], lambertian_surface=case.get('ls', False), record=True)
+    ref = case['records']
+    np.testing.assert_array_equal(out['condition'], ref['condition'])
+    np.testing.assert_array_equal(out['n_scat'], ref['n_scat'])
+    rtol = 2e-5 if case['lowprec_columns'] else 1e-12
+    for f in ('theta_n', 'phi_n', 'path_length'):
+        np.testing.assert_allclose(out[f], ref[f], rtol=rtol, atol=1e-13, err_msg=f)
+    for f in ('Q_n', 'U_n', 'V_n'):
+        np.testing.assert_allclose(out[f], ref[f], rtol=0, atol=1e-13, err_msg=f)
+    _, _, values, offsets, _, _, _ = py_polar_case(case, str(tmp_path / 'py'))
+    np.testing.assert_array_equal(out['stream'][1], offsets)
+    np.testing.assert_array_equal(out['stream'][0], values)
+    # replaying that stream walks the same photons and ends on its last draw
+    rep = coracle.walk(par, case['tau'], PMETA['rho_snw'], case['theta0'], case['lb'], case['R'],
+                       replay=(values, offsets), lambertian_surface=case.get('ls', False),
+                       polar=(pc, bands, 1. / out['wvn'], case['stokes']))
+    for f in ('condition', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'Q_n', 'U_n', 'V_n'):
+        np.testing.assert_array_equal(rep[f], out[f], err_msg=f)
+
+
+def test_c_oracle_polar_catches_a_dropped_stokes_term(tmp_path):
+    """A planted mistake in the restated tables (P12 zeroed: the phase-matrix
+    coupling of I and Q dropped) changes the Stokes vectors the walk ends
+    with, so the record comparison above pins that term."""
+    from tests import golden_polar
+    case = [c for c in PCASES if c['name'] == 'pm_linear_q'][0]
+    d = golden_polar.table_dir(str(tmp_path), case)
+    imp_fn = lambda w: dict(ssa=np.full(w.shape, 0.3), ext=np.full(w.shape, 12000.0))
+    out, par, pc, bands = coracle.polar_run_rank(
+        case['seed'], case['n'], case['wvl0'], case['fwhm'], case['rds'], d, case['stokes'], imp_fn,
+        case['imp_cnc'], case['tau'], PMETA['rho_snw'], PMETA['rho_ice'], case['theta0'], case['lb'],
+        case['R'], record=True)
+    broken = {k: dict(b, Y=dict(b['Y'], P12=np.zeros_like(b['Y']['P12']))) for k, b in bands.items()}
+    rs = np.random.RandomState(case['seed'])
+    walk.draw_wavelengths(rs, case['n'], case['wvl0'], case['fwhm'])
+    bad = coracle.walk(par, case['tau'], PMETA['rho_snw'], case['theta0'], case['lb'], case['R'],
+                       mt=coracle.mt_from_numpy(rs), polar=(pc, broken, 1. / out['wvn'], case['stokes']),
+                       allow_unbound_phi=True)
+    assert np.abs(bad['Q_n'] - case['records']['Q_n']).max() > 1e-3
