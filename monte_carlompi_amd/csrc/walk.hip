@@ -310,17 +310,21 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (state == LIVE) {
-            const int gme = lane_rank(live);
-            for (int k = 0; k < G; ++k) {
-                const uint32_t i0 = w.i;
-                const PolarPacket q = pk[gme * G + k];
-                const bool d = w.polar_step(P, q);
-                if (d || w.i >= P.max_steps) {
-                    if (!d) w.cond = 0;
-                    state = DONE;
-                    break;
-                }
-                if (w.i != i0 + 1u) break;   // a bottom re-emission took a step of its own
+            // the moves run straight through the group's packets; the loop's
+            // one exit is an event (rare), the step cap or the last packet.
+            // After an event the next round builds packets from the photon's
+            // step count then (a bottom re-emission walks a step of its own).
+            const PolarPacket *const mine = pk + lane_rank(live) * G;
+            int k = 0;
+            bool ev;
+            do {
+                ev = w.polar_move(P, mine[k]);
+                ++k;
+            } while (!ev && k < G && w.i < P.max_steps);
+            const bool d = ev && w.polar_event(P, mine[k - 1].dtau, mine[k - 1].w);
+            if (d || w.i >= P.max_steps) {
+                if (!d) w.cond = 0;
+                state = DONE;
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");

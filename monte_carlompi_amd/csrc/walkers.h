@@ -43,15 +43,28 @@ __device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v
 
 // The state-independent part of one step of the polarised walk (exact
 // sampler): free path and absorption word from the step's main Philox block,
-// cos theta and the phase matrix there, the first 2 phi candidate and its
-// acceptance uniform.  One lane can build it for any step of any photon.
+// cos theta, sin theta and the phase matrix there, the first 2 phi candidate
+// (with the sines and cosines of phi and 2 phi) and its acceptance uniform.
+// One lane can build it for any step of any photon.
 struct PolarPacket {
     float dtau;
     uint32_t w;
     float ct, r2, acc;
     float p11, p12, p22, p33, p43, p44;
     int seg;
+    float sint, sinp, cosp, c2r, s2r;
 };
+
+// sin and cos of phi and of 2 phi for 2 phi = r2 revolutions (one
+// transcendental pair; no implicit contraction, so every caller rounds alike)
+__device__ __forceinline__ void phi_trig(float r2, float &sinp, float &cosp, float &c2r, float &s2r)
+{
+#pragma clang fp contract(off)
+    sinp = __builtin_amdgcn_sinf(0.5f * r2);
+    cosp = __builtin_amdgcn_cosf(0.5f * r2);
+    c2r = fmaf(cosp, cosp, -sinp * sinp);
+    s2r = 2.f * sinp * cosp;
+}
 
 template <bool POLAR>
 struct NativeWalkerT {
@@ -404,8 +417,8 @@ struct NativeWalkerT {
             const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 2u}, P.key0, P.key1);
             PolarPacket k;
             make_packet(q, q, T, n, pm_mass, k);   // its free path and absorption word go unused
-            float r2 = k.r2, sinp, cosp, c2r, s2r;
-            polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
+            float r2, sinp, cosp, c2r, s2r;
+            polar_phi(P, k, r2, sinp, cosp, c2r, s2r);
             cost = k.ct;
             rev2 = r2;
             seg = k.seg;
@@ -555,48 +568,63 @@ struct NativeWalkerT {
         o.p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
         o.r2 = 2.f * u01f(pb.y);
         o.acc = u01f(pb.z);
+        o.sint = __builtin_amdgcn_sqrtf(fmaf(-ct, ct, 1.f));
+        phi_trig(o.r2, o.sinp, o.cosp, o.c2r, o.s2r);
     }
 
-    // 2 phi given cos theta: the first candidate r2 (revolutions) and its
-    // acceptance uniform come with the packet, retries draw blocks (photon,
-    // case, i, 2 + k); target I S11 + S12 (Q cos 2phi + U sin 2phi), bound
-    // S11 + |S12| sqrt(Q^2 + U^2).  Returns the accepted r2 with the sine and
-    // cosine of the azimuth (r2 / 2) and of 2 phi (one transcendental pair).
-    __device__ __forceinline__ void polar_phi(const KParams &P, float s11, float s12, float acc, float &r2, float &sinp,
+    // 2 phi given cos theta: the first candidate r2 (revolutions), its
+    // acceptance uniform and its sines and cosines come with the packet,
+    // retries draw blocks (photon, case, i, 2 + k); target I S11 + S12 (Q cos
+    // 2phi + U sin 2phi), bound S11 + |S12| sqrt(Q^2 + U^2).  Returns the
+    // accepted r2 with the sine and cosine of the azimuth (r2 / 2) and of 2
+    // phi.  The first candidate is tested outside the retry loop: it is
+    // accepted with probability >= 1/2, so the common step is straight-line.
+    __device__ __forceinline__ void polar_phi(const KParams &P, const PolarPacket &k, float &r2, float &sinp,
                                               float &cosp, float &c2r, float &s2r) const
     {
 #pragma clang fp contract(off)
+        const float s11 = k.p11, s12 = k.p12;
         const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        r2 = k.r2;
+        sinp = k.sinp;
+        cosp = k.cosp;
+        c2r = k.c2r;
+        s2r = k.s2r;
+        if (k.acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r))) return;
         for (uint32_t kk = 1;; ++kk) {
-            sinp = __builtin_amdgcn_sinf(0.5f * r2);
-            cosp = __builtin_amdgcn_cosf(0.5f * r2);
-            c2r = fmaf(cosp, cosp, -sinp * sinp);
-            s2r = 2.f * sinp * cosp;
-            const float val = fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r));
-            if (acc * bound <= val) break;
             if (kk >= POLAR_MAX_TRIES) {
                 atomicAdd(P.ctl + 2, 1ull);
-                break;
+                return;
             }
             const u32x4 q = philox4x32_10(u32x4{photon_id(), case_id(P), i, 2u + kk}, P.key0, P.key1);
             r2 = 2.f * u01f(q.x);
-            acc = u01f(q.y);
+            const float acc = u01f(q.y);
+            phi_trig(r2, sinp, cosp, c2r, s2r);
+            if (acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r))) return;
         }
     }
 
     // One step of an exact-sampler photon given its packet: 2 phi by
     // rejection against the current Stokes vector (retries draw blocks
-    // (photon, case, i, 2 + k)), rotation, Stokes update, move, events.  The
-    // one-lane walk and the tail mode below both step through here, so a
-    // photon's record does not depend on which of them walked it.
-    __device__ bool polar_step(const KParams &P, const PolarPacket &k)
+    // (photon, case, i, 2 + k)), rotation, Stokes update, move (polar_move),
+    // then the events (polar_event).  The one-lane walk and the tail mode
+    // both step through these two, so a photon's record does not depend on
+    // which of them walked it.  Returns true when the walk has ended.
+    __device__ __forceinline__ bool polar_step(const KParams &P, const PolarPacket &k)
+    {
+        return polar_move(P, k) && polar_event(P, k.dtau, k.w);
+    }
+
+    // The move of a step; true when an event follows (the photon left the
+    // slab through either face, or its absorption word says absorbed).
+    __device__ __forceinline__ bool polar_move(const KParams &P, const PolarPacket &k)
     {
 #pragma clang fp contract(off)
         ++i;
-        float r2 = k.r2, sinp, cosp, c2r, s2r;
-        polar_phi(P, k.p11, k.p12, k.acc, r2, sinp, cosp, c2r, s2r);
+        float r2, sinp, cosp, c2r, s2r;
+        polar_phi(P, k, r2, sinp, cosp, c2r, s2r);
         const float cost = k.ct;
-        const float sint = __builtin_amdgcn_sqrtf(fmaf(-cost, cost, 1.f));
+        const float sint = k.sint;
         const float s2 = fmaf(-muz, muz, 1.f);
         const float st = sint * __builtin_amdgcn_rsqf(s2);
         const float mz0 = muz, mx0 = mux, my0 = muy;
@@ -614,7 +642,14 @@ struct NativeWalkerT {
         const float dtau = k.dtau;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
-        if (!(z > 0.f || z < -tau_tot || k.w < t_abs_all)) return false;
+        return z > 0.f || z < -tau_tot || k.w < t_abs_all;
+    }
+
+    // The event after polar_move returned true (dtau, w: that step's free
+    // path and absorption word).  Returns true when the walk has ended; false
+    // after a Lambertian re-emission, which walks a step of its own.
+    __device__ bool polar_event(const KParams &P, float dtau, uint32_t w)
+    {
         if (z > 0.f) {                               // mc3D:1390-1397
             tau_acc -= fminf(z / muz, dtau);
             cond = 1;
@@ -630,7 +665,7 @@ struct NativeWalkerT {
             cond = 2;
             return true;
         }
-        cond = (k.w < t_abs5) ? 5 : 4;
+        cond = (w < t_abs5) ? 5 : 4;
         return true;
     }
 
