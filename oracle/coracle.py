@@ -215,6 +215,32 @@ def polar_run_rank(seed, n_photon, wvl0, half_width, rds_snw, directory, stokes0
     return out, par, case, bands
 
 
+def polar_run_ranks(seed, n_ranks, n_per_rank, wvl0, half_width, rds_snw, directory, stokes0, imp_fn,
+                    imp_cnc, tau_tot, rho_snw, rho_ice, theta_0_deg, lambertian_bottom, r_lambertian,
+                    lambertian_surface=False, threads=None):
+    """Independent aspherical ranks r = 0..n_ranks-1, each polar_run_rank with
+    ``RandomState(seed + r)`` and ``n_per_rank`` photons, on host threads (the
+    GIL is released inside C).  Returns (records incl. Q_n, U_n, V_n and the
+    per-photon parameters, both concatenated rank-major; rank 0's case
+    tables; every rank's band tables by wavelength).  Used for the native
+    polarised walk's statistics."""
+    def one(r):
+        return polar_run_rank(seed + r, n_per_rank, wvl0, half_width, rds_snw, directory, stokes0, imp_fn,
+                              imp_cnc, tau_tot, rho_snw, rho_ice, theta_0_deg, lambertian_bottom,
+                              r_lambertian, lambertian_surface=lambertian_surface)
+    T = threads or min(n_ranks, 16, len(os.sched_getaffinity(0)))
+    with ThreadPoolExecutor(max_workers=T) as ex:
+        parts = list(ex.map(one, range(n_ranks)))
+    recs = [p[0] for p in parts]
+    out = {k: np.concatenate([q[k] for q in recs]) for k in recs[0] if k != 'steps'}
+    out['steps'] = sum(q['steps'] for q in recs)
+    par = {k: np.concatenate([p[1][k] for p in parts]) for k in parts[0][1]}
+    bands = {}
+    for p in parts:
+        bands.update(p[3])
+    return out, par, parts[0][2], bands
+
+
 def run_ranks(seed, n_ranks, n_per_rank, par_fn, tau_tot, rho_snw, theta_0_deg,
               lambertian_bottom, r_lambertian, threads=None):
     """Independent ranks r = 0..n_ranks-1 with ``RandomState(seed + r)`` walking

@@ -355,3 +355,69 @@ def test_polar_bench_workload_follows_the_oracle_law(walker, tmp_path):
     rec = walker.records()
     st = walker.stokes()
     assert_same_law(o, rec, st, polarised=True)
+
+
+def c_oracle_law(rec_c):
+    return dict(condition=rec_c['condition'], theta_n=rec_c['theta_n'], n_scat=rec_c['n_scat'],
+                Q_n=rec_c['Q_n'], U_n=rec_c['U_n'], V_n=rec_c['V_n'])
+
+
+def assert_same_mean(a, b, what):
+    a = np.asarray(a, float)
+    b = np.asarray(b, float)
+    sd = np.sqrt(a.var() / a.size + b.var() / b.size) + 1e-12
+    assert abs(a.mean() - b.mean()) <= 4 * sd, (what, a.mean(), b.mean(), sd)
+
+
+def test_polar_bench_workload_against_the_c_oracle_at_scale(walker, tmp_path):
+    """The bench.py polar workload, 1e6 GPU photons against 2e5 photons of the
+    C aspherical walk (oracle/mcoracle.c, FP64, 16 seeded ranks): exit
+    fractions, reflected zenith angles and mean final Q, U, V (4 sigma), and
+    the mean scattering count (4 sigma) -- bars ~8x tighter than the 3,000
+    Python-oracle photons allow."""
+    from oracle import coracle
+    from monte_carlompi_amd import optics
+    from monte_carlompi_amd.workloads import RHO_ICE, RHO_SNW, SNOW_POLAR, snow_polar_case
+    d = str(tmp_path)
+    case = snow_polar_case(1_000_000, d)
+    p = SNOW_POLAR
+    rc, _, _, _ = coracle.polar_run_ranks(
+        500, 16, 12_500, p['wvl'], 1e-12, p['r_eff'], optics.aspherical_dir(d, p['shape'], p['roughness'], p['wvl']),
+        p['stokes0'], lambda w: dict(ssa=np.zeros(w.shape), ext=np.zeros(w.shape)), 0.0, case.tau_tot, RHO_SNW,
+        RHO_ICE, p['theta0'], True, p['r_lambertian'])
+    o = c_oracle_law(rc)
+    walker.set_cases([case])
+    walker.set_launch(1024, 1, 2048, 16, 16)
+    walker.walk(mode='native', seed=78, records=True)
+    rec = walker.records()
+    st = walker.stokes()
+    assert_same_law(o, rec, st, polarised=True)
+    assert_same_mean(o['n_scat'], rec['n_scat'], 'n_scat')
+
+
+@pytest.mark.parametrize('name', ['pm_linear_q', 'pm_circular', 'pm_spectral_u'])
+def test_polar_native_against_the_c_oracle_at_scale(walker, name, tmp_path):
+    """Polarised golden cases (Q-, V- and U-polarised incidence; the last
+    with its wavelength spread, several tables), 4e5 GPU photons against 1e5
+    photons of the C aspherical walk in 8 seeded ranks: exit fractions,
+    reflected zenith angles, mean final Q, U, V and mean scattering count,
+    all at 4 sigma."""
+    from oracle import coracle
+    case = [c for c in CASES if c['name'] == name][0]
+    d = golden_polar.table_dir(str(tmp_path), case)
+    imp_fn = lambda w: dict(ssa=np.full(w.shape, IMP['ssa']), ext=np.full(w.shape, IMP['ext']))
+    rc, par, tabs, bands = coracle.polar_run_ranks(
+        600, 8, 12_500, case['wvl0'], case['fwhm'], case['rds'], d, case['stokes'], imp_fn, case['imp_cnc'],
+        case['tau'], META['rho_snw'], META['rho_ice'], case['theta0'], case['lb'], case['R'],
+        lambertian_surface=case.get('ls', False))
+    o = c_oracle_law(rc)
+    # the GPU case from the oracle's own photons: its tables, and band weights
+    # from the ranks' wavelength draws
+    g, pband = gpu_case(case, par, tabs, bands)
+    g.n_photons = 400_000
+    walker.set_cases([g])
+    walker.walk(mode='native', seed=79, records=True)
+    rec = walker.records()
+    st = walker.stokes()
+    assert_same_law(o, rec, st, polarised=True)
+    assert_same_mean(o['n_scat'], rec['n_scat'], 'n_scat')

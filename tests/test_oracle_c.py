@@ -120,3 +120,44 @@ def test_c_oracle_polar_catches_a_dropped_stokes_term(tmp_path):
                        mt=coracle.mt_from_numpy(rs), polar=(pc, broken, 1. / out['wvn'], case['stokes']),
                        allow_unbound_phi=True)
     assert np.abs(bad['Q_n'] - case['records']['Q_n']).max() > 1e-3
+
+
+def test_polar_law_check_rejects_a_dropped_p12(tmp_path):
+    """The at-scale law comparison of the GPU tests (exit fractions,
+    reflected zenith angles, mean final Q, U, V, mean scattering count at 4
+    sigma) has the power to see a wrong phase matrix: 5e4 C-oracle photons of
+    the Q-polarised case pass against 5e4 more of their own law and fail
+    against 5e4 walked with P12 zeroed (the I-Q coupling dropped)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from tests import golden_polar
+    from tests.test_gpu_polar import assert_same_law, assert_same_mean, c_oracle_law
+    case = [c for c in PCASES if c['name'] == 'pm_linear_q'][0]
+    d = golden_polar.table_dir(str(tmp_path), case)
+    imp_fn = lambda w: dict(ssa=np.full(w.shape, 0.3), ext=np.full(w.shape, 12000.0))
+    _, par1, pc, bands = coracle.polar_run_rank(
+        case['seed'], 1, case['wvl0'], 1e-12, case['rds'], d, case['stokes'], imp_fn, case['imp_cnc'],
+        case['tau'], PMETA['rho_snw'], PMETA['rho_ice'], case['theta0'], case['lb'], case['R'])
+    broken = {k: dict(b, Y=dict(b['Y'], P12=np.zeros_like(b['Y']['P12']))) for k, b in bands.items()}
+    per = 6250
+    par = {k: np.full(per, v[0]) for k, v in par1.items()}
+    keys = par['wvl']
+
+    def sample(tabs, seed0):
+        def one(r):
+            mt = coracle.mt_from_numpy(np.random.RandomState(seed0 + r))
+            return coracle.walk(par, case['tau'], PMETA['rho_snw'], case['theta0'], case['lb'], case['R'], mt=mt,
+                                polar=(pc, tabs, keys, case['stokes']), allow_unbound_phi=True)
+        with ThreadPoolExecutor(max_workers=8) as ex:
+            parts = list(ex.map(one, range(8)))
+        out = {k: np.concatenate([p[k] for p in parts]) for k in parts[0] if k != 'steps'}
+        law = c_oracle_law(out)
+        return law, dict(condition=law['condition'], theta_n=law['theta_n'], n_scat=law['n_scat']), \
+            {k: law[k] for k in ('Q_n', 'U_n', 'V_n')}
+
+    a, _, _ = sample(bands, 1000)
+    _, rb, sb = sample(bands, 2000)
+    assert_same_law(a, rb, sb, polarised=True)
+    assert_same_mean(a['n_scat'], rb['n_scat'], 'n_scat')
+    _, rx, sx = sample(broken, 2000)
+    with pytest.raises(AssertionError):
+        assert_same_law(a, rx, sx, polarised=True)
