@@ -208,7 +208,8 @@ def test_polar_tail_mode_gives_the_one_lane_records(walker, tmp_path):
     case = snow_polar_case(150_000, str(tmp_path))
     walker.set_cases([case])
     runs = []
-    for shape in [(512, 2, 2048, 16, 0), (512, 2, 2048, 16, 16), (256, 4, 1024, 8, 8), (1024, 1, 600, 16, 16)]:
+    for shape in [(512, 2, 2048, 16, 0), (512, 2, 2048, 16, 16), (256, 4, 1024, 8, 8), (1024, 1, 600, 16, 16),
+                  (384, 2, 1024, 16, 16)]:
         walker.set_launch(*shape)
         walker.walk(mode='native', seed=21, records=True)
         rec = walker.records()
