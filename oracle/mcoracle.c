@@ -19,7 +19,11 @@
  * a numpy state consumes exactly the deviates the reference would draw.
  * Pins: draw-for-draw equality with numpy (tests/test_oracle_c.py) and
  * record-for-record equality with oracle/walk.py, which the golden records
- * written by the reference pin in turn.
+ * written by the reference pin in turn; for aspherical grains the golden
+ * phase-matrix records (conditions, scattering counts and the uniform
+ * stream exactly, angles and Q, U, V to libm-vs-numpy rounding), and a
+ * planted P12 = 0 that moves the Stokes vectors.  np.interp is restated
+ * after numpy's compiled_base.c rules (interp_np).
  *
  * The MT19937 generator is the published Matsumoto-Nishimura (1998)
  * algorithm; 53-bit doubles are (a>>5, b>>6) -> (a*2^26 + b) / 2^53, the
