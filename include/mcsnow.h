@@ -211,7 +211,12 @@ typedef struct {
  * clears the polar state (mcs_set_cases also does).  Walks then sample the
  * angles by the reference's rejection method (mc3D:905-1004; native mode:
  * Philox blocks (photon, case, step, 2 + attempt)) and update the Stokes
- * vector at every scattering (mc3D:1283-1342). */
+ * vector at every scattering (mc3D:1283-1342).  The native walk draws the
+ * same law by inversion of the S11 marginal plus an azimuth rejection when
+ * the band's table has P11 >= |P12| >= 0 and keeps physical vectors physical
+ * at every grid point and the case's initial vector is physical (I > 0,
+ * Q^2 + U^2 + V^2 <= I^2); otherwise it runs the two-area rejection
+ * (counted in mcs_read_counts slot 11). */
 int mcs_set_polar(mcs_ctx *ctx, const mcs_polar_case *pc, int32_t n_cases,
                   const double *tables, int32_t n_tables, int32_t n_angles);
 
@@ -235,7 +240,10 @@ int mcs_read_tallies(mcs_ctx *ctx, double *out, int32_t n_cases);
  *   0 photons, 1..5 photons by condition, 6 capped, 7 phi on an axis,
  *   8 photon-steps, 9 photon-steps walked in the cooperative tail mode
  *   (mcs_launch.coop_live), 10 photons handed from one wave's cooperative
- *   tail to an idle wave of the same block, 11..15 reserved (0) */
+ *   tail to an idle wave of the same block, 11 scattering angles of the
+ *   native polarised walk drawn by the reference's two-area rejection
+ *   (tables or initial Stokes vectors the exact sampler does not cover, see
+ *   mcs_set_polar), 12..15 reserved (0) */
 int mcs_read_counts(mcs_ctx *ctx, uint64_t *out, int32_t n_cases);
 
 int mcs_last_stats(const mcs_ctx *ctx, mcs_stats *out);

@@ -25,6 +25,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -164,6 +165,9 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
                 atomicAdd(&T.u[U_CAP], 1ull);
             }
             if (axis) atomicAdd(&T.u[U_AXIS], 1ull);
+            if constexpr (W::STOKES && sizeof(typename W::Real) == 4) {
+                if (w.n_rej) atomicAdd(&T.u[U_REJ], (unsigned long long)w.n_rej);
+            }
             if (w.cond == 1) {
                 const double thd = (double)th, phd = (double)ph;
                 int jt = (int)(thd * (MCS_THETA_BINS / (PI_D / 2)));
@@ -648,6 +652,19 @@ static int set_err(mcs_ctx *c, int code, const char *fmt, ...)
                            "%s: %s", #expr, hipGetErrorString(e_));                    \
     } while (0)
 
+// Allocation-failure injection for tests of the MCS_ERR_NOMEM paths: with
+// MCS_FAIL_ALLOC_BYTES=N in the environment (read at every allocation), any
+// device buffer of N bytes or more that the library would allocate fails as
+// if the device were out of memory.
+static bool injected_alloc_failure(size_t bytes)
+{
+    const char *e = std::getenv("MCS_FAIL_ALLOC_BYTES");
+    if (!e || !*e) return false;
+    char *end = nullptr;
+    const unsigned long long n = std::strtoull(e, &end, 10);
+    return end != e && bytes >= n;
+}
+
 static int ensure(mcs_ctx *c, DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 8;
@@ -658,6 +675,9 @@ static int ensure(mcs_ctx *c, DevBuf &b, size_t bytes)
         b.p = nullptr;
         b.cap = 0;
     }
+    c->gkey = 0;  // buffer addresses change: recapture any graph
+    if (injected_alloc_failure(bytes))
+        return set_err(c, MCS_ERR_NOMEM, "hipMalloc(%zu): out of memory (injected by MCS_FAIL_ALLOC_BYTES)", bytes);
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess)
         return set_err(c, MCS_ERR_NOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -691,6 +711,31 @@ static void build_alias(DevBand *db, const mcs_band *b, int n)
         q[l] -= 1.0 - q[s];
         if (q[l] < 1.0) { large.pop_back(); small.push_back(l); }
     }
+}
+
+// Does the scattering matrix [[P11, P12, 0, 0], [P12, P22, 0, 0], [0, 0, P33,
+// -P43], [0, 0, P43, P44]] map every fully polarised vector (1, q, u, v),
+// q^2 + u^2 + v^2 = 1, to a vector of degree of polarisation <= 1?  With r^2 =
+// u^2 + v^2 = 1 - q^2 the (U, V) part reaches at most s r, s the largest
+// singular value of [[P33, -P43], [P43, P44]], so the condition is
+//   f(q) = (P11 + P12 q)^2 - (P12 + P22 q)^2 - s^2 (1 - q^2) >= 0 on [-1, 1],
+// a quadratic in q: its ends and, if inside, its vertex.  Tolerance: the
+// table files carry ~7 significant digits.
+static bool keeps_physical(double p11, double p12, double p22, double p33, double p43, double p44)
+{
+    const double ss = p33 * p33 + 2.0 * p43 * p43 + p44 * p44;
+    const double det = p33 * p44 + p43 * p43;
+    const double s2 = 0.5 * (ss + std::sqrt(std::max(ss * ss - 4.0 * det * det, 0.0)));
+    const double A = p12 * p12 - p22 * p22 + s2;
+    const double B = 2.0 * p12 * (p11 - p22);
+    const double C = p11 * p11 - p12 * p12 - s2;
+    const double tol = -1e-5 * p11 * p11;
+    if (A + B + C < tol || A - B + C < tol) return false;
+    if (A > 0.0) {
+        const double qv = -B / (2.0 * A);
+        if (qv > -1.0 && qv < 1.0 && C - B * B / (4.0 * A) < tol) return false;
+    }
+    return true;
 }
 
 static uint32_t prob_bits(double p)
@@ -816,6 +861,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         D.lamb = C.lambertian_bottom ? 1 : 0; D.band_begin = C.band_begin; D.n_bands = C.n_bands;
         D.surf = C.lambertian_surface ? 1 : 0;
         D.polar = 0;
+        D.s0_exact = 0;
         D.s0[0] = 1.0; D.s0f[0] = 1.f;   // (1, 0, 0, 0): unpolarised until mcs_set_polar
         D.n_photons = C.n_photons; D.gid_base = total;
         total += C.n_photons;
@@ -823,6 +869,12 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
     }
     (void)hipSetDevice(c->device);
     if (c->pending) (void)hipStreamSynchronize(c->stream), c->pending = false;
+    // until the new tables are on the device the context holds no cases: a
+    // failure below (e.g. MCS_ERR_NOMEM after a buffer was freed to grow it)
+    // leaves it in that state, so a later walk reports MCS_ERR_STATE
+    c->cases_set = false;
+    c->polar = false;
+    c->last_records = c->last_stokes = false;
     int r;
     if ((r = ensure(c, c->d_cases, sizeof(DevCase) * n_cases))) return r;
     if ((r = ensure(c, c->d_bands, sizeof(DevBand) * n_bands))) return r;
@@ -855,6 +907,7 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     std::vector<DevBand> db = c->dbands;
     for (auto &D : dc) {
         D.polar = 0;
+        D.s0_exact = 0;
         D.s0[0] = 1.0; D.s0[1] = D.s0[2] = D.s0[3] = 0.0;
         D.s0f[0] = 1.f; D.s0f[1] = D.s0f[2] = D.s0f[3] = 0.f;
     }
@@ -889,6 +942,11 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
             const double *x0 = tables + (size_t)Q.table * 7 * n;
             const int jc = n - 1 - Q.cutoff;   // the cutoff in ascending order
             D.polar = 1;
+            // the exact sampler needs a physical initial vector (I > 0, degree
+            // of polarisation <= 1); other cases take the two-area rejection
+            const double s0i = Q.stokes0[0];
+            D.s0_exact = (s0i > 0.0 && Q.stokes0[1] * Q.stokes0[1] + Q.stokes0[2] * Q.stokes0[2] +
+                                           Q.stokes0[3] * Q.stokes0[3] <= s0i * s0i) ? 1 : 0;
             for (int q = 0; q < 4; ++q) { D.s0[q] = Q.stokes0[q]; D.s0f[q] = (float)Q.stokes0[q]; }
             D.cmax = x0[n - 1]; D.cmin = x0[0]; D.ccut = x0[jc];
             D.fcmax = (float)D.cmax; D.fcmin = (float)D.cmin; D.fccut = (float)D.ccut;
@@ -914,7 +972,10 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
                 bool exact = true;
                 double mass = 0.0;
                 for (int k2 = 0; k2 < n; ++k2) {
-                    if (!(x[n + k2] >= 0.0) || std::fabs(x[2 * n + k2]) > x[n + k2]) exact = false;
+                    if (!(x[n + k2] >= 0.0) || std::fabs(x[2 * n + k2]) > x[n + k2] ||
+                        !keeps_physical(x[n + k2], x[2 * n + k2], x[3 * n + k2], x[4 * n + k2], x[5 * n + k2],
+                                        x[6 * n + k2]))
+                        exact = false;
                     if (k2 + 1 < n) mass += 0.5 * (x[n + k2] + x[n + k2 + 1]) * (x[k2 + 1] - x[k2]);
                 }
                 B.pm_exact = (exact && mass > 0.0) ? 1 : 0;
@@ -925,6 +986,10 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     }
     (void)hipSetDevice(c->device);
     if (c->pending) (void)hipStreamSynchronize(c->stream), c->pending = false;
+    // a failure below may leave device tables that disagree with the host
+    // copies (or freed): the context then holds no cases until mcs_set_cases
+    c->cases_set = false;
+    c->last_records = c->last_stokes = false;
     int r;
     if (any) {
         const size_t nv = (size_t)n_tables * 7 * n;
@@ -960,6 +1025,7 @@ int mcs_set_polar(mcs_ctx *c, const mcs_polar_case *pc, int32_t n_cases, const d
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->dcases.swap(dc);
     c->dbands.swap(db);
+    c->cases_set = true;
     c->polar = any;
     c->pm_n = any ? n : 0;
     c->pm_floats = any ? (size_t)n_tables * ((size_t)14 * n + PM_GUIDE + 1) : 0;
@@ -1125,6 +1191,9 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
         if (nch > 0x7FFFFFFF) return set_err(c, MCS_ERR_ARG, "too many chunks; raise the chunk size");
     }
     chunk0[nc] = (int32_t)nch;
+    // from here on the previous walk's records are no longer readable (their
+    // buffers may be regrown below, and a failure leaves none)
+    c->last_records = c->last_stokes = false;
     int r;
     if (!c->chunk0_valid || c->chunk0 != chunk0) {
         c->chunk0 = chunk0;
@@ -1361,6 +1430,7 @@ static int restore_polar(mcs_ctx *c, const std::vector<DevCase> &dc, const std::
         DevCase &D = c->dcases[k];
         const DevCase &S = dc[k];
         D.polar = S.polar;
+        D.s0_exact = S.s0_exact;
         for (int q = 0; q < 4; ++q) { D.s0[q] = S.s0[q]; D.s0f[q] = S.s0f[q]; }
         D.cmax = S.cmax; D.ccut = S.ccut; D.cmin = S.cmin;
         D.fcmax = S.fcmax; D.fccut = S.fccut; D.fcmin = S.fcmin;

@@ -20,7 +20,7 @@ constexpr int T_QCOND = MCS_BRF_BINS + 1;    // 649..653 by condition 1..5
 constexpr int T_QCAP = MCS_BRF_BINS + 6;     // 654
 constexpr int T_QPATH = MCS_BRF_BINS + 7;    // 655
 // counts (uint64)
-constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED = 10;
+constexpr int U_N = 0, U_CAP = 6, U_AXIS = 7, U_STEPS = 8, U_COOP = 9, U_MOVED = 10, U_REJ = 11;
 // rejection attempts per scattering angle before the walk reports an error
 // (ctl[2]); valid tables accept within a few
 constexpr uint32_t POLAR_MAX_TRIES = 1u << 20;
@@ -35,6 +35,9 @@ struct DevCase {
     // aspherical particles: initial Stokes vector and the rejection sampler's
     // cos(theta) range and split (mc3D:858-883, 1132)
     int32_t polar;
+    // the initial Stokes vector is physical (I > 0, Q^2 + U^2 + V^2 <= I^2):
+    // required by the native walk's exact sampler (see DevBand::pm_exact)
+    int32_t s0_exact;
     double s0[4], cmax, ccut, cmin;
     float s0f[4], fcmax, fccut, fcmin;
 };
@@ -59,8 +62,12 @@ struct DevBand {
     double pA1, pB1, pA2, pB2;
     float fA1, fB1, fA2, fB2;
     // native mode: |P12| <= P11 on the whole grid, so I S11 + S12 (Q cos 2phi
-    // + U sin 2phi) >= 0 for |(Q, U)| <= I and its cos(theta) marginal is S11:
-    // the walk samples it exactly by inversion (fM = integral of S11)
+    // + U sin 2phi) >= 0 for |(Q, U)| <= I and its cos(theta) marginal is I S11:
+    // the walk samples it exactly by inversion (fM = integral of S11).  The
+    // matrix must also keep every physical Stokes vector physical (degree of
+    // polarisation <= 1 after scattering), so that |(Q, U)| <= I holds at
+    // every later scattering; photons of a case whose initial vector is not
+    // physical use the two-area rejection (DevCase::s0_exact)
     int32_t pm_exact;
     float fM;
 };

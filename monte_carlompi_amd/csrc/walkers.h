@@ -92,6 +92,7 @@ struct NativeWalkerT {
     int pm;
     bool pm_exact;               // the band's exact sampler applies (see DevBand)
     float pm_mass;               // integral of S11 over the grid
+    uint32_t n_rej;              // scatterings drawn by the two-area rejection
 
     // Set up photon `photon` of case `case_id` and take its first step (or
     // its whole walk in the Lambertian surface mode); true when it is done.
@@ -109,8 +110,10 @@ struct NativeWalkerT {
         if constexpr (POLAR) {
             const DevBand &PB = P.bands[C.band_begin + bnd];
             pm = C.polar ? PB.pm : -1;
-            pm_exact = PB.pm_exact != 0;   // per-photon constants stay in registers:
-            pm_mass = PB.fM;               // a step reads only the tables
+            // per-photon constants stay in registers: a step reads only the tables
+            pm_exact = PB.pm_exact != 0 && C.s0_exact != 0;
+            pm_mass = PB.fM;
+            n_rej = 0u;
             sI = C.s0f[0]; sQ = C.s0f[1]; sU = C.s0f[2]; sV = C.s0f[3];
         }
         if (C.surf) return surface_walk(P, C);
@@ -400,7 +403,7 @@ struct NativeWalkerT {
     // [0, 4 pi).  Attempt k of step i draws Philox block (photon, case, i,
     // 2 + k): area, cos theta, 2 phi, acceptance.  Returns cos theta and
     // 2 phi in revolutions.
-    __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg) const
+    __device__ void polar_angles(const KParams &P, float &cost, float &rev2, int &seg)
     {
         const uint32_t cs = case_id(P);
         const float *T = pm_table(P);
@@ -424,6 +427,7 @@ struct NativeWalkerT {
             seg = k.seg;
             return;
         }
+        ++n_rej;
         const DevCase &C = P.cases[cs];
         const DevBand &B = P.bands[C.band_begin + band];
         const float cq = __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
@@ -575,7 +579,8 @@ struct NativeWalkerT {
     // 2 phi given cos theta: the first candidate r2 (revolutions), its
     // acceptance uniform and its sines and cosines come with the packet,
     // retries draw blocks (photon, case, i, 2 + k); target I S11 + S12 (Q cos
-    // 2phi + U sin 2phi), bound S11 + |S12| sqrt(Q^2 + U^2).  Returns the
+    // 2phi + U sin 2phi), bound |I| S11 + |S12| sqrt(Q^2 + U^2) (I = 1 after
+    // the first scattering; the initial vector keeps its own I).  Returns the
     // accepted r2 with the sine and cosine of the azimuth (r2 / 2) and of 2
     // phi.  The first candidate is tested outside the retry loop: it is
     // accepted with probability >= 1/2, so the common step is straight-line.
@@ -584,7 +589,7 @@ struct NativeWalkerT {
     {
 #pragma clang fp contract(off)
         const float s11 = k.p11, s12 = k.p12;
-        const float bound = s11 + fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU));
+        const float bound = fmaf(fabsf(sI), s11, fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU)));
         r2 = k.r2;
         sinp = k.sinp;
         cosp = k.cosp;

@@ -6,10 +6,15 @@ The estimators of post_processing.py (pp):
   w_k = sin m_k cos m_k / sum sin m cos m at bin midpoints m (pp:46-100);
 * ``albedo``: sum(wvn | condition 1) / sum(wvn) (pp:125-128);
 * ``non_absorbed``: adds conditions 2 and 3 (pp:269-275);
+* ``spectral_albedo``: albedo and non-absorbed fraction of every record file
+  grouped by snow effective radius and nominal wavelength, with the actual
+  wavelength mean(1 / wvn) of each file (pp:250-312, files grouped as in
+  MonteCarloData, pp:19-43);
 * ``calculate_bins``: int(pi d_dome / (4 active_area)) (pp:435-444).
 Plotting is not reproduced; the CLI prints the numbers.
 
     python -m monte_carlompi_amd.post_processing --output_dir D FILE [FILE ...]
+    python -m monte_carlompi_amd.post_processing --spectral --output_dir D FILE [FILE ...]
 """
 from __future__ import annotations
 
@@ -66,6 +71,36 @@ def non_absorbed(data):
     return float(wvn[(cond == 1) | (cond == 2) | (cond == 3)].sum() / wvn.sum())
 
 
+def group_files(file_list, output_dir='.'):
+    """{rds_snw: {wvl0: data}} from record file names WVL_FWHM_REFF_...
+    read under ``output_dir`` (MonteCarloData, pp:19-43: the radius is the
+    third field and the nominal wavelength the first; a later file with the
+    same pair replaces an earlier one)."""
+    out = {}
+    for name in file_list:
+        base = os.path.basename(name)
+        wvl0, _, rds = parse_name(base)
+        path = os.path.join(output_dir, base)
+        out.setdefault(rds, {})[wvl0] = read_output(path)
+    return out
+
+
+def spectral_albedo(data_dict):
+    """Rows (rds_snw, wvl0, actual_wvl0, albedo, non_absorbed) in order of
+    radius then nominal wavelength, for {rds_snw: {wvl0: data}} (a record
+    DataFrame or a dict of arrays per file) -- the numbers pp:250-312 plots:
+    actual_wvl0 = mean(1 / wvn), albedo = sum(wvn | cond 1) / sum(wvn),
+    non_absorbed adds conditions 2 and 3."""
+    rows = []
+    for rds in sorted(data_dict):
+        for wvl0 in sorted(data_dict[rds]):
+            data = data_dict[rds][wvl0]
+            cond, wvn, _ = _cols(data)
+            actual = float(np.mean(1.0 / wvn))
+            rows.append((rds, wvl0, actual, albedo(data), non_absorbed(data)))
+    return rows
+
+
 def brf_1d_from_tallies(tally_row):
     """Zenith BRF (18 bins of 5 deg) from one case's GPU tallies: the phi
     marginal of the 18 x 36 histogram in pp's estimator."""
@@ -82,8 +117,15 @@ def main():
     ap.add_argument('--output_dir', type=str, default='.')
     ap.add_argument('--active_area', type=float, default=1.)
     ap.add_argument('--d_dome', type=float, default=175.)
+    ap.add_argument('--spectral', action='store_true',
+                    help='albedo per snow radius and wavelength over all files (spectral_albedo)')
     ap.add_argument('file_list', nargs=argparse.REMAINDER)
     a = ap.parse_args()
+    if a.spectral:
+        for rds, wvl0, actual, alb, non_abs in spectral_albedo(group_files(a.file_list, a.output_dir)):
+            print('albedo(%s, %s) = %r   non-absorbed %r   (nominal wavelength %s um)'
+                  % (actual, rds, alb, non_abs, wvl0))
+        return
     n_bins = calculate_bins(a.active_area, a.d_dome)
     print('BRF computing over %d bins' % n_bins)
     for name in a.file_list:

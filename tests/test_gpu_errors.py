@@ -126,3 +126,43 @@ def test_cases_sharing_band_rows(walker):
     bad = carr(_lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 0, 2, 0, 100),
                _lib.McsCase(5.0, 0.5, -np.sqrt(0.75), 0.0, 0, 1, 2, 0, 100))
     assert _lib.lib().mcs_set_cases(walker._ctx, bad, 2, bands, 3) == -1   # MCS_ERR_ARG
+
+
+def test_out_of_memory_is_reported_and_context_recovers(monkeypatch):
+    """Device allocations that fail (MCS_FAIL_ALLOC_BYTES makes every buffer
+    of that size or more fail like hipMalloc would) come back as
+    MCS_ERR_NOMEM from the call that needed them, nothing is launched, and
+    the same context works again once memory is available: growing the case
+    table fails (the context then holds no cases: MCS_ERR_STATE), growing the
+    record buffers fails (the previous records are no longer readable), and
+    a walk without records still runs at the failing size."""
+    w = GpuWalker(0)
+    try:
+        w.set_cases([good_case(1000)])
+        w.walk(records=True)
+        assert w.counts()[0][0] == 1000
+        # case table: 1000 cases need more than the one-case buffer holds
+        monkeypatch.setenv('MCS_FAIL_ALLOC_BYTES', '4096')
+        with pytest.raises(McsError, match='NOMEM'):
+            w.set_cases([good_case(10) for _ in range(1000)])
+        with pytest.raises(McsError, match='STATE'):
+            w.walk()
+        monkeypatch.delenv('MCS_FAIL_ALLOC_BYTES')
+        w.set_cases([good_case(10) for _ in range(1000)])
+        w.walk(records=True)
+        assert int(w.counts()[:, 0].sum()) == 10_000
+        # record buffers: 4 bytes x 2e7 photons = 80 MB each
+        w.set_cases([good_case(20_000_000, tau_tot=0.5)])
+        monkeypatch.setenv('MCS_FAIL_ALLOC_BYTES', str(64 << 20))
+        with pytest.raises(McsError, match='NOMEM'):
+            w.walk(records=True)
+        with pytest.raises(McsError, match='STATE'):
+            w.records()
+        w.walk(records=False)                      # needs no new buffer
+        assert w.counts()[0][0] == 20_000_000
+        monkeypatch.delenv('MCS_FAIL_ALLOC_BYTES')
+        w.walk(records=True)
+        r = w.records()
+        assert r['condition'].size == 20_000_000 and np.all(r['condition'] >= 1)
+    finally:
+        w.close()

@@ -422,3 +422,92 @@ def test_polar_native_against_the_c_oracle_at_scale(walker, name, tmp_path):
     st = walker.stokes()
     assert_same_law(o, rec, st, polarised=True)
     assert_same_mean(o['n_scat'], rec['n_scat'], 'n_scat')
+
+
+def c_oracle_case(walker, case, d, n_gpu, n_c, seed):
+    """The C aspherical walk (16 seeded ranks of n_c / 16 photons) on the
+    tables under ``d`` and the GPU native walk of n_gpu photons of the same
+    case; returns (oracle law incl. phi_n, GPU records, GPU Stokes, GPU
+    counts)."""
+    from oracle import coracle
+    imp_fn = lambda w: dict(ssa=np.full(w.shape, IMP['ssa']), ext=np.full(w.shape, IMP['ext']))
+    rc, par, tabs, bands = coracle.polar_run_ranks(
+        seed, 16, n_c // 16, case['wvl0'], case['fwhm'], case['rds'], d, case['stokes'], imp_fn, case['imp_cnc'],
+        case['tau'], META['rho_snw'], META['rho_ice'], case['theta0'], case['lb'], case['R'],
+        lambertian_surface=case.get('ls', False))
+    g, pband = gpu_case(case, par, tabs, bands)
+    g.n_photons = n_gpu
+    walker.set_cases([g])
+    walker.walk(mode='native', seed=seed + 1, records=True)
+    o = c_oracle_law(rc)
+    o['phi_n'] = rc['phi_n']
+    return o, walker.records(), walker.stokes(), walker.counts()[0]
+
+
+def single_scatter_cos2phi(cond, n_scat, phi):
+    """cos(2 phi_n) of singly scattered reflected photons: under normal
+    incidence phi_n is the scattering azimuth, whose law is I S11 + S12 (Q cos
+    2phi + U sin 2phi), so E[cos 2phi] = Q S12 / (2 I S11) averaged over the
+    reflected directions -- the statistic that sees the azimuth bound."""
+    m = (cond == 1) & (n_scat == 1)
+    return np.cos(2.0 * phi[m])
+
+
+@pytest.mark.parametrize('stokes0,exact', [((2.0, 1.0, 0.0, 0.0), True), ((1.0, 1.0, 1.0, 0.0), False)])
+def test_polar_native_initial_stokes_not_normalised(walker, tmp_path, stokes0, exact):
+    """An initial Stokes vector with I != 1 (the azimuth rejection's bound must
+    scale with I, as the reference's |I| max P11 + max|P12| C does) and one
+    that is not physical (Q^2 + U^2 > I^2: the exact sampler's S11 marginal
+    does not hold, so the walk takes the reference's two-area rejection,
+    counted in slot 11).  Normal incidence, free bottom: 2e6 GPU photons
+    against 2e6 photons of the C aspherical walk -- exit fractions, reflected
+    zenith angles, mean final Q, U, V, mean scattering count and the mean
+    cos 2 phi of singly scattered reflected photons (which a bound that
+    ignores I would flatten to 0), all at 4 sigma."""
+    case = [c for c in CASES if c['name'] == 'pm_linear_q'][0]
+    case = dict(case, stokes=list(stokes0), lb=False, R=0.0)
+    d = golden_polar.table_dir(str(tmp_path), case)
+    o, rec, st, cnt = c_oracle_case(walker, case, d, 2_000_000, 2_000_000, seed=640)
+    assert_same_law(o, rec, st, polarised=True)
+    assert_same_mean(o['n_scat'], rec['n_scat'], 'n_scat')
+    a = single_scatter_cos2phi(o['condition'], o['n_scat'], o['phi_n'])
+    b = single_scatter_cos2phi(rec['condition'], rec['n_scat'], rec['phi_n'])
+    assert a.size > 50_000 and b.size > 50_000
+    assert_same_mean(a, b, 'cos 2 phi, single scattering')
+    if exact:
+        assert cnt[11] == 0, cnt[11]
+        assert abs(a.mean()) > 6 * np.sqrt(a.var() / a.size), a.mean()   # the law is not flat
+    else:
+        assert cnt[11] > 0
+
+
+def test_polar_native_two_area_rejection_table(walker, tmp_path):
+    """A table with |P12| slightly above P11 at one grid point (so the exact
+    sampler is off for it and the walk runs the reference's two-area
+    rejection for every scattering, slot 11 of the counts): 4e5 GPU photons
+    against 4e5 photons of the C aspherical walk on the same files -- exit
+    fractions, reflected zenith angles, mean final Q, U, V and mean
+    scattering count at 4 sigma."""
+    import os
+    case = [c for c in CASES if c['name'] == 'pm_free'][0]
+    case = dict(case, stokes=[1.0, 1.0, 0.0, 0.0])
+    d = golden_polar.table_dir(str(tmp_path), case)
+    path = os.path.join(d, 'P12.dat')
+    with open(path) as fh:
+        lines = fh.read().split('\n')
+    theta = np.array([float(v) for v in lines[0].split()])
+    k = int(np.argmin(np.abs(theta - 100.0)))
+    out = [lines[0]]
+    for ln in lines[1:]:
+        if not ln.strip():
+            continue
+        v = ln.split()
+        v[k] = '%.6e' % -1.05                  # P12 / P11 = -1.05 at theta ~ 100 deg
+        out.append(' '.join(v))
+    with open(path, 'w') as fh:
+        fh.write('\n'.join(out) + '\n')
+    o, rec, st, cnt = c_oracle_case(walker, case, d, 400_000, 400_000, seed=650)
+    steps = int(np.sum(rec['n_scat']))
+    assert cnt[11] >= 0.9 * steps, (cnt[11], steps)   # every scattering angle by the two-area rejection
+    assert_same_law(o, rec, st, polarised=True)
+    assert_same_mean(o['n_scat'], rec['n_scat'], 'n_scat')

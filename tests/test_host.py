@@ -175,3 +175,37 @@ def test_pack_polar_layout():
     np.testing.assert_array_equal(tabs[1, 0], x)
     np.testing.assert_array_equal(tabs[1, 1:], Y1)
     assert pack_polar(cases[:1]) is None
+
+
+def test_spectral_albedo_table(tmp_path):
+    """spectral_albedo (pp:250-312) over record files named as run() names
+    them: grouped by radius then nominal wavelength, actual wavelength
+    mean(1 / wvn), albedo and non-absorbed fraction -- against numbers worked
+    out by hand for small files (wvn weights 1/1.0 and 1/1.1 etc.)."""
+    def rec(wvls, conds):
+        wvls = np.asarray(wvls, float)
+        n = wvls.size
+        return dict(condition=np.asarray(conds), wvn=1.0 / wvls, theta_n=np.full(n, 0.3), phi_n=np.full(n, 1.0),
+                    n_scat=np.arange(n), path_length=np.full(n, 0.01), snow_depth=np.full(n, 1.0))
+    files = {
+        # (wvl0, rds) -> (wavelengths, conditions)
+        (1.0, 100.0): ([1.0, 1.0, 1.1, 0.9], [1, 4, 2, 1]),
+        (1.3, 100.0): ([1.3, 1.3], [1, 3]),
+        (1.0, 50.0): ([1.0, 1.0, 1.0, 1.0, 1.0], [1, 1, 1, 4, 5]),
+    }
+    names = []
+    for (wvl0, rds), (wv, cd) in files.items():
+        name = '%s_0.01_%s_%d_59.99999999999999_HG.txt' % (wvl0, rds, len(wv))
+        MonteCarlo.write_records(str(tmp_path / name), rec(wv, cd))
+        names.append(name)
+    rows = post_processing.spectral_albedo(post_processing.group_files(names, str(tmp_path)))
+    assert [(r[0], r[1]) for r in rows] == [(50.0, 1.0), (100.0, 1.0), (100.0, 1.3)]
+    # r = 50: five photons at 1.0 um, three reflected; nothing leaves through the bottom
+    assert rows[0][2:] == (1.0, 0.6, 0.6)
+    # r = 100, 1.0 um: wvn 1, 1, 1/1.1, 1/0.9; reflected 1 + 1/0.9; bottom diffuse 1/1.1
+    q = 1 + 1 + 1 / 1.1 + 1 / 0.9
+    np.testing.assert_allclose(rows[1][2], (1.0 + 1.0 + 1.1 + 0.9) / 4, rtol=1e-15)
+    np.testing.assert_allclose(rows[1][3], (1 + 1 / 0.9) / q, rtol=1e-14)
+    np.testing.assert_allclose(rows[1][4], (1 + 1 / 0.9 + 1 / 1.1) / q, rtol=1e-14)
+    # r = 100, 1.3 um: one reflected, one direct transmission (condition 3 counts as non-absorbed)
+    np.testing.assert_allclose(rows[2][2:], (1.3, 0.5, 1.0), rtol=1e-15)
