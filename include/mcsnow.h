@@ -187,6 +187,37 @@ int mcs_sync(mcs_ctx *ctx);
 /* replaces: comm.gather + flatten_list (parallelize.py:17-26, mc3D:1618) */
 int mcs_read_records(mcs_ctx *ctx, const mcs_records *out, int64_t n_photons);
 
+/* Caller-owned DEVICE arrays that mcs_walk_noalloc writes (photon order). */
+typedef struct {
+    double *tallies;           /* n_cases x MCS_TALLY_F64 (mcs_read_tallies)  */
+    uint64_t *counts;          /* n_cases x MCS_TALLY_U64 (mcs_read_counts)   */
+    /* records (want_records = 1; all five required), sum(n_photons) each:   */
+    uint32_t *n_scat;
+    uint32_t *cond_band;       /* condition | band << 8                       */
+    float *theta_n, *phi_n, *path_length;
+    float *stokes;             /* aspherical cases: Q, U, V planes of
+                                  sum(n_photons) each (NULL: not written)     */
+} mcs_dev_out;
+
+#define MCS_NOALLOC_MAX_CASES 65536
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc -- the native walk of mcs_walk with every buffer
+ * supplied: inputs are the tables of mcs_set_cases / mcs_set_polar, outputs
+ * the caller's device arrays in `out`; the walk's scratch (control words,
+ * work-queue counter, chunk table: 256.1 KiB) is a static device array of
+ * the library, one per device, kept until the process ends.  Allocates
+ * nothing (no hipMalloc, hipFree or allocating library call), copies nothing
+ * between host and device.  args: mode MCS_NATIVE, photon_band, replay_*
+ * NULL, use_graph 0 (MCS_ERR_ARG otherwise); at most MCS_NOALLOC_MAX_CASES
+ * cases.  Records go straight to photon order (no compaction step).  Calls
+ * on one device share its scratch, so two calls that overlap on one device
+ * (two contexts, two host threads) race; sequential calls are safe (each
+ * clears what it uses).  Afterwards mcs_read_tallies / mcs_read_counts /
+ * mcs_read_records report MCS_ERR_STATE (the results are in `out`). */
+int mcs_walk_noalloc(mcs_ctx *ctx, const mcs_walk_args *args, const mcs_dev_out *out);
+
 /* Aspherical ice grains (Yang et al. 2013 tables, shape != 'sphere' and not
  * HG): the scattering phase matrix replaces Henyey-Greenstein and every
  * photon carries a Stokes vector.  One entry per case set by mcs_set_cases. */

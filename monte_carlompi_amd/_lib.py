@@ -26,7 +26,7 @@ MAX_BANDS = 256
 EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs_set_cases',
            'mcs_set_launch', 'mcs_get_launch', 'mcs_walk', 'mcs_sync', 'mcs_read_records',
            'mcs_read_tallies', 'mcs_read_counts', 'mcs_last_stats', 'mcs_autotune', 'mcs_philox',
-           'mcs_set_polar', 'mcs_read_stokes',
+           'mcs_set_polar', 'mcs_read_stokes', 'mcs_walk_noalloc',
            'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_last_error', 'mcs_mlp_param_count',
            'mcs_mlp_set_params', 'mcs_mlp_get_params', 'mcs_mlp_forward', 'mcs_mlp_last_ms',
            'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps']
@@ -80,6 +80,15 @@ class McsPolarCase(ctypes.Structure):
 class McsRecords(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in
                 ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length')]
+
+
+class McsDevOut(ctypes.Structure):
+    """Device pointers of mcs_walk_noalloc's outputs (mcs_dev_out)."""
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ('tallies', 'counts', 'n_scat', 'cond_band', 'theta_n', 'phi_n', 'path_length', 'stokes')]
+
+
+NOALLOC_MAX_CASES = 65536
 
 
 class McsStats(ctypes.Structure):
@@ -138,6 +147,7 @@ def lib():
         'mcs_philox': ([P, P, ctypes.c_uint32, ctypes.c_uint32, P, ctypes.c_int64], I),
         'mcs_set_polar': ([P, P, ctypes.c_int32, P, ctypes.c_int32, ctypes.c_int32], I),
         'mcs_read_stokes': ([P, P, P, P, ctypes.c_int64], I),
+        'mcs_walk_noalloc': ([P, ctypes.POINTER(McsWalkArgs), ctypes.POINTER(McsDevOut)], I),
         'mcs_mlp_create': ([I, ctypes.POINTER(McsMlpDims), ctypes.POINTER(P)], I),
         'mcs_mlp_destroy': ([P], I),
         'mcs_mlp_last_error': ([P], ctypes.c_char_p),

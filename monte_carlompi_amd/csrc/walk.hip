@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -106,13 +107,17 @@ __device__ void flush_slot(BlockState &bs, int s, const KParams &P)
     if (lane == 0) __atomic_store_n(&bs.hdr[s].state, 0, __ATOMIC_RELAXED);
 }
 
-template <class W>
+// DIRECT: records go straight to photon order (pos = gid, no compaction
+// buffer, no id store) -- a separate kernel instance, so the usual one keeps
+// its registers
+template <class W, bool DIRECT>
 __device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint64_t gid,
                                              const W &w, typename W::Real th,
                                              typename W::Real ph, typename W::Real pa)
 {
     using R = typename W::Real;
-    P.rec_gid[pos] = (uint32_t)gid;
+    if constexpr (DIRECT) pos = gid;
+    else P.rec_gid[pos] = (uint32_t)gid;
     P.rec_nscat[pos] = w.i - 1;
     P.rec_cb[pos] = (uint32_t)w.cond | ((uint32_t)w.band << 8);
     ((R *)P.rec_theta)[pos] = th;
@@ -121,7 +126,7 @@ __device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint6
 }
 
 // Record + tally the DONE lanes of the wave; flush chunks that complete.
-template <class W>
+template <class W, bool DIRECT = false>
 __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, int &state,
                                        int my_slot, uint64_t my_gid)
 {
@@ -143,7 +148,8 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
             typename W::Real th, ph, pa;
             bool axis;
             w.result(P, H.case_id, th, ph, pa, axis);
-            if (P.rec_gid) write_record(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
+            if (DIRECT ? P.rec_nscat != nullptr : P.rec_gid != nullptr)
+                write_record<W, DIRECT>(P, (size_t)H.gid0 + base + lane_rank(same), my_gid, w, th, ph, pa);
             if constexpr (W::STOKES) {
                 if (P.rec_stokes) {   // photon order: Q, U, V planes of n_total
                     typename W::Real *st = (typename W::Real *)P.rec_stokes;
@@ -351,7 +357,7 @@ __device__ __forceinline__ void set_priority(uint32_t i)
 // instance that fits its shape).  The hand-over of cooperative-tail photons
 // between waves is left out of the 6-wave build: at 80 VGPRs its code would
 // spill in the step loop.
-template <class W, int WPE>
+template <class W, int WPE, bool DIRECT = false>
 __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_walk(const KParams P)
 {
@@ -392,7 +398,7 @@ k_walk(const KParams P)
 
     for (;;) {
         // ---------------- cold: retire finished lanes, refill empty ones
-        if (__ballot(state == DONE)) retire(bs, P, w, state, my_slot, my_gid);
+        if (__ballot(state == DONE)) retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
         const bool exhausted = lds_load(&bs.exhausted) != 0;
         if (!exhausted && __ballot(state == EMPTY)) refill(bs, P, w, state, my_slot, my_gid);
         const unsigned long long live = __ballot(state == LIVE);
@@ -413,7 +419,7 @@ k_walk(const KParams P)
                     __builtin_amdgcn_s_setprio(3);
                     coop_phase<false>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
                                       &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
-                    retire(bs, P, w, state, my_slot, my_gid);
+                    retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
                 }
                 break;
             }
@@ -426,7 +432,7 @@ k_walk(const KParams P)
                 __ballot(state == LIVE && !(w.pm >= 0 && w.pm_exact)) == 0ull) {
                 __builtin_amdgcn_s_setprio(3);
                 polar_tail(w, state, P, wave);
-                retire(bs, P, w, state, my_slot, my_gid);
+                retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
                 break;
             }
         }
@@ -460,7 +466,7 @@ k_walk(const KParams P)
                     coop_phase<true>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
                                      &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
                     // every photon the wave held is finished or handed over
-                    retire(bs, P, w, state, my_slot, my_gid);
+                    retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
                 }
                 if (!coop_receive(bs, P, w, state, my_slot, my_gid, coop_tab, wave)) break;
                 to_coop = true;
@@ -549,6 +555,33 @@ __global__ void k_clear(unsigned long long *a, uint64_t na, unsigned long long *
     }
 }
 
+// Chunk table of a walk on the device: chunk0[k] = sum over cases j < k of
+// ceil(n_photons_j / chunk), k = 0..n_cases (one block of 1024 threads; each
+// thread sums a contiguous run of cases, then a scan over the threads).
+__global__ void __launch_bounds__(1024) k_chunk_table(const DevCase *cases, int n_cases, int chunk, int32_t *chunk0)
+{
+    __shared__ int64_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n_cases + 1023) / 1024;
+    const int k0 = min(t * per, n_cases), k1 = min(k0 + per, n_cases);
+    int64_t s = 0;
+    for (int k = k0; k < k1; ++k) s += (cases[k].n_photons + chunk - 1) / chunk;
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {   // inclusive scan (Hillis-Steele)
+        const int64_t v = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int64_t run = t > 0 ? part[t - 1] : 0;   // chunks before this thread's first case
+    for (int k = k0; k < k1; ++k) {
+        chunk0[k] = (int32_t)run;
+        run += (cases[k].n_photons + chunk - 1) / chunk;
+    }
+    if (t == 1023) chunk0[n_cases] = (int32_t)part[1023];
+}
+
 // The walk's checks for the host: photons, photon-steps, capped, phi on an
 // axis (summed over cases), control words 0 and 1, and a written-flag, into
 // pinned host memory (one block of 256 threads).
@@ -575,6 +608,15 @@ __global__ void k_summary(const unsigned long long *tu, int nc, const unsigned l
         out[7] = 1ull;
     }
 }
+
+// Scratch of mcs_walk_noalloc, one copy per device for the life of the
+// process: 16 walk control words (ctl[0..7], the chunk counter at word 8)
+// then the chunk table (MCS_NOALLOC_MAX_CASES + 1 int32).  Host code uses
+// the address hipGetSymbolAddress gives for the current device (noalloc_base),
+// never the array's name.
+constexpr size_t NOALLOC_CTL_U64 = 16;
+constexpr size_t NOALLOC_SCRATCH_U64 = NOALLOC_CTL_U64 + (MCS_NOALLOC_MAX_CASES + 1 + 1) / 2;
+__device__ __align__(256) unsigned long long scratch[NOALLOC_SCRATCH_U64];
 
 __global__ void k_philox(const uint32_t *ctr, uint32_t k0, uint32_t k1, uint32_t *out, uint64_t n)
 {
@@ -631,7 +673,32 @@ struct mcs_ctx {
     bool chunk0_valid = false;
     // pinned host words written by k_summary (h_sum) and its device address
     unsigned long long *h_sum = nullptr, *d_sum = nullptr;
+    // the last walk's tallies and counts are in the context's buffers (false
+    // after mcs_walk_noalloc, which writes them to the caller's)
+    bool last_tallies_ctx = true;
+    // device address of the mcs_walk_noalloc scratch on this context's device
+    unsigned long long *noalloc_scratch = nullptr;
+    hipError_t noalloc_err = hipSuccess;
 };
+
+// Device address of `scratch` per device, looked up once per device (with
+// that device current) by the first mcs_create on it.
+static std::mutex g_scratch_mu;
+static unsigned long long *g_scratch_addr[64];
+
+static hipError_t noalloc_base(int device, unsigned long long **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (device < 0 || device >= 64) return hipErrorInvalidDevice;
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = (unsigned long long *)p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
 
 static int set_err(mcs_ctx *c, int code, const char *fmt, ...)
 {
@@ -769,6 +836,10 @@ int mcs_create(int device, mcs_ctx **out)
         (void)mcs_destroy(c);
         return MCS_ERR_HIP;
     }
+    // the no-alloc walk's scratch on this device (looked up once per device);
+    // a failed lookup only disables mcs_walk_noalloc
+    c->noalloc_err = noalloc_base(device, &c->noalloc_scratch);
+    if (c->noalloc_err != hipSuccess) c->noalloc_scratch = nullptr;
     *out = c;
     return MCS_OK;
 }
@@ -1067,9 +1138,8 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     const uint64_t nclear = (uint64_t)(TF + TU) * nc + 16;
     const unsigned blocks_clear = (unsigned)std::min<uint64_t>((nclear + 255) / 256, 1024);
     hipLaunchKernelGGL(k_clear, dim3(blocks_clear), dim3(256), 0, c->stream,
-                       (unsigned long long *)c->d_tally_f.p, (uint64_t)TF * nc,
-                       (unsigned long long *)c->d_tally_u.p, (uint64_t)TU * nc,
-                       (unsigned long long *)c->d_ctl.p, (uint64_t)16);
+                       (unsigned long long *)P.tally_f, (uint64_t)TF * nc, P.tally_u, (uint64_t)TU * nc,
+                       P.ctl, (uint64_t)16);
     int block = (mode == MCS_REPLAY) ? std::min(c->launch.block, ReplayWalker::MAX_BLOCK)
                                      : c->launch.block;
     int grid = c->launch.blocks_per_cu * c->cus;
@@ -1087,14 +1157,21 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
         // + the tail mode's owner lanes and packets (16 ints and 64 packets a wave)
         const size_t lds = (size_t)P.pm_lds_floats * sizeof(float) +
                            (size_t)(block / 64) * (16 * sizeof(int) + 64 * sizeof(PolarPacket));
-        hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
+        if (P.rec_direct)
+            hipLaunchKernelGGL((k_walk<PolarWalker, 3, true>), dim3(grid), dim3(block), lds, c->stream, P);
+        else
+            hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
     } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         // shapes asking for more than 5 waves per SIMD get the 80-VGPR instance
         // (6 waves per SIMD; its spills sit outside the step loop)
         const int waves_per_simd = (c->launch.blocks_per_cu * (block / 64) + 3) / 4;
-        if (waves_per_simd > 5)
+        if (waves_per_simd > 5 && P.rec_direct)
+            hipLaunchKernelGGL((k_walk<NativeWalker, 6, true>), dim3(grid), dim3(block), tab, c->stream, P);
+        else if (waves_per_simd > 5)
             hipLaunchKernelGGL((k_walk<NativeWalker, 6>), dim3(grid), dim3(block), tab, c->stream, P);
+        else if (P.rec_direct)
+            hipLaunchKernelGGL((k_walk<NativeWalker, 5, true>), dim3(grid), dim3(block), tab, c->stream, P);
         else
             hipLaunchKernelGGL((k_walk<NativeWalker, 5>), dim3(grid), dim3(block), tab, c->stream, P);
     } else {
@@ -1137,8 +1214,8 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                                (uint32_t *)c->d_out[0].p, (uint32_t *)c->d_out[1].p, (double *)c->d_out[2].p,
                                (double *)c->d_out[3].p, (double *)c->d_out[4].p);
     }
-    hipLaunchKernelGGL(k_summary, dim3(1), dim3(256), 0, c->stream, (const unsigned long long *)c->d_tally_u.p, nc,
-                       (const unsigned long long *)c->d_ctl.p, c->d_sum);
+    hipLaunchKernelGGL(k_summary, dim3(1), dim3(256), 0, c->stream, (const unsigned long long *)P.tally_u, nc,
+                       (const unsigned long long *)P.ctl, c->d_sum);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return set_err(c, MCS_ERR_HIP, "kernel launch: %s", hipGetErrorString(e));
     return MCS_OK;
@@ -1289,6 +1366,85 @@ int mcs_walk(mcs_ctx *c, const mcs_walk_args *a)
     c->last_mode = a->mode;
     c->last_records = records;
     c->last_stokes = records && c->polar;
+    c->last_tallies_ctx = true;
+    c->last_n = n;
+    c->pending = true;
+    if (a->async) return MCS_OK;
+    return finish_walk(c);
+}
+
+int mcs_walk_noalloc(mcs_ctx *c, const mcs_walk_args *a, const mcs_dev_out *o)
+{
+    if (!c || !a || !o) return MCS_ERR_ARG;
+    if (!c->cases_set) return set_err(c, MCS_ERR_STATE, "mcs_set_cases must come first");
+    if (a->mode != MCS_NATIVE) return set_err(c, MCS_ERR_ARG, "mcs_walk_noalloc walks in native mode only");
+    if (a->photon_band || a->replay_dev || a->replay_off || a->use_graph)
+        return set_err(c, MCS_ERR_ARG, "mcs_walk_noalloc takes no photon_band, replay input or graph launch");
+    if (a->max_steps < 0 || a->max_steps > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "max_steps outside [0, 2^32)");
+    const int nc = (int)c->cases.size();
+    if (nc > MCS_NOALLOC_MAX_CASES)
+        return set_err(c, MCS_ERR_ARG, "mcs_walk_noalloc: %d cases, at most %d (MCS_NOALLOC_MAX_CASES)", nc,
+                       MCS_NOALLOC_MAX_CASES);
+    if (!o->tallies || !o->counts) return set_err(c, MCS_ERR_ARG, "tallies and counts buffers are required");
+    if (!c->noalloc_scratch)
+        return set_err(c, MCS_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(c->noalloc_err));
+    const int64_t n = c->total_photons;
+    const bool records = a->want_records != 0;
+    if (records) {
+        if (!o->n_scat || !o->cond_band || !o->theta_n || !o->phi_n || !o->path_length)
+            return set_err(c, MCS_ERR_ARG, "want_records needs all five record arrays");
+        if (n > 0xFFFFFFFFll) return set_err(c, MCS_ERR_ARG, "records need fewer than 2^32 photons");
+    }
+    (void)hipSetDevice(c->device);
+    if (c->pending) {
+        int r = finish_walk(c);
+        if (r) return r;
+    }
+    const int chunk = c->launch.chunk;
+    int64_t nch = 0;
+    for (int k = 0; k < nc; ++k) nch += (c->dcases[k].n_photons + chunk - 1) / chunk;
+    if (nch > 0x7FFFFFFF) return set_err(c, MCS_ERR_ARG, "too many chunks; raise the chunk size");
+    c->last_records = c->last_stokes = false;
+    unsigned long long *const scr = c->noalloc_scratch;   // this device's scratch
+    int32_t *const chunk0 = reinterpret_cast<int32_t *>(scr + NOALLOC_CTL_U64);
+    KParams P;
+    memset(&P, 0, sizeof P);
+    P.cases = (const DevCase *)c->d_cases.p;
+    P.bands = (const DevBand *)c->d_bands.p;
+    P.case_chunk0 = chunk0;
+    P.n_cases = nc;
+    P.chunk = chunk;
+    P.n_chunks = (uint32_t)nch;
+    P.refill_min = (uint32_t)c->launch.refill_min;
+    P.coop_live = c->launch.coop_live;
+    P.key0 = (uint32_t)a->seed;
+    P.key1 = (uint32_t)(a->seed >> 32);
+    P.max_steps = a->max_steps ? (uint32_t)a->max_steps : (1u << 24);
+    P.tally_f = o->tallies;
+    P.tally_u = (unsigned long long *)o->counts;
+    P.ctl = scr;
+    P.chunk_counter = (unsigned int *)(scr + 8);
+    P.pm = (const double *)c->d_pm.p;
+    P.pmf = (const float *)c->d_pmf.p;
+    P.pm_n = c->pm_n;
+    P.pm_stride = c->pm_n * 14 + PM_GUIDE + 1;
+    P.pm_lds_floats = (c->polar && c->pm_floats * sizeof(float) <= 16 * 1024) ? (int32_t)c->pm_floats : 0;
+    P.n_total = (uint64_t)n;
+    if (records) {
+        P.rec_direct = 1;
+        P.rec_nscat = o->n_scat;
+        P.rec_cb = o->cond_band;
+        P.rec_theta = o->theta_n;
+        P.rec_phi = o->phi_n;
+        P.rec_path = o->path_length;
+        if (c->polar) P.rec_stokes = o->stokes;
+    }
+    c->h_sum[7] = 0ull;   // set again by this walk's k_summary
+    hipLaunchKernelGGL(k_chunk_table, dim3(1), dim3(1024), 0, c->stream, P.cases, nc, chunk, chunk0);
+    int r = enqueue_walk(c, MCS_NATIVE, P, (uint32_t)nch, n, false, true);
+    if (r) return r;
+    c->last_mode = MCS_NATIVE;
+    c->last_tallies_ctx = false;
     c->last_n = n;
     c->pending = true;
     if (a->async) return MCS_OK;
@@ -1396,6 +1552,7 @@ int mcs_read_tallies(mcs_ctx *c, double *out, int32_t n_cases)
 {
     if (!c || !out) return MCS_ERR_ARG;
     if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (!c->last_tallies_ctx) return set_err(c, MCS_ERR_STATE, "the last walk wrote its tallies to the caller's buffers");
     if (n_cases != (int32_t)c->cases.size()) return set_err(c, MCS_ERR_ARG, "n_cases mismatch");
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipMemcpy(out, c->d_tally_f.p, sizeof(double) * TF * n_cases, hipMemcpyDeviceToHost));
@@ -1406,6 +1563,7 @@ int mcs_read_counts(mcs_ctx *c, uint64_t *out, int32_t n_cases)
 {
     if (!c || !out) return MCS_ERR_ARG;
     if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
+    if (!c->last_tallies_ctx) return set_err(c, MCS_ERR_STATE, "the last walk wrote its counts to the caller's buffers");
     if (n_cases != (int32_t)c->cases.size()) return set_err(c, MCS_ERR_ARG, "n_cases mismatch");
     (void)hipSetDevice(c->device);
     HIPCHK(c, hipMemcpy(out, c->d_tally_u.p, sizeof(uint64_t) * TU * n_cases, hipMemcpyDeviceToHost));

@@ -113,7 +113,7 @@ struct KParams {
     int32_t pm_n;
     int32_t pm_lds_floats;       // > 0: the FP32 tables are copied into dynamic LDS at block start
     int32_t pm_stride;
-    int32_t pad1;
+    int32_t rec_direct;          // 1: records go straight to photon order (pos = gid), no compaction
     void *rec_stokes;
     uint64_t n_total;
 };

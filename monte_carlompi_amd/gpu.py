@@ -209,6 +209,32 @@ class GpuWalker:
         self._keep = keep
         return self
 
+    def walk_noalloc(self, tallies, counts, records=None, stokes=None, seed=0, max_steps=0, async_=False):
+        """The native walk into caller-owned device memory (mcs_walk_noalloc:
+        no device allocation, no host copies).  ``tallies`` (n_cases x 656
+        float64) and ``counts`` (n_cases x 16 uint64) are device addresses;
+        ``records`` an optional dict of device addresses n_scat, cond_band
+        (uint32) and theta_n, phi_n, path_length (float32), photon order;
+        ``stokes`` an optional device address of 3 x n float32 (Q, U, V
+        planes, aspherical cases).  Any object with ``data_ptr()`` (a torch
+        tensor on the walker's device) or an int address is accepted."""
+        def addr(x):
+            if x is None:
+                return None
+            return int(x.data_ptr()) if hasattr(x, 'data_ptr') else int(x)
+        a = McsWalkArgs()
+        a.mode = MCS_NATIVE
+        a.want_records = int(records is not None)
+        a.seed = int(seed) & 0xFFFFFFFFFFFFFFFF
+        a.max_steps = int(max_steps)
+        a.async_ = int(bool(async_))
+        rec = records or {}
+        o = _lib.McsDevOut(addr(tallies), addr(counts), addr(rec.get('n_scat')), addr(rec.get('cond_band')),
+                           addr(rec.get('theta_n')), addr(rec.get('phi_n')), addr(rec.get('path_length')),
+                           addr(stokes))
+        self._check(lib().mcs_walk_noalloc(self._ctx, ctypes.byref(a), ctypes.byref(o)))
+        return self
+
     def sync(self):
         self._check(lib().mcs_sync(self._ctx))
 

@@ -50,7 +50,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         pytest.skip('no C compiler')
     structs = {'mcs_case': _lib.McsCase, 'mcs_band': _lib.McsBand, 'mcs_walk_args': _lib.McsWalkArgs,
                'mcs_launch': _lib.McsLaunch, 'mcs_records': _lib.McsRecords, 'mcs_stats': _lib.McsStats,
-               'mcs_polar_case': _lib.McsPolarCase}
+               'mcs_polar_case': _lib.McsPolarCase, 'mcs_dev_out': _lib.McsDevOut}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mcsnow.h"', 'int main(void) {']
     for cname, cls in structs.items():
         lines.append('printf("%s %%zu\\n", sizeof(%s));' % (cname, cname))
@@ -69,6 +69,14 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
             assert int(got['%s.%s' % (cname, f)]) == getattr(cls, f).offset, (cname, f)
     assert ctypes.sizeof(_lib.McsRecords) == 48
     assert ctypes.sizeof(_lib.McsStats) == 40
+    assert ctypes.sizeof(_lib.McsDevOut) == 64
+
+
+def test_header_constants_match_the_bindings():
+    text = open(os.path.join(ROOT, 'include', 'mcsnow.h')).read()
+    assert int(re.search(r'#define MCS_NOALLOC_MAX_CASES (\d+)', text).group(1)) == _lib.NOALLOC_MAX_CASES
+    assert int(re.search(r'#define MCS_TALLY_F64 (\d+)', text).group(1)) == _lib.TALLY_F64
+    assert int(re.search(r'#define MCS_TALLY_U64 (\d+)', text).group(1)) == _lib.TALLY_U64
 
 
 def test_create_without_gpu_fails_cleanly():
