@@ -337,6 +337,25 @@ int mcs_mlp_get_params(mcs_mlp *mlp, float *params, int64_t n);
 int mcs_mlp_forward(mcs_mlp *mlp, const float *x, int64_t n, float *y);
 int mcs_mlp_last_ms(const mcs_mlp *mlp, double *forward_ms);
 
+/* Serving on device memory.  A feature map turns raw inputs into the
+ * network's inputs: MCS_FEATURES_IDENTITY (inputs are the n_in features) or
+ * MCS_FEATURES_SNOW_BRF (rows of 5 raw case inputs r_eff [um], omega, g,
+ * theta0 [deg], tau -> 9 features: log r_eff, log10(1 - omega), g,
+ * cos theta0, log10 tau, exp(-tau / mu0), sqrt((1 - omega) / (1 - omega g)),
+ * log10(tau (1 - omega g)), log10(tau / mu0), each standardised as
+ * (f - feat_mu) / feat_sd; needs n_in = 9).  The outputs pass through
+ * out_k = exp(out_scale_k y_k + out_shift_k) - out_offset_k (targets learned
+ * in log space), fused into the last GEMM.  No reference counterpart: the
+ * network answers cases the walk would otherwise simulate. */
+#define MCS_FEATURES_IDENTITY 0
+#define MCS_FEATURES_SNOW_BRF 1
+int mcs_mlp_set_serving(mcs_mlp *mlp, int32_t feature_map, const float *feat_mu, const float *feat_sd,
+                        const float *out_scale, const float *out_shift, const float *out_offset);
+/* y [n x n_out] on the device from raw inputs x [n x 5 or n_in] on the
+ * device (caller-owned device memory; returns when the results are written).
+ * mcs_mlp_last_ms then gives the device time of the whole call. */
+int mcs_mlp_serve(mcs_mlp *mlp, const float *d_x, int64_t n, float *d_y);
+
 /* Loss = sum over rows and outputs of w_o (y - t)^2 / n (w: n_out weights).
  * Gradients of that loss for one batch (n <= max_batch, multiple of 64),
  * flat layout above; no update.  */

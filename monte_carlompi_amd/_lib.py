@@ -29,7 +29,10 @@ EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs
            'mcs_set_polar', 'mcs_read_stokes', 'mcs_walk_noalloc',
            'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_last_error', 'mcs_mlp_param_count',
            'mcs_mlp_set_params', 'mcs_mlp_get_params', 'mcs_mlp_forward', 'mcs_mlp_last_ms',
-           'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps']
+           'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps', 'mcs_mlp_set_serving',
+           'mcs_mlp_serve']
+MCS_FEATURES_IDENTITY = 0
+MCS_FEATURES_SNOW_BRF = 1
 
 
 class McsCase(ctypes.Structure):
@@ -125,6 +128,15 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError('%s is not built: run `python -m monte_carlompi_amd.build` '
                            '(there is no CPU fallback)' % LIB_PATH)
+    # One HIP runtime per process.  torch carries its own libamdhip64 /
+    # libhsa-runtime64, which its libraries load by file name: if this library
+    # bound /opt/rocm's copies first, importing torch later would load a second
+    # runtime that cannot open the GPU.  Loading torch first makes this
+    # library bind to torch's copies (same sonames).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     P = ctypes.c_void_p
     I = ctypes.c_int
@@ -159,6 +171,8 @@ def lib():
         'mcs_mlp_grads': ([P, P, P, P, ctypes.c_int32, P, ctypes.POINTER(ctypes.c_float)], I),
         'mcs_mlp_set_dataset': ([P, P, P, ctypes.c_int64], I),
         'mcs_mlp_train_steps': ([P, P, ctypes.c_int32, ctypes.c_int32, P, ctypes.POINTER(McsAdam), P], I),
+        'mcs_mlp_set_serving': ([P, ctypes.c_int32, P, P, P, P, P], I),
+        'mcs_mlp_serve': ([P, P, ctypes.c_int64, P], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

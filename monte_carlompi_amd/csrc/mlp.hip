@@ -29,7 +29,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int BM = 64, BN = 64, BK = 32, LDSP = BK + 8;  // LDS row pitch (bf16)
 constexpr int K0 = 32;                                    // padded input width
 
-enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3, EPI_ADD_F32 = 4 };
+enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3, EPI_ADD_F32 = 4, EPI_EXP = 5 };
 
 __device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
 
@@ -38,6 +38,7 @@ struct GemmArgs {
     const bf16 *B;  // logical B [N x K] (C = A B^T); stored [N][ldb] or (B_T) [K][ldb]
     int M, N, K, lda, ldb;
     const float *bias;  // [N] or null
+    const float *oscale, *oshift, *ooff;  // EPI_EXP: C = exp(oscale v + oshift) - ooff per column
     float *C;           // EPI_F32: C [M][ldc]; EPI_SILU: Z [M][ldc]
     bf16 *Cb;           // EPI_SILU / EPI_SILU_NOZ: silu(z) [M][ldcb]; EPI_DSILU: dZ [M][ldcb]
     const float *Zin;   // EPI_DSILU: pre-activation [M][ldc]
@@ -145,7 +146,9 @@ __global__ void __launch_bounds__(256) k_gemm(const GemmArgs g)
 // accumulators; K staged through two LDS buffers 32 at a time, the next stage
 // prefetched into registers while the current one feeds the MFMAs.
 // K must be a multiple of 32 (padded layer widths are).  Epilogues: EPI_F32
-// (bias, FP32 store, columns n < N) or EPI_SILU_NOZ (bias + SiLU, BF16 store).
+// (bias, FP32 store, columns n < N), EPI_EXP (the same through the output
+// transform exp(scale v + shift) - offset of log-space targets) or
+// EPI_SILU_NOZ (bias + SiLU, BF16 store).
 constexpr int FM = 128, FN = 128, FP = BK + 8;   // tile and LDS row pitch (bf16)
 
 template <int EPI>
@@ -208,6 +211,8 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(const GemmArgs g)
         const int n = n0 + wn * 64 + j * 32 + (lane & 31);
         if (n >= g.N) continue;
         const float bias = g.bias ? g.bias[n] : 0.f;
+        float osc = 1.f, osh = 0.f, oof = 0.f;
+        if (EPI == EPI_EXP) { osc = g.oscale[n]; osh = g.oshift[n]; oof = g.ooff[n]; }
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -216,6 +221,7 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(const GemmArgs g)
                 if (m >= g.M) continue;
                 const float v = acc[i][j][r] + bias;
                 if (EPI == EPI_F32) g.C[(size_t)m * g.ldc + n] = v;
+                else if (EPI == EPI_EXP) g.C[(size_t)m * g.ldc + n] = expf(fmaf(osc, v, osh)) - oof;
                 else g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * sigm(v));
             }
         }
@@ -229,6 +235,34 @@ __global__ void k_cast_x(const float *x, int n_in, int rows, bf16 *X)
     if (i >= rows * K0) return;
     const int r = i / K0, c = i % K0;
     X[i] = (bf16)(c < n_in ? x[(size_t)r * n_in + c] : 0.f);
+}
+
+// Input features of the snowpack BRF surrogate from raw case inputs (r_eff
+// [um], omega, g, theta0 [deg], tau), rows of 5 floats, FP64 arithmetic:
+//   log r, log10(1 - omega), g, cos theta0, log10 tau, exp(-tau / mu0),
+//   sqrt((1 - omega) / (1 - omega g)), log10(tau (1 - omega g)), log10(tau / mu0)
+// each standardised as (f - mu_k) / sd_k, zero padded to K0 columns, bf16.
+constexpr int SNOW_FEATURES = 9;
+__global__ void k_features_snow(const float *in, int rows, const float *mu, const float *sd, bf16 *X)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const float *q = in + (size_t)r * 5;
+    const double reff = q[0], om = q[1], g = q[2], th = q[3], tau = q[4];
+    const double mu0 = cos(th * (3.141592653589793 / 180.0));
+    double f[SNOW_FEATURES];
+    f[0] = log(reff);
+    f[1] = log10(fmax(1.0 - om, 1e-12));
+    f[2] = g;
+    f[3] = mu0;
+    f[4] = log10(tau);
+    f[5] = exp(-tau / mu0);
+    f[6] = sqrt((1.0 - om) / (1.0 - om * g));
+    f[7] = log10(tau * (1.0 - om * g));
+    f[8] = log10(tau / mu0);
+    bf16 *row = X + (size_t)r * K0;
+    for (int k = 0; k < K0; ++k)
+        row[k] = (bf16)(k < SNOW_FEATURES ? (float)((f[k] - (double)mu[k]) / (double)sd[k]) : 0.f);
 }
 
 // minibatch gather from the resident dataset
@@ -338,6 +372,11 @@ struct mcs_mlp {
     float *xd = nullptr, *td = nullptr;
     int64_t n_data = 0;
     double last_ms = 0.0;
+    // serving (mcs_mlp_set_serving): input feature map, its standardisation
+    // (K0 floats each) and the output transform (n_out floats each)
+    int32_t feature_map = MCS_FEATURES_IDENTITY;
+    float *fmu = nullptr, *fsd = nullptr, *oscale = nullptr, *oshift = nullptr, *ooff = nullptr;
+    bool serving = false;
 };
 
 static int merr(mcs_mlp *c, int code, const char *fmt, ...)
@@ -399,7 +438,8 @@ static void gemm_fwd(mcs_mlp *c, const GemmArgs &g)
 }
 
 // inference of `rows` rows already in c->X; outputs [rows][n_out] into `y`
-static void infer_pass(mcs_mlp *c, int rows, float *y)
+// (through the output transform when `exp_out`)
+static void infer_pass(mcs_mlp *c, int rows, float *y, bool exp_out = false)
 {
     const int L = c->d.n_hidden_layers;
     const bf16 *in = c->X;
@@ -417,7 +457,12 @@ static void infer_pass(mcs_mlp *c, int rows, float *y)
     GemmArgs g = ga(in, ld_in, c->wb + lo.wb_off, lo.pcol, rows, c->d.n_out, lo.pcol);
     g.bias = c->p + lo.b_off;
     g.C = y; g.ldc = c->d.n_out;
-    gemm_fwd<EPI_F32>(c, g);
+    if (exp_out) {
+        g.oscale = c->oscale; g.oshift = c->oshift; g.ooff = c->ooff;
+        gemm_fwd<EPI_EXP>(c, g);
+    } else {
+        gemm_fwd<EPI_F32>(c, g);
+    }
 }
 
 // training forward of `rows` rows already in c->X, keeping Z for backward
@@ -579,7 +624,8 @@ int mcs_mlp_destroy(mcs_mlp *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->p, c->gr, c->m, c->v, c->mask, c->pb_index, c->wb, c->X, c->dY, c->dZa, c->dZb,
-                    c->Y, c->T, c->w, c->yout, c->xin, c->loss, c->idx, c->xd, c->td};
+                    c->Y, c->T, c->w, c->yout, c->xin, c->loss, c->idx, c->xd, c->td,
+                    c->fmu, c->fsd, c->oscale, c->oshift, c->ooff};
     for (void *b : bufs) if (b) (void)hipFree(b);
     for (float *z : c->Z) if (z) (void)hipFree(z);
     for (bf16 *a : c->A) if (a) (void)hipFree(a);
@@ -667,6 +713,76 @@ int mcs_mlp_forward(mcs_mlp *c, const float *x, int64_t n, float *y)
         total_ms += ms;
     }
     c->last_ms = total_ms;
+    return MCS_OK;
+}
+
+int mcs_mlp_set_serving(mcs_mlp *c, int32_t feature_map, const float *feat_mu, const float *feat_sd,
+                        const float *out_scale, const float *out_shift, const float *out_offset)
+{
+    if (!c || !out_scale || !out_shift || !out_offset) return MCS_ERR_ARG;
+    int nf = 0;
+    if (feature_map == MCS_FEATURES_IDENTITY) {
+        nf = c->d.n_in;
+    } else if (feature_map == MCS_FEATURES_SNOW_BRF) {
+        if (c->d.n_in != SNOW_FEATURES)
+            return merr(c, MCS_ERR_ARG, "MCS_FEATURES_SNOW_BRF needs n_in = %d", SNOW_FEATURES);
+        if (!feat_mu || !feat_sd) return merr(c, MCS_ERR_ARG, "feature standardisation required");
+        nf = SNOW_FEATURES;
+        for (int k = 0; k < nf; ++k)
+            if (!std::isfinite(feat_mu[k]) || !(feat_sd[k] > 0) || !std::isfinite(feat_sd[k]))
+                return merr(c, MCS_ERR_ARG, "feature %d: invalid standardisation", k);
+    } else {
+        return merr(c, MCS_ERR_ARG, "unknown feature map %d", feature_map);
+    }
+    const int no = c->d.n_out;
+    for (int k = 0; k < no; ++k)
+        if (!std::isfinite(out_scale[k]) || !std::isfinite(out_shift[k]) || !std::isfinite(out_offset[k]))
+            return merr(c, MCS_ERR_ARG, "output %d: non-finite transform", k);
+    (void)hipSetDevice(c->device);
+    c->serving = false;
+    if (!c->fmu) {
+        MCHK(c, dalloc(&c->fmu, K0));
+        MCHK(c, dalloc(&c->fsd, K0));
+        MCHK(c, dalloc(&c->oscale, no));
+        MCHK(c, dalloc(&c->oshift, no));
+        MCHK(c, dalloc(&c->ooff, no));
+    }
+    std::vector<float> mu(K0, 0.f), sd(K0, 1.f);
+    for (int k = 0; k < nf && feature_map != MCS_FEATURES_IDENTITY; ++k) { mu[k] = feat_mu[k]; sd[k] = feat_sd[k]; }
+    MCHK(c, hipMemcpy(c->fmu, mu.data(), K0 * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->fsd, sd.data(), K0 * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->oscale, out_scale, (size_t)no * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->oshift, out_shift, (size_t)no * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->ooff, out_offset, (size_t)no * 4, hipMemcpyHostToDevice));
+    c->feature_map = feature_map;
+    c->serving = true;
+    return MCS_OK;
+}
+
+int mcs_mlp_serve(mcs_mlp *c, const float *d_in, int64_t n, float *d_out)
+{
+    if (!c || n < 0 || (n > 0 && (!d_in || !d_out))) return MCS_ERR_ARG;
+    if (!c->serving) return merr(c, MCS_ERR_STATE, "mcs_mlp_set_serving must come first");
+    (void)hipSetDevice(c->device);
+    const int MB = c->d.max_batch, no = c->d.n_out;
+    const int ni = c->feature_map == MCS_FEATURES_SNOW_BRF ? 5 : c->d.n_in;
+    MCHK(c, hipEventRecord(c->e0, c->stream));
+    for (int64_t s = 0; s < n; s += MB) {
+        const int rows = (int)std::min<int64_t>(MB, n - s);
+        if (c->feature_map == MCS_FEATURES_SNOW_BRF)
+            hipLaunchKernelGGL(k_features_snow, dim3((rows + 255) / 256), dim3(256), 0, c->stream, d_in + s * ni, rows,
+                               c->fmu, c->fsd, c->X);
+        else
+            hipLaunchKernelGGL(k_cast_x, dim3((rows * K0 + 255) / 256), dim3(256), 0, c->stream, d_in + s * ni, ni, rows,
+                               c->X);
+        infer_pass(c, rows, d_out + s * no, true);
+    }
+    MCHK(c, hipEventRecord(c->e1, c->stream));
+    MCHK(c, hipGetLastError());
+    MCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    MCHK(c, hipEventElapsedTime(&ms, c->e0, c->e1));
+    c->last_ms = ms;
     return MCS_OK;
 }
 

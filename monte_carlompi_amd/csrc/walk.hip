@@ -29,6 +29,7 @@
 #include <cstring>
 #include <mutex>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "mcsnow.h"
@@ -676,6 +677,9 @@ struct mcs_ctx {
     // the last walk's tallies and counts are in the context's buffers (false
     // after mcs_walk_noalloc, which writes them to the caller's)
     bool last_tallies_ctx = true;
+    // pinned host staging of record read-back (grown on demand)
+    void *h_stage = nullptr;
+    size_t stage_cap = 0;
     // device address of the mcs_walk_noalloc scratch on this context's device
     unsigned long long *noalloc_scratch = nullptr;
     hipError_t noalloc_err = hipSuccess;
@@ -813,6 +817,36 @@ static uint32_t prob_bits(double p)
     return (uint32_t)llround(t);
 }
 
+// Pinned host staging of at least `bytes` (read-back copies run at full
+// PCIe rate into it, then host threads widen into the caller's arrays).
+static int stage(mcs_ctx *c, size_t bytes)
+{
+    if (c->stage_cap >= bytes) return MCS_OK;
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
+    c->h_stage = nullptr;
+    c->stage_cap = 0;
+    HIPCHK(c, hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault));
+    c->stage_cap = bytes;
+    return MCS_OK;
+}
+
+// Run f(j0, j1) over [0, n) on up to 16 host threads (slices of >= 64 Ki).
+template <class F>
+static void parallel_rows(int64_t n, F f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int64_t T = std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::min(hw, 16u), (n + 65535) / 65536}));
+    if (T == 1) { f((int64_t)0, n); return; }
+    std::vector<std::thread> th;
+    const int64_t per = (n + T - 1) / T;
+    for (int64_t t = 1; t < T; ++t) {
+        const int64_t a = std::min(n, t * per), b = std::min(n, a + per);
+        th.emplace_back([=] { f(a, b); });
+    }
+    f(0, std::min(n, per));
+    for (auto &x : th) x.join();
+}
+
 extern "C" {
 
 int mcs_create(int device, mcs_ctx **out)
@@ -859,6 +893,7 @@ int mcs_destroy(mcs_ctx *c)
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->h_sum) (void)hipHostFree(c->h_sum);
+    if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MCS_OK;
@@ -1502,27 +1537,40 @@ int mcs_read_records(mcs_ctx *c, const mcs_records *o, int64_t n)
     if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
     if (!c->last_records) return set_err(c, MCS_ERR_STATE, "last walk kept no records");
     if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
+    if (n == 0) return MCS_OK;
     (void)hipSetDevice(c->device);
-    std::vector<uint32_t> nscat(n), cb(n);
-    HIPCHK(c, hipMemcpy(nscat.data(), c->d_out[0].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(cb.data(), c->d_out[1].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
-    for (int64_t j = 0; j < n; ++j) {
-        if (o->n_scat) o->n_scat[j] = nscat[j];
-        if (o->condition) o->condition[j] = (int8_t)(cb[j] & 0xff);
-        if (o->band) o->band[j] = (uint8_t)(cb[j] >> 8);
-    }
+    const bool rep = c->last_mode == MCS_REPLAY;
+    const size_t rsz = rep ? 8 : 4;
+    // photon-order device arrays (n_scat, cond | band << 8, theta, phi, path)
+    // -> pinned staging in one stream, then widened on host threads
+    int r;
+    if ((r = stage(c, (size_t)n * (8 + 3 * rsz)))) return r;
+    uint32_t *s_ns = (uint32_t *)c->h_stage, *s_cb = s_ns + n;
+    char *s_f = (char *)(s_cb + n);
+    HIPCHK(c, hipMemcpyAsync(s_ns, c->d_out[0].p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(s_cb, c->d_out[1].p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     double *dst[3] = {o->theta_n, o->phi_n, o->path_length};
-    if (c->last_mode == MCS_REPLAY) {
-        for (int k = 0; k < 3; ++k)
-            if (dst[k]) HIPCHK(c, hipMemcpy(dst[k], c->d_out[2 + k].p, 8 * (size_t)n, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> tmp(n);
+    for (int k = 0; k < 3; ++k)
+        if (dst[k])
+            HIPCHK(c, hipMemcpyAsync(s_f + (size_t)k * n * rsz, c->d_out[2 + k].p, rsz * (size_t)n,
+                                     hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    parallel_rows(n, [&](int64_t a, int64_t b) {
+        for (int64_t j = a; j < b; ++j) {
+            if (o->n_scat) o->n_scat[j] = s_ns[j];
+            if (o->condition) o->condition[j] = (int8_t)(s_cb[j] & 0xff);
+            if (o->band) o->band[j] = (uint8_t)(s_cb[j] >> 8);
+        }
         for (int k = 0; k < 3; ++k) {
             if (!dst[k]) continue;
-            HIPCHK(c, hipMemcpy(tmp.data(), c->d_out[2 + k].p, 4 * (size_t)n, hipMemcpyDeviceToHost));
-            for (int64_t j = 0; j < n; ++j) dst[k][j] = tmp[j];
+            if (rep) {
+                memcpy(dst[k] + a, (const double *)(s_f + (size_t)k * n * rsz) + a, sizeof(double) * (size_t)(b - a));
+            } else {
+                const float *src = (const float *)(s_f + (size_t)k * n * rsz);
+                for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
+            }
         }
-    }
+    });
     return MCS_OK;
 }
 
@@ -1532,19 +1580,26 @@ int mcs_read_stokes(mcs_ctx *c, double *q, double *u, double *v, int64_t n)
     if (c->pending) return set_err(c, MCS_ERR_STATE, "walk still pending: call mcs_sync");
     if (!c->last_stokes) return set_err(c, MCS_ERR_STATE, "last walk kept no Stokes vectors (records off or no polar cases)");
     if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
+    if (n == 0) return MCS_OK;
     (void)hipSetDevice(c->device);
+    const bool rep = c->last_mode == MCS_REPLAY;
+    const size_t rsz = rep ? 8 : 4;
+    int r;
+    if ((r = stage(c, 3 * rsz * (size_t)n))) return r;
+    HIPCHK(c, hipMemcpyAsync(c->h_stage, c->d_stokes.p, 3 * rsz * (size_t)n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     double *dst[3] = {q, u, v};
-    if (c->last_mode == MCS_REPLAY) {
-        for (int k = 0; k < 3; ++k)
-            if (dst[k] && n) HIPCHK(c, hipMemcpy(dst[k], (double *)c->d_stokes.p + (size_t)k * n, 8 * (size_t)n, hipMemcpyDeviceToHost));
-    } else {
-        std::vector<float> tmp(n);
+    parallel_rows(n, [&](int64_t a, int64_t b) {
         for (int k = 0; k < 3; ++k) {
-            if (!dst[k] || !n) continue;
-            HIPCHK(c, hipMemcpy(tmp.data(), (float *)c->d_stokes.p + (size_t)k * n, 4 * (size_t)n, hipMemcpyDeviceToHost));
-            for (int64_t j = 0; j < n; ++j) dst[k][j] = tmp[j];
+            if (!dst[k]) continue;
+            if (rep) {
+                memcpy(dst[k] + a, (const double *)c->h_stage + (size_t)k * n + a, sizeof(double) * (size_t)(b - a));
+            } else {
+                const float *src = (const float *)c->h_stage + (size_t)k * n;
+                for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
+            }
         }
-    }
+    });
     return MCS_OK;
 }
 

@@ -239,26 +239,37 @@ class GpuWalker:
         self._check(lib().mcs_sync(self._ctx))
 
     # ------------------------------------------------------------------
-    def records(self):
+    RECORD_DTYPES = (('condition', np.int8), ('band', np.uint8), ('n_scat', np.int64), ('theta_n', np.float64),
+                     ('phi_n', np.float64), ('path_length', np.float64), ('wvn', np.float64),
+                     ('snow_depth', np.float64))
+
+    def records(self, out=None):
         """Per-photon records in photon order: the CSV columns of mc3D:1626-1636
-        plus the band index."""
+        plus the band index.  ``out``: a dict of arrays of those dtypes and
+        the photon count to fill (a consumer that reads walk after walk can
+        keep its buffers; new arrays otherwise)."""
         n = self.n_photons
-        out = dict(condition=np.empty(n, np.int8), band=np.empty(n, np.uint8),
-                   n_scat=np.empty(n, np.int64), theta_n=np.empty(n), phi_n=np.empty(n),
-                   path_length=np.empty(n))
+        if out is None:
+            out = {k: np.empty(n, t) for k, t in self.RECORD_DTYPES}
+        else:
+            for k, t in self.RECORD_DTYPES:
+                a = out[k]
+                if a.dtype != t or a.shape != (n,) or not a.flags.c_contiguous:
+                    raise ValueError('out[%r] must be a contiguous %s array of %d entries' % (k, np.dtype(t), n))
         R = McsRecords(*[out[k].ctypes.data for k in
                          ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length')])
         self._check(lib().mcs_read_records(self._ctx, ctypes.byref(R), n))
-        wvn = np.empty(n)
-        depth = np.empty(n)
+        wvn, depth = out['wvn'], out['snow_depth']
         base = 0
         for k, c in enumerate(self.cases):
             sl = slice(base, base + c.n_photons)
-            wvn[sl] = self._band_wvn[k][out['band'][sl]]
-            depth[sl] = self._band_depth[k][out['band'][sl]]
+            if self._band_wvn[k].size == 1:          # one band: constant columns
+                wvn[sl] = self._band_wvn[k][0]
+                depth[sl] = self._band_depth[k][0]
+            else:
+                np.take(self._band_wvn[k], out['band'][sl], out=wvn[sl])
+                np.take(self._band_depth[k], out['band'][sl], out=depth[sl])
             base += c.n_photons
-        out['wvn'] = wvn
-        out['snow_depth'] = depth
         return out
 
     def stokes(self):

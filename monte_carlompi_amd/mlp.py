@@ -84,6 +84,28 @@ class Mlp:
         self._check(lib().mcs_mlp_forward(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
         return y
 
+    def set_serving(self, feature_map, out_scale, out_shift, out_offset, feat_mu=None, feat_sd=None):
+        """Feature map (_lib.MCS_FEATURES_*) and output transform
+        exp(scale y + shift) - offset of mcs_mlp_serve."""
+        from ._lib import MCS_FEATURES_IDENTITY
+        no = self.dims[3]
+        arrs = [np.ascontiguousarray(a, dtype=np.float32).reshape(no) for a in (out_scale, out_shift, out_offset)]
+        fm = fs = None
+        if feature_map != MCS_FEATURES_IDENTITY:
+            fm = np.ascontiguousarray(feat_mu, dtype=np.float32)
+            fs = np.ascontiguousarray(feat_sd, dtype=np.float32)
+        self._serving = arrs + [fm, fs]
+        self._check(lib().mcs_mlp_set_serving(self._h, int(feature_map), None if fm is None else fm.ctypes.data,
+                                              None if fs is None else fs.ctypes.data, arrs[0].ctypes.data,
+                                              arrs[1].ctypes.data, arrs[2].ctypes.data))
+
+    def serve(self, x_dev, n, y_dev):
+        """mcs_mlp_serve on device memory: ``x_dev`` (n x 5 or n_in float32)
+        and ``y_dev`` (n x n_out float32) are device addresses or objects
+        with data_ptr()."""
+        addr = lambda t: int(t.data_ptr()) if hasattr(t, 'data_ptr') else int(t)
+        self._check(lib().mcs_mlp_serve(self._h, addr(x_dev), int(n), addr(y_dev)))
+
     def last_ms(self):
         v = ctypes.c_double()
         self._check(lib().mcs_mlp_last_ms(self._h, ctypes.byref(v)))

@@ -6,7 +6,12 @@ summaries of rocprofv3 CSV output (kernel stats, FETCH_SIZE / WRITE_SIZE).
         FETCH_SIZE / WRITE_SIZE report for a known number of bytes)
     python -m monte_carlompi_amd.profiling sweep [--repeats 2]
         the batched sweep (16,384 cases) alone, to profile it
+    python -m monte_carlompi_amd.profiling mlp [--repeats 3]
+        the surrogate network's kernels at their benchmark shapes: serving 1e6
+        queries (features, 4 x 256 SiLU GEMMs, output transform) and batch-4096
+        Adam steps (random weights and data: timing and counters only)
     python -m monte_carlompi_amd.profiling valu COUNTERS_CSV [--kernel k_walk] [--json OUT]
+        [--steps PHOTON_STEPS_PER_LAUNCH] [--launch SHAPE]
         VALU issue occupancy, dual issue and lane utilisation from SQ_INSTS_VALU,
         SQ_ACTIVE_INST_VALU(2), SQ_THREAD_CYCLES_VALU, GRBM_GUI_ACTIVE (medians
         over dispatches to OUT)
@@ -67,6 +72,59 @@ def run_polar(repeats=3, launch=(1024, 1, 2048, 16, 16), photons=1_000_000):
             st = w.stats()
             print('polar %d: %.2f ms, %d photons, %d photon-steps' % (k, st['kernel_ms'], st['photons'],
                                                                       st['steps']), flush=True)
+
+
+def run_mlp(repeats=3, queries=1_000_000, batch=4096, steps=100):
+    """The surrogate network's kernels at the bench shapes (a workload to
+    profile on its own): serving `queries` raw inputs on the device, then
+    `steps` Adam steps of batch `batch` on a resident random dataset."""
+    import torch
+    from .mlp import Mlp
+    from ._lib import MCS_FEATURES_SNOW_BRF
+    from . import surrogate as S
+    net = Mlp(S.N_FEAT, 256, 4, S.N_OUT, max_batch=65536)
+    net.init_params(seed=1)
+    # small output scale: the exp of the transform stays finite on random weights
+    net.set_serving(MCS_FEATURES_SNOW_BRF, np.full(S.N_OUT, 1e-3), np.zeros(S.N_OUT), np.zeros(S.N_OUT),
+                    S.FEAT_MU, S.FEAT_SD)
+    q = torch.from_numpy(S.sample_inputs(queries, 1).astype(np.float32)).cuda()
+    y = torch.empty((queries, S.N_OUT), dtype=torch.float32, device='cuda')
+    torch.cuda.synchronize()
+    for k in range(repeats):
+        net.serve(q, queries, y)
+        print('serve %d: %.3f ms for %d queries' % (k, net.last_ms(), queries), flush=True)
+    rng = np.random.default_rng(2)
+    n_data = 65536
+    net.set_dataset(rng.normal(size=(n_data, S.N_FEAT)).astype(np.float32),
+                    rng.normal(size=(n_data, S.N_OUT)).astype(np.float32))
+    w = np.full(S.N_OUT, 1.0 / S.N_OUT, np.float32)
+    import time
+    for k in range(repeats):
+        idx = rng.integers(0, n_data, steps * batch).astype(np.int32)
+        t0 = time.perf_counter()
+        net.train_steps(idx, batch, w, lr=1e-4)
+        print('train %d: %d steps in %.3f s' % (k, steps, time.perf_counter() - t0), flush=True)
+    net.close()
+
+
+def run_snow1m(repeats=6, launch=(1024, 1, 977, 16, 32), photons=1_000_000, json_out=None):
+    """The headline workload (bench.py: snow_1m, 1e6 photons, records on) with
+    a fixed launch shape `repeats` times; prints and optionally writes the
+    photon-steps per launch (for the per-step instruction counts)."""
+    from .gpu import GpuWalker
+    from .workloads import snow_1m_case
+    steps = []
+    with GpuWalker(0) as w:
+        w.set_cases([snow_1m_case(photons)])
+        w.set_launch(*launch)
+        for k in range(repeats):
+            w.walk(mode='native', seed=k, records=True)
+            st = w.stats()
+            steps.append(st['steps'])
+            print('snow1m %d: %.4f ms, %d photon-steps' % (k, st['kernel_ms'], st['steps']), flush=True)
+    if json_out:
+        with open(json_out, 'w') as fh:
+            json.dump(dict(photon_steps=float(np.mean(steps)), launch=list(launch), photons=photons), fh)
 
 
 def valu_metrics(path, kernel, simds=1024, n_xcd=8, clock_hz=2.4e9):
@@ -145,8 +203,13 @@ def main():
     sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
     po = sub.add_parser('polar'); po.add_argument('--repeats', type=int, default=3)
     po.add_argument('--photons', type=int, default=1_000_000)
+    ml = sub.add_parser('mlp'); ml.add_argument('--repeats', type=int, default=3)
+    sn = sub.add_parser('snow1m'); sn.add_argument('--repeats', type=int, default=6)
+    sn.add_argument('--launch', default='1024,1,977,16,32'); sn.add_argument('--json')
     vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
     vm.add_argument('--json', help='write the per-dispatch medians here')
+    vm.add_argument('--steps', type=float, help='photon-steps per launch of the profiled workload')
+    vm.add_argument('--launch', help='launch shape of the profiled workload')
     s = sub.add_parser('stats'); s.add_argument('csv')
     t = sub.add_parser('traffic')
     for k in ('kernel', 'fetch', 'write', 'cal_fetch', 'cal_write'):
@@ -160,6 +223,10 @@ def main():
         run_sweep(a.repeats)
     elif a.cmd == 'polar':
         run_polar(a.repeats, photons=a.photons)
+    elif a.cmd == 'mlp':
+        run_mlp(a.repeats)
+    elif a.cmd == 'snow1m':
+        run_snow1m(a.repeats, tuple(int(v) for v in a.launch.split(',')), json_out=a.json)
     elif a.cmd == 'valu':
         rows = valu_metrics(a.csv, a.kernel)
         for r in rows:
@@ -170,6 +237,11 @@ def main():
             med = {k: float(np.median([r[k] for r in rows if k in r])) for k in keys
                    if any(k in r for r in rows)}
             med.update(kernel=a.kernel, dispatches=len(rows))
+            if a.steps:
+                med['photon_steps'] = a.steps
+                med['valu_insts_per_step'] = med['SQ_INSTS_VALU'] / a.steps
+            if a.launch:
+                med['launch'] = [int(v) for v in a.launch.split(',')]
             with open(a.json, 'w') as fh:
                 json.dump(med, fh, indent=1)
     elif a.cmd == 'stats':

@@ -66,28 +66,30 @@ def sample_inputs(n, seed):
 
 
 N_FEAT = 9
+# standardisation (f - mu) / sd of the nine features (fixed, roughly the
+# query distribution's mean and spread); the device feature map
+# (MCS_FEATURES_SNOW_BRF) takes the same constants
+FEAT_MU = np.array([5.2, -2.0, 0.875, 0.7, 2.0, 0.05, 0.35, 1.1, 2.15])
+FEAT_SD = np.array([1.13, 0.75, 0.0144, 0.25, 1.15, 0.15, 0.17, 1.15, 1.15])
+
+
+def raw_features(inp):
+    """The nine network inputs before standardisation, all functions of
+    (r_eff, omega, g, theta0, tau): log r_eff, log10(1 - omega), g,
+    cos theta0, log10 tau, plus three classic radiative-transfer
+    combinations -- the direct-beam transmittance exp(-tau / mu0), the
+    similarity parameter sqrt((1 - omega) / (1 - omega g)), the scaled depth
+    log10(tau (1 - omega g)) -- and the slant depth log10(tau / mu0)."""
+    r, om, g, th, tau = (np.asarray(inp[:, k], np.float64) for k in range(5))
+    mu0 = np.cos(np.radians(th))
+    return np.stack([np.log(r), np.log10(np.maximum(1.0 - om, 1e-12)), g, mu0, np.log10(tau),
+                     np.exp(-tau / mu0), np.sqrt((1.0 - om) / (1.0 - om * g)),
+                     np.log10(tau * (1.0 - om * g)), np.log10(tau / mu0)], axis=1)
 
 
 def features(inp):
-    """Standardised network inputs, all functions of (r_eff, omega, g,
-    theta0, tau): log r_eff, log10(1 - omega), g, cos theta0, log10 tau, plus
-    three classic radiative-transfer combinations of them -- the direct-beam
-    transmittance exp(-tau / mu0), the similarity parameter
-    sqrt((1 - omega) / (1 - omega g)), the scaled depth log10(tau (1 - omega g))
-    and the slant depth log10(tau / mu0)."""
-    r, om, g, th, tau = (inp[:, k] for k in range(5))
-    mu0 = np.cos(np.radians(th))
-    f = np.empty((inp.shape[0], N_FEAT), np.float32)
-    f[:, 0] = (np.log(r) - 5.2) / 1.13
-    f[:, 1] = (np.log10(np.maximum(1.0 - om, 1e-12)) + 2.0) / 0.75
-    f[:, 2] = (g - 0.875) / 0.0144
-    f[:, 3] = (mu0 - 0.7) / 0.25
-    f[:, 4] = (np.log10(tau) - 2.0) / 1.15
-    f[:, 5] = (np.exp(-tau / mu0) - 0.05) / 0.15
-    f[:, 6] = (np.sqrt((1.0 - om) / (1.0 - om * g)) - 0.35) / 0.17
-    f[:, 7] = (np.log10(tau * (1.0 - om * g)) - 1.1) / 1.15
-    f[:, 8] = (np.log10(tau / mu0) - 2.15) / 1.15
-    return f
+    """Standardised network inputs (raw_features - FEAT_MU) / FEAT_SD."""
+    return ((raw_features(inp) - FEAT_MU) / FEAT_SD).astype(np.float32)
 
 
 # targets are learned in log space so that the loss follows relative error
@@ -222,8 +224,28 @@ class Surrogate:
         return self.history
 
     def predict(self, inp):
+        """Host path: features and output transform in numpy around
+        mcs_mlp_forward (host buffers)."""
         y = self.net.forward(features(inp))
         return from_log(y * self.y_std + self.y_mean)
+
+    def enable_serving(self):
+        """Device path: features (MCS_FEATURES_SNOW_BRF) and the output
+        transform exp(y_std y + y_mean) - OFFSETS run on the GPU."""
+        from ._lib import MCS_FEATURES_SNOW_BRF
+        self.net.set_serving(MCS_FEATURES_SNOW_BRF, self.y_std, self.y_mean, OFFSETS, FEAT_MU, FEAT_SD)
+
+    def predict_device(self, q_dev, out=None):
+        """Answers for queries already on the device (torch float32 tensor
+        n x 5: r_eff, omega, g, theta0, tau) into a device tensor n x 651
+        (allocated when ``out`` is None): one mcs_mlp_serve call."""
+        import torch
+        n = int(q_dev.shape[0])
+        if out is None:
+            out = torch.empty((n, N_OUT), dtype=torch.float32, device=q_dev.device)
+        torch.cuda.synchronize(q_dev.device)       # q_dev written on torch's stream
+        self.net.serve(q_dev, n, out)
+        return out
 
     def predict_ms(self):
         return self.net.last_ms()
@@ -239,28 +261,48 @@ class Surrogate:
 
 
 # ---------------------------------------------------------------- judging
-def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=400):
-    """Per-case albedo relative error and BRF error of the prediction against
-    a Monte Carlo run.  BRF: over bins holding >= min_bin_photons reflected
-    photons (after symmetrisation), the rms relative difference with the
-    audit walk's own binomial noise removed in quadrature, and the raw median."""
+# A BRF bin of a case fails when its relative difference from the 1e6-photon
+# walk exceeds the 3 % bar by more than Z_BIN standard errors of that walk's
+# own estimate (binomial: 1 / sqrt(photons in the bin)); Z_BIN = 4.3 makes the
+# chance that a prediction equal to the true BRF fails any of ~300 judged
+# bins through the walk's noise alone about 1 %.  Bins judged: those holding
+# at least MIN_BIN_PHOTONS reflected photons (two mirrored bins pooled).
+Z_BIN = 4.3
+MIN_BIN_PHOTONS = 100
+
+
+def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=MIN_BIN_PHOTONS, brf_bar=0.03, z_bin=Z_BIN):
+    """Per-case errors of predictions against Monte Carlo runs of `photons`
+    photons:
+      albedo          |A_pred - A_mc| / A_mc
+      brf_bins_fail   number of judged bins with |rel| > brf_bar + z_bin / sqrt(n_bin)
+      brf_bin_excess  max over judged bins of |rel| - z_bin / sqrt(n_bin) (the
+                      per-bin error beyond the walk's noise allowance)
+      brf_rms_excess  rms relative difference over judged bins with the walk's
+                      binomial noise removed in quadrature
+      brf_median      median |rel| over judged bins (walk noise included)."""
     alb_err = np.abs(pred[:, BRF_BINS] - mc[:, BRF_BINS]) / np.maximum(mc[:, BRF_BINS], 1e-12)
     H = symmetrise_brf(mc_tallies[:, :BRF_BINS]) / mc_tallies[:, gpumod.T_QDOWN][:, None, None]
     nbin = (H * photons * 2).reshape(-1, BRF_BINS)          # two mirrored bins pooled
     rel = (pred[:, :BRF_BINS] - mc[:, :BRF_BINS]) / np.where(mc[:, :BRF_BINS] > 0, mc[:, :BRF_BINS], 1)
-    noise2 = 1.0 / np.maximum(nbin, 1)
+    sig = 1.0 / np.sqrt(np.maximum(nbin, 1))
     ok = nbin >= min_bin_photons
-    rms_corr = np.empty(pred.shape[0])
-    med = np.empty(pred.shape[0])
-    for i in range(pred.shape[0]):
-        m = ok[i]
-        if not m.any():
-            rms_corr[i] = 0.0
-            med[i] = 0.0
-            continue
-        rms_corr[i] = np.sqrt(max(0.0, np.mean(rel[i, m] ** 2) - np.mean(noise2[i, m])))
-        med[i] = np.median(np.abs(rel[i, m]))
-    return dict(albedo=alb_err, brf_rms_excess=rms_corr, brf_median=med)
+    arel = np.abs(rel)
+    fails = ok & (arel > brf_bar + z_bin * sig)
+    excess = np.where(ok, arel - z_bin * sig, -np.inf).max(axis=1)
+    excess = np.where(np.isfinite(excess), excess, 0.0)
+    cnt = ok.sum(axis=1)
+    msq = np.where(ok, rel * rel, 0).sum(axis=1) / np.maximum(cnt, 1)
+    mns = np.where(ok, sig * sig, 0).sum(axis=1) / np.maximum(cnt, 1)
+    rms_corr = np.sqrt(np.maximum(0.0, msq - mns))
+    med = np.array([np.median(arel[i, ok[i]]) if cnt[i] else 0.0 for i in range(pred.shape[0])])
+    return dict(albedo=alb_err, brf_bins_fail=fails.sum(axis=1), brf_bin_excess=excess,
+                brf_rms_excess=rms_corr, brf_median=med, bins_judged=cnt)
+
+
+def case_fails(err, alb_bar=0.01):
+    """The fallback bars: albedo error > alb_bar or any BRF bin failing."""
+    return (err['albedo'] > alb_bar) | (err['brf_bins_fail'] > 0)
 
 
 def region_of(inp, n_tau=8, n_eps=6, n_mu=3):
@@ -274,22 +316,41 @@ def region_of(inp, n_tau=8, n_eps=6, n_mu=3):
     return (a * n_eps + b) * n_mu + c
 
 
+def _pct(x):
+    x = np.asarray(x, float)
+    return dict(median=float(np.median(x)), p95=float(np.quantile(x, 0.95)), max=float(x.max()))
+
+
 def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_photons=1_000_000,
           seed=0, alb_bar=0.01, brf_bar=0.03, region_fail_frac=0.05, fallback_photons=1_000_000,
-          run_fallback=True):
-    """Surrogate answers for every query, a seeded audit by the plain walk,
-    region-wise fallback.  Returns answers and a report."""
+          run_fallback=True, validate=0):
+    """Serve every query: the network on the GPU for all of them, a seeded
+    audit subset re-run by the plain walk (its answers replace the
+    network's), and the walk for every query of an input region where more
+    than `region_fail_frac` of the audited cases miss a bar (albedo error >
+    alb_bar, or a BRF bin beyond brf_bar plus the walk's noise allowance).
+    With `validate` > 0, that many further queries (not audited) are walked
+    at audit_photons after serving, and the served answers -- network or
+    fallback -- are judged against them by the same bars.
+    Returns (answers as a device tensor n x 651, report)."""
+    import torch
+    dev = torch.device('cuda', getattr(walker, 'device', 0))
+    n = queries.shape[0]
     t0 = time.perf_counter()
-    pred = sur.predict(queries)
+    q_dev = torch.from_numpy(np.ascontiguousarray(queries, dtype=np.float32)).to(dev)
+    answers = sur.predict_device(q_dev)
     t_pred = time.perf_counter() - t0
     pred_dev_s = sur.predict_ms() * 1e-3
     rng = np.random.Generator(np.random.PCG64(seed))
-    n = queries.shape[0]
     n_audit = max(1, int(round(audit_frac * n)))
-    aud = np.sort(rng.choice(n, n_audit, replace=False))
+    perm = rng.permutation(n)
+    aud = np.sort(perm[:n_audit])
+    t1 = time.perf_counter()
     mc, mct, aud_dev_s = run_mc(walker, queries[aud], audit_photons, seed=seed + 7)
-    err = case_errors(pred[aud], mc, mct, audit_photons)
-    fail = (err['albedo'] > alb_bar) | (err['brf_rms_excess'] > brf_bar)
+    aud_idx = torch.from_numpy(aud).to(dev)
+    pred_aud = answers.index_select(0, aud_idx).cpu().numpy()
+    err = case_errors(pred_aud, mc, mct, audit_photons, brf_bar=brf_bar)
+    fail = case_fails(err, alb_bar)
     reg_q = region_of(queries)
     reg_a = reg_q[aud]
     bad_regions = []
@@ -299,36 +360,86 @@ def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_pho
             bad_regions.append(int(r))
     fb = np.isin(reg_q, bad_regions)
     fb[aud] = False
-    answers = pred.copy()
-    answers[aud] = mc                               # audited queries keep the walk's answer
+    answers.index_copy_(0, aud_idx, torch.from_numpy(mc.astype(np.float32)).to(dev))   # audited: the walk's answer
     fb_dev_s = 0.0
     if run_fallback and fb.any():
-        fmc, _, fb_dev_s = run_mc(walker, queries[fb], fallback_photons, seed=seed + 99)
-        answers[fb] = fmc
+        fidx = np.nonzero(fb)[0]
+        fmc, _, fb_dev_s = run_mc(walker, queries[fidx], fallback_photons, seed=seed + 99)
+        answers.index_copy_(0, torch.from_numpy(fidx).to(dev), torch.from_numpy(fmc.astype(np.float32)).to(dev))
+    torch.cuda.synchronize(dev)
+    t_served = time.perf_counter() - t0
     dec = np.clip(np.log10(queries[aud, 4]).astype(int), 0, 3)
     diag = dict(fail_albedo=int(np.sum(err['albedo'] > alb_bar)),
-                fail_brf=int(np.sum(err['brf_rms_excess'] > brf_bar)),
+                fail_brf_bins=int(np.sum(err['brf_bins_fail'] > 0)),
                 fail_frac_by_tau_decade=[float(np.mean(fail[dec == d])) if np.any(dec == d) else None
-                                         for d in range(4)])
+                                         for d in range(4)],
+                bins_judged_median=float(np.median(err['bins_judged'])))
     rep = dict(n_queries=int(n), n_audit=int(n_audit), audit_fail_frac=float(np.mean(fail)),
                diagnostics=diag,
                fallback_regions=bad_regions, fallback_frac=float(np.mean(fb)),
+               fallback_cases=int(fb.sum()),
                predict_s_host=t_pred, predict_s_device=pred_dev_s, audit_s_device=aud_dev_s,
-               fallback_s_device=fb_dev_s,
-               albedo_err=dict(median=float(np.median(err['albedo'])),
-                               p95=float(np.quantile(err['albedo'], 0.95)), max=float(err['albedo'].max())),
-               brf_rms_excess=dict(median=float(np.median(err['brf_rms_excess'])),
-                                   p95=float(np.quantile(err['brf_rms_excess'], 0.95))),
-               brf_median_abs=dict(median=float(np.median(err['brf_median']))))
+               fallback_s_device=fb_dev_s, served_s_host=t_served, audit_walk_s_host=time.perf_counter() - t1,
+               albedo_err=_pct(err['albedo']), brf_bin_excess=_pct(err['brf_bin_excess']),
+               brf_rms_excess=_pct(err['brf_rms_excess']),
+               brf_median_abs=dict(median=float(np.median(err['brf_median']))),
+               bars=dict(albedo=alb_bar, brf_bin=brf_bar, z_bin=Z_BIN, min_bin_photons=MIN_BIN_PHOTONS,
+                         region_fail_frac=region_fail_frac))
     rep['cases_per_s_surrogate'] = n / max(pred_dev_s, 1e-12)
     rep['cases_per_s_served'] = n / max(pred_dev_s + aud_dev_s + fb_dev_s, 1e-12)
+    rep['cases_per_s_served_host'] = n / max(t_served, 1e-12)
+    rep['plain_mc_cases_per_s_batched'] = n_audit / max(aud_dev_s, 1e-12)
+    if validate:
+        val = np.sort(perm[n_audit:n_audit + int(validate)])
+        vmc, vmct, _ = run_mc(walker, queries[val], audit_photons, seed=seed + 555)
+        got = answers.index_select(0, torch.from_numpy(val).to(dev)).cpu().numpy()
+        verr = case_errors(got, vmc, vmct, audit_photons, brf_bar=brf_bar)
+        vfail = case_fails(verr, alb_bar)
+        rep['validation'] = dict(cases=int(val.size), served_by_walk=int(fb[val].sum()),
+                                 fail_frac=float(np.mean(vfail)),
+                                 fail_albedo_frac=float(np.mean(verr['albedo'] > alb_bar)),
+                                 fail_brf_frac=float(np.mean(verr['brf_bins_fail'] > 0)),
+                                 albedo_err=_pct(verr['albedo']), brf_bin_excess=_pct(verr['brf_bin_excess']),
+                                 brf_rms_excess=_pct(verr['brf_rms_excess']))
     return answers, rep
 
 
 def plain_mc_rate(walker, inp, photons=1_000_000, seed=5):
-    """Cases/s of the plain walk at `photons` per case (batched, device time)."""
+    """Cases/s of the plain walk at `photons` per case, all cases in one
+    batched launch (device time)."""
     _, _, dev_s = run_mc(walker, inp, photons, seed)
     return inp.shape[0] / dev_s
+
+
+def plain_mc_rate_per_case(walker, inp, photons=1_000_000, seed=5):
+    """Cases/s of the plain walk run the way the drop-in run() runs it: one
+    launch per case (device time of each walk kernel, summed)."""
+    dev_s = 0.0
+    for k, case in enumerate(mc_cases(inp, photons)):
+        walker.set_cases([case])
+        walker.walk(mode='native', seed=seed + k, records=False)
+        dev_s += walker.stats()['kernel_ms'] * 1e-3
+    return inp.shape[0] / dev_s
+
+
+def train(walker, n_cases=200_000, photons=100_000, epochs=200, seed=0, thin_frac=0.0, log_every=0):
+    """Config-4 training: walk-generated targets for n_cases sampled inputs
+    (plus a `thin_frac` share of extra thin layers), then Adam on the GPU.
+    Returns (surrogate, report)."""
+    t0 = time.perf_counter()
+    inp = sample_training_inputs(n_cases, seed, thin_frac=thin_frac)
+    tg, tl, gen_s = run_mc(walker, inp, photons, seed=seed * 1000 + 1)
+    var = target_variance(tl, tg, photons)
+    del tl
+    t_data = time.perf_counter() - t0
+    sur = Surrogate()
+    t1 = time.perf_counter()
+    sur.fit(inp, tg, epochs=epochs, log_every=log_every, seed=seed, var=var)
+    train_s = time.perf_counter() - t1
+    sur.enable_serving()
+    return sur, dict(train_cases=int(inp.shape[0]), train_photons=int(photons), data_walk_s_device=gen_s,
+                     data_s_host=t_data, train_s_host=train_s, epochs=int(epochs),
+                     adam_steps=int(epochs * (inp.shape[0] // 4096)), final_loss=float(sur.history[-1]))
 
 
 def main():
@@ -337,38 +448,32 @@ def main():
     ap.add_argument('--train-cases', type=int, default=200_000)
     ap.add_argument('--train-photons', type=int, default=100_000)
     ap.add_argument('--epochs', type=int, default=200)
-    ap.add_argument('--thin-frac', type=float, default=0.25)
+    ap.add_argument('--thin-frac', type=float, default=0.0)
     ap.add_argument('--queries', type=int, default=1_000_000)
     ap.add_argument('--audit-frac', type=float, default=0.01)
     ap.add_argument('--audit-photons', type=int, default=1_000_000)
-    ap.add_argument('--rate-cases', type=int, default=512)
+    ap.add_argument('--validate', type=int, default=2000)
+    ap.add_argument('--rate-cases', type=int, default=200)
     ap.add_argument('--seed', type=int, default=0)
     ap.add_argument('--save', type=str, default='')
     ap.add_argument('--report', type=str, default='')
     a = ap.parse_args()
     w = GpuWalker(0)
     w.set_launch(256, 4, 4096, 8)
-    t0 = time.perf_counter()
-    inp = sample_training_inputs(a.train_cases, a.seed, thin_frac=a.thin_frac)
-    tg, tl, gen_s = run_mc(w, inp, a.train_photons, seed=a.seed * 1000 + 1)
-    var = target_variance(tl, tg, a.train_photons)
-    del tl
-    print('data: %d cases x %d photons, walk %.1f s device (%.1f s wall)'
-          % (inp.shape[0], a.train_photons, gen_s, time.perf_counter() - t0), flush=True)
-    sur = Surrogate()
-    t1 = time.perf_counter()
-    sur.fit(inp, tg, epochs=a.epochs, log_every=max(1, a.epochs // 10), seed=a.seed, var=var)
-    train_s = time.perf_counter() - t1
+    sur, trep = train(w, a.train_cases, a.train_photons, a.epochs, a.seed, a.thin_frac,
+                      log_every=max(1, a.epochs // 10))
+    print('trained: %s' % json.dumps(trep), flush=True)
     if a.save:
         sur.save(a.save)
     q = sample_inputs(a.queries, a.seed + 12345)
-    _, rep = serve(sur, w, q, audit_frac=a.audit_frac, audit_photons=a.audit_photons, seed=a.seed + 3)
-    mc_rate = plain_mc_rate(w, sample_inputs(a.rate_cases, a.seed + 777), a.audit_photons)
-    rep.update(train_cases=int(inp.shape[0]), train_photons=a.train_photons, data_walk_s_device=gen_s,
-               train_s=train_s, epochs=a.epochs, final_loss=sur.history[-1],
-               plain_mc_cases_per_s=mc_rate,
-               gain_surrogate=rep['cases_per_s_surrogate'] / mc_rate,
-               gain_served=rep['cases_per_s_served'] / mc_rate)
+    _, rep = serve(sur, w, q, audit_frac=a.audit_frac, audit_photons=a.audit_photons, seed=a.seed + 3,
+                   validate=a.validate)
+    per_case = plain_mc_rate_per_case(w, sample_inputs(a.rate_cases, a.seed + 777), a.audit_photons)
+    rep.update(trep)
+    rep['plain_mc_cases_per_s_per_case'] = per_case
+    rep['gain_surrogate'] = rep['cases_per_s_surrogate'] / rep['plain_mc_cases_per_s_batched']
+    rep['gain_served_vs_batched_mc'] = rep['cases_per_s_served'] / rep['plain_mc_cases_per_s_batched']
+    rep['gain_served_vs_per_case_mc'] = rep['cases_per_s_served'] / per_case
     print(json.dumps(rep, indent=1), flush=True)
     if a.report:
         with open(a.report, 'w') as fh:

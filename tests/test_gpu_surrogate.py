@@ -25,6 +25,7 @@ def test_surrogate_learns_and_beats_plain_mc_rate(walker):
     sur = S.Surrogate(max_batch=8192)
     print('surrogate: fit', flush=True)
     sur.fit(inp, tg, epochs=60, batch=1024, lr=3e-3)
+    sur.enable_serving()
     assert sur.history[-1] < 0.5 * sur.history[0]
     test_inp = S.sample_inputs(128, seed=2)
     print('surrogate: audit walk', flush=True)
@@ -44,5 +45,15 @@ def test_surrogate_learns_and_beats_plain_mc_rate(walker):
     ans, rep = S.serve(sur, walker, q[:20_000], audit_frac=0.01, audit_photons=200_000, seed=4,
                        fallback_photons=20_000)
     assert rep['n_audit'] == 200 and 0.0 <= rep['fallback_frac'] <= 1.0
-    assert ans.shape == (20_000, S.N_OUT)
+    assert tuple(ans.shape) == (20_000, S.N_OUT)
+    # the device serving path (features and output transform on the GPU)
+    # gives the host path's answers: inputs rounded alike to bf16, outputs
+    # through expf instead of float64 exp
+    import torch
+    q2 = q[:100_000].astype(np.float32)            # the device path reads float32 inputs
+    host = sur.predict(q2.astype(np.float64))
+    dev = sur.predict_device(torch.from_numpy(q2).cuda()).cpu().numpy()
+    rel = np.abs(dev - host) / np.maximum(np.abs(host), 1e-3)
+    assert np.mean(np.all(rel < 1e-4, axis=1)) > 0.995, np.quantile(rel, [0.5, 0.999])
+    assert rel.max() < 0.05, rel.max()
     sur.net.close()
