@@ -62,6 +62,7 @@ def main():
     ap.add_argument('--no-polar', action='store_true', help='skip the aspherical (phase-matrix) block')
     ap.add_argument('--no-surrogate', action='store_true', help='skip the surrogate (configs 4/5) block')
     ap.add_argument('--no-e2e', action='store_true', help='skip the end-to-end MonteCarlo.run block')
+    ap.add_argument('--no-replay', action='store_true', help='skip the FP64 replay block')
     ap.add_argument('--launch', type=str, default=','.join(str(v) for v in PINNED_LAUNCH),
                     help='launch shape block,blocks_per_cu,chunk,refill_min[,coop_live] of the headline walk')
     ap.add_argument('--autotune', action='store_true',
@@ -115,6 +116,14 @@ def main():
         w.walk(mode='native', seed=500 + k, records=True)
         w.records(out=rec_buf)
     pcie_ms = (time.perf_counter() - t2) * 1e3 / 5
+    # the same case in the bit-reproducible mode (tail mode off: every record
+    # depends only on seed, case and photon), for reference
+    w.set_launch(*shape[:4], 0)
+    repro_ms = []
+    for k in range(3):
+        w.walk(mode='native', seed=700 + k, records=True)
+        repro_ms.append(w.stats()['kernel_ms'])
+    w.set_launch(*shape)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     photons_per_s = args.photons / (ms_per_step * 1e-3)
     avg_kern = sum(kern_ms) / len(kern_ms)
@@ -148,6 +157,7 @@ def main():
         'autotune_ms': ({str(tuple(c)): round(float(t), 4) for c, t in zip(snow_1m_candidates(args.photons, w.cus), tune_ms)}
                         if tune_ms is not None else None),
         'coop_steps_frac': round(coop_frac, 4),
+        'reproducible_mode_kernel_ms': round(sorted(repro_ms)[1], 4),
         'photons_handed_over': handed_over,
         'albedo': round(float(fr['albedo'][0]), 6),
         'roofline': {
@@ -170,6 +180,8 @@ def main():
     }
     if not args.no_e2e:
         out['end_to_end'] = e2e_block(w, args.photons)
+    if not args.no_replay:
+        out['replay'] = replay_block(w)
     if not args.no_sweep:
         out['sweep'] = sweep_block(w)
     if not args.no_polar:
@@ -277,6 +289,46 @@ def polar_block(w, no_cpu):
                       % (cc['photons'], cc['threads'], cc['walk_s'], cc['steps_per_s'])}
         out['gpu_over_cpu_c'] = round(out['photons_per_s'] / cc['photons_per_s'], 1)
     return out
+
+
+def replay_block(w, n=100_000):
+    """The FP64 replay walk (the parity path): n photons of the headline case
+    consuming the reference's uniform-deviate stream, recorded by the C
+    restatement (MT19937, oracle/mcoracle.c) outside the timed region; the
+    kernel time of the replay launch, with every record's condition and
+    n_scat checked against the stream's own walk."""
+    import numpy as np
+    from oracle import coracle
+    from oracle import walk as owalk
+    from monte_carlompi_amd.gpu import Band, Case
+    from monte_carlompi_amd.workloads import RHO_SNW, SNOW_1M, sphere_ext_cff_mss
+    p = SNOW_1M
+    ext = sphere_ext_cff_mss(p['r_eff'])
+    tau = p['depth_m'] * ext * RHO_SNW
+    ssp = lambda wv: (dict(ssa=np.full(wv.shape, p['ssa']), ext=np.full(wv.shape, ext), g=np.full(wv.shape, p['g'])),
+                      dict(ssa=np.zeros(wv.shape), ext=np.zeros(wv.shape)))
+    out, par = coracle.run_rank(7, n, p['wvl'], 1e-12, ssp, 0.0, tau, RHO_SNW, p['theta0'], True, 0.5, record=True)
+    vals, offs = out['stream']
+    case = Case(tau_tot=tau, theta0_deg=p['theta0'], n_photons=n, lambertian_bottom=True, r_lambertian=0.5,
+                bands=[Band(wvl=float(par['wvl'][0]), g=p['g'], ssa_ice=p['ssa'], ext_cff=ext * RHO_SNW)])
+    w.set_cases([case])
+    w.set_launch(256, 4, 512, 8, 0)
+    pband = np.zeros(n, np.uint8)
+    w.walk(mode='replay', replay=(vals, offs), photon_band=pband, records=True)    # warm-up (uploads)
+    times = []
+    for k in range(3):
+        w.walk(mode='replay', replay=(vals, offs), photon_band=pband, records=True)
+        times.append(w.stats()['kernel_ms'])
+    rec = w.records()
+    same = bool(np.array_equal(rec['condition'], out['condition']) and np.array_equal(rec['n_scat'], out['n_scat']))
+    ms = float(np.median(times))
+    dev = float(offs[-1])
+    return {'workload': '%d photons of the headline case, FP64 walk consuming the reference\'s deviate stream '
+                        '(recorded by oracle/mcoracle.c, MT19937 seed 7)' % n,
+            'photons': n, 'deviates': int(dev), 'kernel_ms': round(ms, 4),
+            'photons_per_s': round(n / (ms * 1e-3), 1), 'photon_steps_per_s': round(out['steps'] / (ms * 1e-3), 1),
+            'deviate_gbs': round(dev * 8 / (ms * 1e-3) / 1e9, 2),
+            'records_equal_to_stream_walk': same}
 
 
 def e2e_block(w, n):

@@ -271,21 +271,46 @@ Z_BIN = 4.3
 MIN_BIN_PHOTONS = 100
 
 
-def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=MIN_BIN_PHOTONS, brf_bar=0.03, z_bin=Z_BIN):
+def bin_weights():
+    """w_jk of brf_from_tallies per 18 x 36 bin: BRF_jk = H_jk / (Q_down w_jk)."""
+    edges = np.linspace(0.0, np.pi / 2, THETA_BINS + 1)
+    mid = (np.diff(edges) / 2.) + edges[:-1]
+    w = np.sin(mid) * np.cos(mid)
+    return np.repeat(w / np.sum(w) / PHI_BINS, PHI_BINS)
+
+
+def case_errors(pred, mc, photons, min_bin_photons=MIN_BIN_PHOTONS, brf_bar=0.03, z_bin=Z_BIN,
+                pred_photons=None):
     """Per-case errors of predictions against Monte Carlo runs of `photons`
-    photons:
+    photons (one wavelength).  A BRF bin's relative difference is taken
+    against the prediction, rel = (mc - pred) / pred, and its binomial
+    standard error from the count the prediction expects there, n = pred x 2
+    w_jk x photons (two mirrored bins pooled): both unbiased for an exact
+    prediction, unlike scaling by the walk's own noisy count.  Bins expecting
+    >= min_bin_photons photons are judged.
       albedo          |A_pred - A_mc| / A_mc
-      brf_bins_fail   number of judged bins with |rel| > brf_bar + z_bin / sqrt(n_bin)
-      brf_bin_excess  max over judged bins of |rel| - z_bin / sqrt(n_bin) (the
-                      per-bin error beyond the walk's noise allowance)
-      brf_rms_excess  rms relative difference over judged bins with the walk's
-                      binomial noise removed in quadrature
-      brf_median      median |rel| over judged bins (walk noise included)."""
+      brf_bins_fail   judged bins with |rel| > brf_bar + z_bin / sqrt(n)
+      brf_bin_excess  max over judged bins of |rel| - z_bin / sqrt(n)
+      brf_rms_excess  rms of rel over judged bins, walk noise removed in quadrature
+      brf_median      median |rel| over judged bins (walk noise included).
+    ``pred_photons`` (per case, 0 for network answers): where a prediction is
+    itself a walk's answer (a fallback), its own binomial noise is added in
+    quadrature, and its albedo gets 3 standard errors of the difference of
+    the two walks beside the 1 % -- otherwise two exact walks of 1e6 photons
+    would fail each other."""
     alb_err = np.abs(pred[:, BRF_BINS] - mc[:, BRF_BINS]) / np.maximum(mc[:, BRF_BINS], 1e-12)
-    H = symmetrise_brf(mc_tallies[:, :BRF_BINS]) / mc_tallies[:, gpumod.T_QDOWN][:, None, None]
-    nbin = (H * photons * 2).reshape(-1, BRF_BINS)          # two mirrored bins pooled
-    rel = (pred[:, :BRF_BINS] - mc[:, :BRF_BINS]) / np.where(mc[:, :BRF_BINS] > 0, mc[:, :BRF_BINS], 1)
-    sig = 1.0 / np.sqrt(np.maximum(nbin, 1))
+    pb = pred[:, :BRF_BINS]
+    nbin = np.maximum(pb, 0.0) * (2.0 * photons) * bin_weights()[None, :]
+    rel = (mc[:, :BRF_BINS] - pb) / np.where(pb > 0, pb, 1.0)
+    sig = 1.0 / np.sqrt(np.maximum(nbin, 1.0))
+    if pred_photons is not None:
+        pp = np.asarray(pred_photons, float)
+        walked = pp > 0
+        ratio = np.where(walked, photons / np.maximum(pp, 1.0), 0.0)      # noise of the prediction / of the judge
+        sig = sig * np.sqrt(1.0 + ratio)[:, None]
+        a = np.clip(mc[:, BRF_BINS], 1e-12, 1.0)
+        sa = np.sqrt(a * (1 - a) * (1.0 / photons + np.where(walked, 1.0 / np.maximum(pp, 1.0), 0.0))) / a
+        alb_err = np.where(walked, np.maximum(alb_err - 3.0 * sa, 0.0), alb_err)
     ok = nbin >= min_bin_photons
     arel = np.abs(rel)
     fails = ok & (arel > brf_bar + z_bin * sig)
@@ -297,7 +322,32 @@ def case_errors(pred, mc, mc_tallies, photons, min_bin_photons=MIN_BIN_PHOTONS, 
     rms_corr = np.sqrt(np.maximum(0.0, msq - mns))
     med = np.array([np.median(arel[i, ok[i]]) if cnt[i] else 0.0 for i in range(pred.shape[0])])
     return dict(albedo=alb_err, brf_bins_fail=fails.sum(axis=1), brf_bin_excess=excess,
-                brf_rms_excess=rms_corr, brf_median=med, bins_judged=cnt)
+                brf_rms_excess=rms_corr, brf_median=med, bins_judged=cnt,
+                rel=rel.astype(np.float32), sig2=(sig * sig).astype(np.float32), ok=ok)
+
+
+def per_bin_rms_excess(err, cases=None, min_cases=20, z=4.0, bar=0.03):
+    """Per BRF bin, over a set of cases: the mean squared relative error with
+    the judging walks' noise removed, m_k = mean(rel^2) - mean(sigma^2), from
+    the cases where the bin is judged.  Returns (rms, over): rms_k =
+    sqrt(max(m_k, 0)) (NaN for bins judged in fewer than `min_cases` cases),
+    and over_k = m_k - z SE_k > bar^2 with SE_k the standard error of m_k
+    over those cases -- the bin's rms error exceeds `bar` with z-sigma
+    confidence.  Averaging over cases resolves per-bin errors well below one
+    walk's per-bin noise (1-3 % at 1e6 photons)."""
+    rel, s2, ok = err['rel'], err['sig2'], err['ok']
+    if cases is not None:
+        rel, s2, ok = rel[cases], s2[cases], ok[cases]
+    n = ok.sum(axis=0)
+    d = np.where(ok, rel.astype(np.float64) ** 2 - s2.astype(np.float64), 0.0)
+    nn = np.maximum(n, 1)
+    m = d.sum(axis=0) / nn
+    var = np.where(ok, (d - m[None, :]) ** 2, 0.0).sum(axis=0) / np.maximum(n - 1, 1)
+    se = np.sqrt(var / nn)
+    enough = n >= min_cases
+    rms = np.where(enough, np.sqrt(np.maximum(m, 0.0)), np.nan)
+    over = enough & (m - z * se > bar * bar)
+    return rms, over
 
 
 def case_fails(err, alb_bar=0.01):
@@ -349,14 +399,20 @@ def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_pho
     mc, mct, aud_dev_s = run_mc(walker, queries[aud], audit_photons, seed=seed + 7)
     aud_idx = torch.from_numpy(aud).to(dev)
     pred_aud = answers.index_select(0, aud_idx).cpu().numpy()
-    err = case_errors(pred_aud, mc, mct, audit_photons, brf_bar=brf_bar)
+    err = case_errors(pred_aud, mc, audit_photons, brf_bar=brf_bar)
     fail = case_fails(err, alb_bar)
+    # a region falls back when more than region_fail_frac of its audited cases
+    # miss a bar, or when some BRF bin's rms error over its audited cases
+    # (walk noise removed) exceeds the 3 % bar with 4-sigma confidence
     reg_q = region_of(queries)
     reg_a = reg_q[aud]
     bad_regions = []
+    region_bin_rms = {}
     for r in np.unique(reg_a):
         m = reg_a == r
-        if np.mean(fail[m]) > region_fail_frac:
+        rb, over = per_bin_rms_excess(err, m, bar=brf_bar)
+        region_bin_rms[int(r)] = int(over.sum())
+        if np.mean(fail[m]) > region_fail_frac or over.any():
             bad_regions.append(int(r))
     fb = np.isin(reg_q, bad_regions)
     fb[aud] = False
@@ -383,6 +439,8 @@ def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_pho
                albedo_err=_pct(err['albedo']), brf_bin_excess=_pct(err['brf_bin_excess']),
                brf_rms_excess=_pct(err['brf_rms_excess']),
                brf_median_abs=dict(median=float(np.median(err['brf_median']))),
+               audit_bins_over_bar=int(per_bin_rms_excess(err, bar=brf_bar)[1].sum()),
+               region_bins_over_bar={k: v for k, v in region_bin_rms.items() if v},
                bars=dict(albedo=alb_bar, brf_bin=brf_bar, z_bin=Z_BIN, min_bin_photons=MIN_BIN_PHOTONS,
                          region_fail_frac=region_fail_frac))
     rep['cases_per_s_surrogate'] = n / max(pred_dev_s, 1e-12)
@@ -393,10 +451,22 @@ def serve(sur: Surrogate, walker: GpuWalker, queries, audit_frac=0.01, audit_pho
         val = np.sort(perm[n_audit:n_audit + int(validate)])
         vmc, vmct, _ = run_mc(walker, queries[val], audit_photons, seed=seed + 555)
         got = answers.index_select(0, torch.from_numpy(val).to(dev)).cpu().numpy()
-        verr = case_errors(got, vmc, vmct, audit_photons, brf_bar=brf_bar)
+        verr = case_errors(got, vmc, audit_photons, brf_bar=brf_bar,
+                           pred_photons=np.where(fb[val], fallback_photons, 0))
         vfail = case_fails(verr, alb_bar)
+        vnet = ~fb[val]
+        vdec = np.clip(np.log10(queries[val, 4]).astype(int), 0, 3)
+        vbin, vover = per_bin_rms_excess(verr, vnet, bar=brf_bar)
         rep['validation'] = dict(cases=int(val.size), served_by_walk=int(fb[val].sum()),
                                  fail_frac=float(np.mean(vfail)),
+                                 fail_frac_network=float(np.mean(vfail[vnet])) if vnet.any() else None,
+                                 fail_frac_walk=float(np.mean(vfail[~vnet])) if (~vnet).any() else None,
+                                 fail_frac_by_tau_decade=[float(np.mean(vfail[vdec == d])) if np.any(vdec == d)
+                                                          else None for d in range(4)],
+                                 network_bins_over_bar=int(vover.sum()),
+                                 network_bin_rms=dict(median=float(np.nanmedian(vbin)),
+                                                      p95=float(np.nanquantile(vbin, 0.95)),
+                                                      max=float(np.nanmax(vbin))),
                                  fail_albedo_frac=float(np.mean(verr['albedo'] > alb_bar)),
                                  fail_brf_frac=float(np.mean(verr['brf_bins_fail'] > 0)),
                                  albedo_err=_pct(verr['albedo']), brf_bin_excess=_pct(verr['brf_bin_excess']),

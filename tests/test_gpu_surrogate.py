@@ -31,7 +31,7 @@ def test_surrogate_learns_and_beats_plain_mc_rate(walker):
     print('surrogate: audit walk', flush=True)
     mc, mct, mc_s = S.run_mc(walker, test_inp, 200_000, seed=13)
     pred = sur.predict(test_inp)
-    err = S.case_errors(pred, mc, mct, 200_000)
+    err = S.case_errors(pred, mc, 200_000)
     base = np.abs(tg[:, S.BRF_BINS].mean() - mc[:, S.BRF_BINS]) / mc[:, S.BRF_BINS]
     assert np.median(err['albedo']) < 0.02, np.median(err['albedo'])
     assert np.median(err['albedo']) < 0.2 * np.median(base)

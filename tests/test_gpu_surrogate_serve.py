@@ -6,8 +6,9 @@ audit misses the bars fall back to the walk.  The served answers -- network or
 fallback -- are then judged against 2,000 further queries walked
 independently at 1e6 photons (the audit's own queries carry the walk's
 answers, so they cannot judge the server).  Bars per case: albedo within 1 %,
-and no BRF bin (>= 100 reflected photons) off by more than 3 % plus 4.3
-standard errors of the judging walk."""
+and no BRF bin (>= 100 expected photons) off by more than 3 % plus 4.3
+standard errors of the judging walk; per bin over the judged cases: no bin's
+rms error (walk noise removed) above 3 % with 4-sigma confidence."""
 import numpy as np
 import pytest
 
@@ -37,6 +38,9 @@ def test_served_answers_meet_the_bars_at_config5_size():
         assert v['cases'] == 2000
         assert v['albedo_err']['p95'] <= 0.01, v
         assert v['fail_frac'] <= 0.02, v
+        # per bin over the network-served judged answers (judging walks' noise
+        # removed): no bin's rms error above 3 % with 4-sigma confidence
+        assert v['network_bins_over_bar'] == 0, v
         assert rep['fallback_frac'] <= 0.05, rep
         # what the network gains: the network path alone, and the served rate
         # with the audit and fallback walks, against the plain walk's cases/s

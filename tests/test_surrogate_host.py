@@ -47,3 +47,61 @@ def test_target_variance_and_symmetrisation():
 def test_regions_cover_the_input_box():
     r = S.region_of(S.sample_inputs(50_000, seed=5))
     assert r.min() >= 0 and r.max() < 8 * 6 * 3 and len(np.unique(r)) > 100
+
+
+def _synthetic_walks(rng, n_cases, photons, a=0.6):
+    """Tallies of walks of `photons` photons (wvn 1) whose reflected photons
+    fall into the 18 x 36 bins with a smooth, phi-symmetric law, and the
+    exact targets of that law."""
+    th = (np.arange(18) + 0.5) * 5.0
+    ph = (np.arange(36) + 0.5) * 10.0
+    law = (1.0 + 0.5 * np.cos(np.radians(th)))[:, None] * (1.2 + np.cos(np.radians(ph)))[None, :]
+    law = law * np.sin(np.radians(th))[:, None] * np.cos(np.radians(th))[:, None]
+    law = (law / law.sum()).ravel()
+    p = np.concatenate([a * law, [0.25, 0.1, 1 - a - 0.35]])   # bins, T_diffuse, T_direct, absorbed
+    t = np.zeros((n_cases, gpu.TALLY_F64))
+    exact = np.zeros((1, gpu.TALLY_F64))
+    exact[0, :S.BRF_BINS] = photons * a * law
+    exact[0, gpu.T_QDOWN] = photons
+    exact[0, gpu.T_QCOND:gpu.T_QCOND + 3] = photons * np.array([a, 0.25, 0.1])
+    for i in range(n_cases):
+        c = rng.multinomial(photons, p)
+        t[i, :S.BRF_BINS] = c[:S.BRF_BINS]
+        t[i, gpu.T_QDOWN] = photons
+        t[i, gpu.T_QCOND] = c[:S.BRF_BINS].sum()
+        t[i, gpu.T_QCOND + 1:gpu.T_QCOND + 3] = c[S.BRF_BINS:S.BRF_BINS + 2]
+    return t, S.targets_from_tallies(exact)
+
+
+def test_per_bin_bar_false_alarm_rates():
+    """The per-case BRF bar (3 % + 4.3 standard errors of the judging walk):
+    an exact prediction rarely fails a 1e6-photon judge through its noise, a
+    walk's own answer judged by another walk fails rarely once its noise is
+    counted (pred_photons) but often if it is not; the per-bin rms over cases
+    recovers a 5 % error that single cases cannot resolve."""
+    rng = np.random.default_rng(11)
+    photons = 1_000_000
+    t, exact = _synthetic_walks(rng, 300, photons)
+    mc = S.targets_from_tallies(t)
+    pred = np.repeat(exact, t.shape[0], axis=0)
+    f_exact = np.mean(S.case_fails(S.case_errors(pred, mc, photons)))
+    assert f_exact <= 0.03, f_exact
+    t2, _ = _synthetic_walks(rng, 300, photons)
+    other = S.targets_from_tallies(t2)
+    f_walk = np.mean(S.case_fails(S.case_errors(other, mc, photons, pred_photons=np.full(300, photons))))
+    f_walk_raw = np.mean(S.case_fails(S.case_errors(other, mc, photons)))
+    assert f_walk <= 0.02 and f_walk_raw >= 0.04, (f_walk, f_walk_raw)
+    # per bin over the cases (walk noise removed): ~0 for the exact
+    # prediction, the offset for one 5 % off everywhere (which the per-case
+    # test cannot see through a 1e6-photon walk's per-bin noise)
+    exact_rms, exact_over = S.per_bin_rms_excess(S.case_errors(pred, mc, photons))
+    assert not exact_over.any() and np.nanmedian(exact_rms) < 0.005, np.nanmedian(exact_rms)
+    off = pred.copy()
+    off[:, :S.BRF_BINS] *= 1.05
+    off_rms, off_over = S.per_bin_rms_excess(S.case_errors(off, mc, photons))
+    assert off_over.sum() > 100, off_over.sum()
+    assert 0.04 < np.nanmedian(off_rms) < 0.06, np.nanmedian(off_rms)
+    # 2 % off everywhere is within the bar: no bin is flagged
+    ok2 = pred.copy()
+    ok2[:, :S.BRF_BINS] *= 1.02
+    assert not S.per_bin_rms_excess(S.case_errors(ok2, mc, photons))[1].any()
