@@ -212,9 +212,28 @@ __device__ __forceinline__ void coop_local(const u32x4 &cur, float ca, float cb,
     L = Quat{ha * hc, -hb * hs, ha * hs, hb * hc};
 }
 
+// Steps per lane and round of the cooperative mode: the frame and depth scans
+// run once per round, so more steps per lane spread their cost (and give a
+// lane more independent work while the round's dependent chain runs).  Four
+// in the 96-VGPR build with the hand-over (the single-case launches: 2-5 %
+// faster than two on MI355X); two in the 80-VGPR build, where four would
+// spill in the cooperative rounds.
+template <bool HAND>
+constexpr int coop_k() { return HAND ? 4 : 2; }
+
+__device__ __forceinline__ Quat qconj(const Quat &q) { return Quat{q.w, -q.x, -q.y, -q.z}; }
+__device__ __forceinline__ float frame_uz(const Quat &q) { return fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f); }
+
 // One cooperative round of a wave (all 64 lanes call; no divergence around
 // the lane permutes).  `tab`: the wave's entries; `map`: group -> entry;
-// `me`: this lane's photon's entry.
+// `me`: this lane's photon's entry.  Lane jj of a group walks the K steps
+// k0 .. k0 + K - 1, k0 = i0 + 1 + K jj (the Philox blocks the one-lane walk
+// draws for them): it multiplies their local rotations, one inclusive scan
+// over the group gives the frame after its last step, and the frames after
+// its earlier steps follow by multiplying back with the conjugates; depth and
+// optical path are one prefix sum of the K steps' dtau uz and dtau, earlier
+// steps by difference.
+template <int K>
 __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KParams &P, CoopEntry *tab,
                                            int *map, int me, CoopGroups &cg, unsigned long long *coop_steps)
 {
@@ -248,36 +267,58 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     const CoopEntry &E = tab[cg.ent];
     const float ca = E.a, cb = E.b, ccc = E.cc, cd = E.d, ce = E.e, ctau = E.tau_tot;
     const uint32_t ct_abs = E.t_abs;
-    // lane jj walks steps ka = i0 + 1 + 2 jj and kb = ka + 1 (the same Philox
-    // blocks the one-lane walk draws for them)
-    const uint32_t ka = cg.i0 + 1u + 2u * (uint32_t)jj, kb = ka + 1u;
-    const u32x4 ua = philox_step(cg.pre, ka, P.key0, P.key1);
-    const u32x4 ub = philox_step(cg.pre, kb, P.key0, P.key1);
-    float dta, dtb;
-    Quat La, Lb;
-    coop_local(ua, ca, cb, ccc, cd, ce, dta, La);
-    coop_local(ub, ca, cb, ccc, cd, ce, dtb, Lb);
-    Quat q = qmul(La, Lb);
+    const uint32_t k0 = cg.i0 + 1u + (uint32_t)K * (uint32_t)jj;
+    float dt[K];
+    uint32_t wv[K];
+    Quat Lr[K];
+    Quat q;
+#pragma unroll
+    for (int m = 0; m < K; ++m) {
+        const u32x4 u = philox_step(cg.pre, k0 + (uint32_t)m, P.key0, P.key1);
+        coop_local(u, ca, cb, ccc, cd, ce, dt[m], Lr[m]);
+        wv[m] = u.w;
+        q = (m == 0) ? Lr[0] : qmul(q, Lr[m]);
+    }
     if (jj == 0) q = qmul(cg.f, q);   // the group's carried frame on the left
     // frames first: the depth changes need the directions
     group_scan_q(q, lg, jj);
-    // frame after step ka: F_b (x) conj(L_b)
-    const Quat qa = qmul(q, Quat{Lb.w, -Lb.x, -Lb.y, -Lb.z});
-    const float uza = fmaf(-2.f, fmaf(qa.x, qa.x, qa.y * qa.y), 1.f);
-    const float uzb = fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f);
-    const float dzb = dtb * uzb;
-    float sz = fmaf(dta, uza, dzb), sp = dta + dtb;
+    float uz[K];
+    float sz = 0.f, sp = 0.f;
+    {
+        Quat F = q;                    // frame after the lane's last step
+#pragma unroll
+        for (int m = K - 1; m >= 0; --m) {
+            uz[m] = frame_uz(F);
+            sz = fmaf(dt[m], uz[m], sz);
+            sp += dt[m];
+            if (m > 0) F = qmul(F, qconj(Lr[m]));
+        }
+    }
     group_scan_s(sz, sp, lg, jj);
-    const float zb = cg.z0 + sz, pb = cg.tau0 + sp;
-    const float za = zb - dzb, pa = pb - dtb;
-    const bool top_a = za > 0.f && uza > 0.f, bot_a = za < -ctau && uza < 0.f;
-    const bool top_b = zb > 0.f && uzb > 0.f, bot_b = zb < -ctau && uzb < 0.f;
-    const bool eva = member && (top_a || bot_a || ua.w < ct_abs || ka >= P.max_steps);
-    const bool evb = member && (top_b || bot_b || ub.w < ct_abs || kb >= P.max_steps);
-    const unsigned long long ev = __ballot(eva || evb);
+    // depths and paths after each step, from the last backwards; the first
+    // step with an event (exit, absorption, step cap) is kept
+    int A = -1;
+    bool top = false, bot = false;
+    float zk = 0.f, pk = 0.f, dtau = 0.f, uzk = 0.f;
+    uint32_t wk = 0u;
+    {
+        float z = cg.z0 + sz, pth = cg.tau0 + sp;
+#pragma unroll
+        for (int m = K - 1; m >= 0; --m) {
+            const bool t_ = z > 0.f && uz[m] > 0.f, b_ = z < -ctau && uz[m] < 0.f;
+            const bool e_ = member && (t_ || b_ || wv[m] < ct_abs || k0 + (uint32_t)m >= P.max_steps);
+            if (e_) {
+                A = m; top = t_; bot = b_; zk = z; pk = pth; dtau = dt[m]; uzk = uz[m]; wk = wv[m];
+            }
+            z -= dt[m] * uz[m];
+            pth -= dt[m];
+        }
+    }
+    const unsigned long long ev = __ballot(A >= 0);
     if (ev == 0ull) {
-        // no event: every group walked 2G steps; carry its last lane's state
+        // no event: every group walked K G steps; carry its last lane's state
         // (the frame renormalised against drift)
+        const float zb = cg.z0 + sz, pb = cg.tau0 + sp;
         Quat f;
         if (lg == 6) {
             cg.z0 = rdlane63(zb); cg.tau0 = rdlane63(pb);
@@ -289,22 +330,26 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
         }
         const float n = __builtin_amdgcn_rsqf(fmaf(f.w, f.w, fmaf(f.x, f.x, fmaf(f.y, f.y, f.z * f.z))));
         cg.f = Quat{f.w * n, f.x * n, f.y * n, f.z * n};
-        cg.i0 += 2u * (uint32_t)G;
+        cg.i0 += (uint32_t)(K * G);
         return;
     }
     // this lane's outcome, should one of its steps end the segment: its first
-    // step if that one has an event, else its second
-    const bool A = eva;
-    const bool top = A ? top_a : top_b, bot = A ? bot_a : bot_b;
-    const uint32_t wv = A ? ua.w : ub.w, k = A ? ka : kb;
-    const bool ab = wv < ct_abs, cap = k >= P.max_steps;
-    const float zk = A ? za : zb, pk = A ? pa : pb, dtau = A ? dta : dtb, uz = A ? uza : uzb;
-    const Quat qe{A ? qa.w : q.w, A ? qa.x : q.x, A ? qa.y : q.y, A ? qa.z : q.z};
+    // step with an event (else its last); the frame after that step
+    if (A < 0) {
+        A = K - 1;
+        zk = cg.z0 + sz; pk = cg.tau0 + sp; dtau = dt[K - 1]; uzk = uz[K - 1]; wk = wv[K - 1];
+    }
+    Quat qe = q;
+#pragma unroll
+    for (int m = K - 1; m >= 1; --m)
+        if (m > A) qe = qmul(qe, qconj(Lr[m]));
+    const uint32_t k = k0 + (uint32_t)A;
+    const bool ab = wk < ct_abs, cap = k >= P.max_steps;
     const float ox = 2.f * fmaf(qe.x, qe.z, qe.w * qe.y);
     const float oy = 2.f * fmaf(qe.y, qe.z, -qe.w * qe.x);
     const int type = top ? 1 : (bot ? 2 : (ab ? 3 : (cap ? 4 : 0)));
     const float corr = top ? zk : zk + ctau;
-    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uz), dtau) : pk;
+    const float pout = (top || bot) ? pk - fminf(corr * __builtin_amdgcn_rcpf(uzk), dtau) : pk;
     const float zout = bot ? -ctau : zk;
     // each photon's lane reads the first event lane of its group (or the last)
     int src = lane;
@@ -315,9 +360,9 @@ __device__ __forceinline__ void coop_round(NativeWalker &w, int &state, const KP
     }
     const int r_type = (int)bperm(src, (uint32_t)type);
     const uint32_t r_k = bperm(src, k);
-    const uint32_t r_w = bperm(src, wv);
+    const uint32_t r_w = bperm(src, wk);
     const float r_z = bperm(src, zout), r_p = bperm(src, pout);
-    const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uz);
+    const float r_x = bperm(src, ox), r_y = bperm(src, oy), r_uz = bperm(src, uzk);
     cg.em = 0ull;   // photons' lanes hold the state again: regroup next round
     if (elig) {
         atomicAdd(coop_steps, (unsigned long long)(r_k - w.i));
@@ -427,7 +472,7 @@ __device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int 
     cg.lg = 0;
     cg.ent = 0;
     for (uint32_t round = 1; __ballot(state == LIVE); ++round) {
-        coop_round(w, state, P, tab, map, me, cg, coop_steps);
+        coop_round<coop_k<HAND>()>(w, state, P, tab, map, me, cg, coop_steps);
         if constexpr (HAND) {
             if ((round & 1u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
         }

@@ -282,15 +282,18 @@ __global__ void k_gather(const float *xd, const float *td, const int32_t *idx, i
 // dY = 2 w (y - t) / rows (bf16) and loss = sum w (y - t)^2 / rows
 __global__ void k_loss(const float *Y, const float *T, const float *w, int rows, int np, bf16 *dY, float *loss)
 {
+    // rows are strided over blocks and columns over a block's threads: no
+    // per-element index division
     __shared__ float red[256];
     float acc = 0.f;
     const float inv = 1.f / rows;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (size_t)rows * np;
-         i += (size_t)gridDim.x * blockDim.x) {
-        const int c = (int)(i % np);
-        const float d = Y[i] - T[i];
-        acc += w[c] * d * d;
-        dY[i] = (bf16)(2.f * w[c] * d * inv);
+    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+        const size_t o = (size_t)r * np;
+        for (int c = threadIdx.x; c < np; c += blockDim.x) {
+            const float d = Y[o + c] - T[o + c], wc = w[c];
+            acc += wc * d * d;
+            dY[o + c] = (bf16)(2.f * wc * d * inv);
+        }
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -301,15 +304,31 @@ __global__ void k_loss(const float *Y, const float *T, const float *w, int rows,
     if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
 }
 
-// db[c] += sum over rows [blockIdx.y * 256, +256) of dZ[r][c] (db zeroed first)
-__global__ void k_colsum(const bf16 *dZ, int rows, int cols, int ld, float *db)
+// db[c] += column sums of dZ (db zeroed first).  Block (64, 4): a wave per
+// row phase, a lane per column; a block covers 64 columns x COLSUM_ROWS rows,
+// four independent partial sums per lane, the four waves reduced through LDS
+// and one atomic per column and block.
+constexpr int COLSUM_ROWS = 128;
+__global__ void __launch_bounds__(256) k_colsum(const bf16 *dZ, int rows, int cols, int ld, float *db)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
-    const int r0 = blockIdx.y * 256, r1 = min(rows, r0 + 256);
-    float s = 0.f;
-    for (int r = r0; r < r1; ++r) s += (float)dZ[(size_t)r * ld + c];
-    unsafeAtomicAdd(&db[c], s);
+    __shared__ float part[4][64];
+    const int c = blockIdx.x * 64 + threadIdx.x;
+    const int r0 = blockIdx.y * COLSUM_ROWS, r1 = min(rows, r0 + COLSUM_ROWS);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (c < cols) {
+        int r = r0 + threadIdx.y;
+        for (; r + 12 < r1; r += 16) {
+            s0 += (float)dZ[(size_t)r * ld + c];
+            s1 += (float)dZ[(size_t)(r + 4) * ld + c];
+            s2 += (float)dZ[(size_t)(r + 8) * ld + c];
+            s3 += (float)dZ[(size_t)(r + 12) * ld + c];
+        }
+        for (; r < r1; r += 4) s0 += (float)dZ[(size_t)r * ld + c];
+    }
+    part[threadIdx.y][threadIdx.x] = (s0 + s1) + (s2 + s3);
+    __syncthreads();
+    if (threadIdx.y == 0 && c < cols)
+        unsafeAtomicAdd(&db[c], (part[0][threadIdx.x] + part[1][threadIdx.x]) + (part[2][threadIdx.x] + part[3][threadIdx.x]));
 }
 
 __global__ void k_fill_f32(float *p, size_t n, float v)
@@ -495,8 +514,8 @@ static void backward_pass(mcs_mlp *c, int rows)
     hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, c->loss, (size_t)1, 0.f);
     // weight and bias gradients are sums of split-K / row-slice partials
     hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, c->stream, c->gr, c->n_pad, 0.f);
-    const dim3 colsum_rows(1, (rows + 255) / 256);
-    hipLaunchKernelGGL(k_loss, dim3(256), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY, c->loss);
+    const int colsum_blocks_y = (rows + COLSUM_ROWS - 1) / COLSUM_ROWS;
+    hipLaunchKernelGGL(k_loss, dim3(std::min(rows, 1024)), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY, c->loss);
     // output layer
     const Layer &lo = c->layers[L];
     const bf16 *Aprev = c->A[L - 1];
@@ -504,7 +523,7 @@ static void backward_pass(mcs_mlp *c, int rows)
         GemmArgs g = ga(c->dY, c->np, Aprev, lo.pcol, lo.prow, lo.pcol, rows);  // dW = dY^T A
         g.C = c->gr + lo.w_off; g.ldc = lo.pcol;
         gemm<true, true, EPI_ADD_F32>(c, g);
-        hipLaunchKernelGGL(k_colsum, dim3((lo.prow + 255) / 256, colsum_rows.y), dim3(256), 0, c->stream, c->dY,
+        hipLaunchKernelGGL(k_colsum, dim3((lo.prow + 63) / 64, colsum_blocks_y), dim3(64, 4), 0, c->stream, c->dY,
                            rows, lo.prow, c->np, c->gr + lo.b_off);
         GemmArgs h = ga(c->dY, c->np, c->wb + lo.wb_off, lo.pcol, rows, lo.pcol, lo.prow);  // dA = dY W
         h.Zin = c->Z[L - 1]; h.ldc = lo.pcol;
@@ -519,7 +538,7 @@ static void backward_pass(mcs_mlp *c, int rows)
         GemmArgs g = ga(dz, ly.prow, inp, ldi, ly.prow, ly.pcol, rows);  // dW_l = dZ^T in
         g.C = c->gr + ly.w_off; g.ldc = ly.pcol;
         gemm<true, true, EPI_ADD_F32>(c, g);
-        hipLaunchKernelGGL(k_colsum, dim3((ly.prow + 255) / 256, colsum_rows.y), dim3(256), 0, c->stream, dz,
+        hipLaunchKernelGGL(k_colsum, dim3((ly.prow + 63) / 64, colsum_blocks_y), dim3(64, 4), 0, c->stream, dz,
                            rows, ly.prow, ly.prow, c->gr + ly.b_off);
         if (l > 0) {
             GemmArgs h = ga(dz, ly.prow, c->wb + ly.wb_off, ly.pcol, rows, ly.pcol, ly.prow);

@@ -642,6 +642,7 @@ struct mcs_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_rb[4] = {};   // read-back pipeline: one per chunk in flight
     char err[512] = {0};
     mcs_launch launch{256, 4, 2048, 8, 16};
     // host copies
@@ -865,6 +866,10 @@ int mcs_create(int device, mcs_ctx **out)
     c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rb[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rb[1], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rb[2], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_rb[3], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_sum, 8 * sizeof(unsigned long long), hipHostMallocMapped) != hipSuccess ||
         hipHostGetDevicePointer((void **)&c->d_sum, c->h_sum, 0) != hipSuccess) {
         (void)mcs_destroy(c);
@@ -892,6 +897,7 @@ int mcs_destroy(mcs_ctx *c)
     for (auto &b : c->d_out) if (b.p) (void)hipFree(b.p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    for (auto &e : c->ev_rb) if (e) (void)hipEventDestroy(e);
     if (c->h_sum) (void)hipHostFree(c->h_sum);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1542,35 +1548,50 @@ int mcs_read_records(mcs_ctx *c, const mcs_records *o, int64_t n)
     const bool rep = c->last_mode == MCS_REPLAY;
     const size_t rsz = rep ? 8 : 4;
     // photon-order device arrays (n_scat, cond | band << 8, theta, phi, path)
-    // -> pinned staging in one stream, then widened on host threads
+    // -> pinned staging, in up to 4 chunks whose copies run while host
+    // threads widen the chunks before them into the caller's arrays
     int r;
     if ((r = stage(c, (size_t)n * (8 + 3 * rsz)))) return r;
     uint32_t *s_ns = (uint32_t *)c->h_stage, *s_cb = s_ns + n;
     char *s_f = (char *)(s_cb + n);
-    HIPCHK(c, hipMemcpyAsync(s_ns, c->d_out[0].p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(s_cb, c->d_out[1].p, 4 * (size_t)n, hipMemcpyDeviceToHost, c->stream));
     double *dst[3] = {o->theta_n, o->phi_n, o->path_length};
-    for (int k = 0; k < 3; ++k)
-        if (dst[k])
-            HIPCHK(c, hipMemcpyAsync(s_f + (size_t)k * n * rsz, c->d_out[2 + k].p, rsz * (size_t)n,
-                                     hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    parallel_rows(n, [&](int64_t a, int64_t b) {
-        for (int64_t j = a; j < b; ++j) {
-            if (o->n_scat) o->n_scat[j] = s_ns[j];
-            if (o->condition) o->condition[j] = (int8_t)(s_cb[j] & 0xff);
-            if (o->band) o->band[j] = (uint8_t)(s_cb[j] >> 8);
-        }
-        for (int k = 0; k < 3; ++k) {
-            if (!dst[k]) continue;
-            if (rep) {
-                memcpy(dst[k] + a, (const double *)(s_f + (size_t)k * n * rsz) + a, sizeof(double) * (size_t)(b - a));
-            } else {
-                const float *src = (const float *)(s_f + (size_t)k * n * rsz);
-                for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
+    const int nch = n >= (1 << 18) ? 4 : 1;
+    const int64_t per = (n + nch - 1) / nch;
+    for (int q = 0; q < nch; ++q) {
+        const int64_t a = std::min(n, q * per), e = std::min(n, a + per);
+        if (e <= a) continue;
+        const size_t m = (size_t)(e - a);
+        HIPCHK(c, hipMemcpyAsync(s_ns + a, (const uint32_t *)c->d_out[0].p + a, 4 * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipMemcpyAsync(s_cb + a, (const uint32_t *)c->d_out[1].p + a, 4 * m, hipMemcpyDeviceToHost, c->stream));
+        for (int k = 0; k < 3; ++k)
+            if (dst[k])
+                HIPCHK(c, hipMemcpyAsync(s_f + ((size_t)k * n + a) * rsz, (const char *)c->d_out[2 + k].p + a * rsz,
+                                         rsz * m, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev_rb[q], c->stream));
+    }
+    for (int q = 0; q < nch; ++q) {
+        const int64_t a0 = std::min(n, q * per), e0 = std::min(n, a0 + per);
+        if (e0 <= a0) continue;
+        HIPCHK(c, hipEventSynchronize(c->ev_rb[q]));
+        parallel_rows(e0 - a0, [&](int64_t a, int64_t b) {
+            a += a0;
+            b += a0;
+            for (int64_t j = a; j < b; ++j) {
+                if (o->n_scat) o->n_scat[j] = s_ns[j];
+                if (o->condition) o->condition[j] = (int8_t)(s_cb[j] & 0xff);
+                if (o->band) o->band[j] = (uint8_t)(s_cb[j] >> 8);
             }
-        }
-    });
+            for (int k = 0; k < 3; ++k) {
+                if (!dst[k]) continue;
+                if (rep) {
+                    memcpy(dst[k] + a, (const double *)(s_f + (size_t)k * n * rsz) + a, sizeof(double) * (size_t)(b - a));
+                } else {
+                    const float *src = (const float *)(s_f + (size_t)k * n * rsz);
+                    for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
+                }
+            }
+        });
+    }
     return MCS_OK;
 }
 

@@ -107,7 +107,7 @@ def run_mlp(repeats=3, queries=1_000_000, batch=4096, steps=100):
     net.close()
 
 
-def run_snow1m(repeats=6, launch=(1024, 1, 977, 16, 32), photons=1_000_000, json_out=None):
+def run_snow1m(repeats=6, launch=(1024, 1, 1954, 16, 32), photons=1_000_000, json_out=None):
     """The headline workload (bench.py: snow_1m, 1e6 photons, records on) with
     a fixed launch shape `repeats` times; prints and optionally writes the
     photon-steps per launch (for the per-step instruction counts)."""
@@ -205,7 +205,7 @@ def main():
     po.add_argument('--photons', type=int, default=1_000_000)
     ml = sub.add_parser('mlp'); ml.add_argument('--repeats', type=int, default=3)
     sn = sub.add_parser('snow1m'); sn.add_argument('--repeats', type=int, default=6)
-    sn.add_argument('--launch', default='1024,1,977,16,32'); sn.add_argument('--json')
+    sn.add_argument('--launch', default='1024,1,1954,16,32'); sn.add_argument('--json')
     vm = sub.add_parser('valu'); vm.add_argument('csv'); vm.add_argument('--kernel', default='k_walk')
     vm.add_argument('--json', help='write the per-dispatch medians here')
     vm.add_argument('--steps', type=float, help='photon-steps per launch of the profiled workload')

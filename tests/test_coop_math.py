@@ -13,6 +13,7 @@ in numpy and checked on the CPU:
   of the direction after k scatterings agree (Kolmogorov-Smirnov).
 The GPU tests (test_gpu_coop.py) check the kernel itself against the oracle."""
 import numpy as np
+import pytest
 from scipy import stats
 
 
@@ -211,3 +212,52 @@ def test_two_steps_per_lane_give_every_frame_and_depth():
                 zb_k = z0 + np.sum(dta[g0:j + 1] * uza[g0:j + 1] + dtb[g0:j + 1] * uzb[g0:j + 1])
                 np.testing.assert_allclose(zb_k, z, atol=1e-10)
                 np.testing.assert_allclose(zb_k - dtb[j] * uzb[j], za, atol=1e-10)
+
+
+@pytest.mark.parametrize('K', [2, 3, 4])
+def test_k_steps_per_lane_give_every_frame_and_depth(K):
+    """The kernel's round with K steps per lane (coop_round<K>): lane j of a
+    group walks steps K j + 1 .. K j + K; the scan runs over the product of
+    its K local rotations (the carried frame on lane 0's left), the frame
+    after its last step is the inclusive product, the earlier frames follow
+    backwards, F_{m-1} = F_m (x) conj(L_m), and the depths backwards from one
+    prefix sum of the K steps' dtau uz.  Checked against the sequential walk
+    for every group size."""
+    rng = np.random.default_rng(40 + K)
+    conj = lambda q: q * np.array([1.0, -1.0, -1.0, -1.0])
+    for lg in range(0, 7):
+        G = 1 << lg
+        d0 = rng.normal(size=3)
+        d0 /= np.linalg.norm(d0)
+        f = frame_of(d0[None])[0]
+        L = [local_quat(rng.uniform(-1, 1, 64), rng.uniform(0, 1, 64)) for _ in range(K)]
+        dt = [rng.exponential(size=64) for _ in range(K)]
+        lanes = np.arange(64)
+        Q = L[0]
+        for m in range(1, K):
+            Q = qmul(Q, L[m])
+        first = (lanes & (G - 1)) == 0
+        Q[first] = qmul(np.broadcast_to(f, (64, 4)), Q)[first]
+        F = group_scan_q(Q.copy(), lg) if lg > 0 else Q
+        frames = [None] * K
+        frames[K - 1] = F
+        for m in range(K - 1, 0, -1):
+            frames[m - 1] = qmul(frames[m], conj(L[m]))
+        uz = [frame_dir(fr)[:, 2] for fr in frames]
+        sz = sum(dt[m] * uz[m] for m in range(K))
+        z0 = -2.5
+        for g0 in range(0, 64, G):
+            Fs = f.copy()
+            z = z0
+            for j in range(g0, g0 + G):
+                seq_z = []
+                for m in range(K):
+                    Fs = qmul(Fs, L[m][j])
+                    np.testing.assert_allclose(frames[m][j], Fs, atol=1e-11, err_msg='K=%d G=%d' % (K, G))
+                    z += dt[m][j] * frame_dir(Fs)[2]
+                    seq_z.append(z)
+                # kernel: depth after the last step from the prefix sum, earlier ones backwards
+                zk = z0 + np.sum(sz[g0:j + 1])
+                for m in range(K - 1, -1, -1):
+                    np.testing.assert_allclose(zk, seq_z[m], atol=1e-9)
+                    zk -= dt[m][j] * uz[m][j]

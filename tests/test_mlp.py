@@ -41,7 +41,8 @@ def bf16_round(a):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize('dims', [(5, 64, 2, 70, 128), (5, 256, 4, 651, 4096), (3, 128, 3, 100, 256)])
+@pytest.mark.parametrize('dims', [(5, 64, 2, 70, 128), (5, 256, 4, 651, 4096), (3, 128, 3, 100, 256),
+                                  (4, 64, 2, 37, 192)])
 def test_gpu_forward_and_grads_match_reference(dims):
     from monte_carlompi_amd.mlp import Mlp
     n_in, H, L, n_out, batch = dims
