@@ -109,9 +109,11 @@ def main():
     fr = fractions_from_tallies(w.tallies())
     # PCIe-inclusive (not `value`): the same walk plus the read-back of every
     # record into host arrays in the reference's record dtypes (int8 condition,
-    # int64 n_scat, float64 angles / path / wvn / depth), the host buffers kept
-    # from walk to walk as a streaming consumer keeps them
-    rec_buf = w.records()
+    # int64 n_scat, float64 angles / path / wvn / depth), written in that
+    # layout on the device and copied by DMA into pinned host buffers kept
+    # from walk to walk, as a streaming consumer keeps them
+    rec_buf = w.record_buffers()
+    w.records(out=rec_buf)
     t2 = time.perf_counter()
     for k in range(5):
         w.walk(mode='native', seed=500 + k, records=True)
@@ -381,7 +383,9 @@ def surrogate_block(w):
     x 1e5 photons from batched launches, 200 epochs of batch-4096 Adam on
     the BF16-MFMA kernels), then 1e6 random queries served on the GPU with a
     seeded 1 % audit by the walk at 1e6 photons and region-wise fallback;
-    2,000 further queries walked at 1e6 photons judge the served answers."""
+    2,000 further queries walked at 1e6 photons judge the served answers.
+    The same network and queries are served again with a 0.2 % audit
+    (`audit_0p2pct`): a 1 % audit alone caps the served rate at 100x."""
     import numpy as np
     from monte_carlompi_amd import roofline
     from monte_carlompi_amd import surrogate as S
@@ -392,6 +396,9 @@ def surrogate_block(w):
     q = S.sample_inputs(1_000_000, 12345)
     _, rep = S.serve(sur, w, q, audit_frac=0.01, audit_photons=1_000_000, seed=3, validate=2000)
     t2 = time.perf_counter()
+    # the same network and queries with a 0.2 % audit (2,000 walks), judged
+    # by 2,000 fresh queries
+    _, rep2 = S.serve(sur, w, q, audit_frac=0.002, audit_photons=1_000_000, seed=4, validate=2000)
     per_case = S.plain_mc_rate_per_case(w, S.sample_inputs(200, 777), 1_000_000)
     sur.net.close()
     flops = roofline.mlp_flops_per_query() * rep['n_queries'] / rep['predict_s_device'] / 1e12
@@ -407,6 +414,19 @@ def surrogate_block(w):
     out['gain_served_vs_batched_mc'] = round(rep['cases_per_s_served'] / rep['plain_mc_cases_per_s_batched'], 2)
     out['gain_served_vs_per_case_mc'] = round(rep['cases_per_s_served'] / per_case, 2)
     out['served_ceiling_vs_batched_mc'] = round(1.0 / (rep['n_audit'] / rep['n_queries']), 1)
+    v2 = rep2['validation']
+    out['audit_0p2pct'] = {
+        'n_audit': rep2['n_audit'], 'audit_fail_frac': round(rep2['audit_fail_frac'], 6),
+        'fallback_regions': rep2['fallback_regions'], 'fallback_frac': round(rep2['fallback_frac'], 6),
+        'predict_s_device': round(rep2['predict_s_device'], 6), 'audit_s_device': round(rep2['audit_s_device'], 6),
+        'fallback_s_device': round(rep2['fallback_s_device'], 6), 'served_s_host': round(rep2['served_s_host'], 4),
+        'cases_per_s_served': round(rep2['cases_per_s_served'], 1),
+        'cases_per_s_served_host': round(rep2['cases_per_s_served_host'], 1),
+        'gain_served_vs_batched_mc': round(rep2['cases_per_s_served'] / rep['plain_mc_cases_per_s_batched'], 2),
+        'gain_served_vs_per_case_mc': round(rep2['cases_per_s_served'] / per_case, 2),
+        'served_ceiling_vs_batched_mc': round(rep2['n_queries'] / rep2['n_audit'], 1),
+        'validation': {k: v2[k] for k in ('cases', 'served_by_walk', 'fail_frac', 'fail_frac_network',
+                                          'network_bins_over_bar', 'albedo_err', 'network_bin_rms')}}
     return out
 
 

@@ -141,15 +141,19 @@ typedef struct {
 /* Host output arrays (each sum(n_photons) long, photon order; any may be
  * NULL).  The CSV record of mc3D:1489-1490 / 1626-1636 is
  * (condition, wvn[band], theta_n, phi_n, n_scat, path_length,
- *  snow_depth[band]). */
+ *  snow_depth[band]).  The device writes the requested columns in these
+ * types; an array in pinned host memory (hipHostMalloc, torch pin_memory) is
+ * filled by one DMA copy, any other through the library's pinned staging. */
 typedef struct {
     int8_t *condition;         /* 1 top, 2 bottom diffuse, 3 bottom direct,
                                   4 absorbed by ice, 5 by impurity, 0 capped  */
-    uint8_t *band;
+    uint8_t *band;             /* band index within the photon's case         */
     int64_t *n_scat;
     double *theta_n;
     double *phi_n;
     double *path_length;
+    double *wvn;               /* mcs_band.wvn of the photon's band           */
+    double *snow_depth;        /* mcs_band.snow_depth of the photon's band    */
 } mcs_records;
 
 typedef struct {

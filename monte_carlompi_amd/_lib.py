@@ -82,7 +82,7 @@ class McsPolarCase(ctypes.Structure):
 
 class McsRecords(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in
-                ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length')]
+                ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'wvn', 'snow_depth')]
 
 
 class McsDevOut(ctypes.Structure):

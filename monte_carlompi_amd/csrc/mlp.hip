@@ -279,21 +279,32 @@ __global__ void k_gather(const float *xd, const float *td, const int32_t *idx, i
         T[(size_t)r * np + c] = c < n_out ? td[s * n_out + c] : 0.f;
 }
 
-// dY = 2 w (y - t) / rows (bf16) and loss = sum w (y - t)^2 / rows
-__global__ void k_loss(const float *Y, const float *T, const float *w, int rows, int np, bf16 *dY, float *loss)
+// dY = 2 w (y - t) / rows (bf16) and *loss_out = sum w (y - t)^2 / rows.
+// Four columns per thread (np is a multiple of 64); each block leaves its
+// partial sum in part[], and the last block to finish (ticket) adds them in
+// block order and re-arms the ticket, so the loss is deterministic.
+constexpr int LOSS_BLOCKS = 1024;
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_loss(const float *Y, const float *T, const float *w, int rows, int np,
+                                              bf16 *dY, float *part, unsigned *ticket, float *loss_out)
 {
-    // rows are strided over blocks and columns over a block's threads: no
-    // per-element index division
     __shared__ float red[256];
-    float acc = 0.f;
+    __shared__ bool last;
     const float inv = 1.f / rows;
-    for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-        const size_t o = (size_t)r * np;
-        for (int c = threadIdx.x; c < np; c += blockDim.x) {
-            const float d = Y[o + c] - T[o + c], wc = w[c];
-            acc += wc * d * d;
-            dY[o + c] = (bf16)(2.f * wc * d * inv);
-        }
+    const uint32_t np4 = (uint32_t)np >> 2, n4 = (uint32_t)rows * np4;
+    float acc = 0.f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+        const float4 y = reinterpret_cast<const float4 *>(Y)[i];
+        const float4 t = reinterpret_cast<const float4 *>(T)[i];
+        const float4 wc = reinterpret_cast<const float4 *>(w)[i % np4];
+        const float d0 = y.x - t.x, d1 = y.y - t.y, d2 = y.z - t.z, d3 = y.w - t.w;
+        acc += (wc.x * d0 * d0 + wc.y * d1 * d1) + (wc.z * d2 * d2 + wc.w * d3 * d3);
+        bf16x4 o;
+        o[0] = (bf16)(2.f * wc.x * d0 * inv);
+        o[1] = (bf16)(2.f * wc.y * d1 * inv);
+        o[2] = (bf16)(2.f * wc.z * d2 * inv);
+        o[3] = (bf16)(2.f * wc.w * d3 * inv);
+        reinterpret_cast<bf16x4 *>(dY)[i] = o;
     }
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -301,7 +312,27 @@ __global__ void k_loss(const float *Y, const float *T, const float *w, int rows,
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(loss, red[0] * inv);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&part[blockIdx.x], red[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    float sum = 0.f;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
+        sum += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    red[threadIdx.x] = sum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *loss_out = red[0] * inv;
+        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // db[c] += column sums of dZ (db zeroed first).  Block (64, 4): a wave per
@@ -388,6 +419,8 @@ struct mcs_mlp {
     std::vector<bf16 *> A;
     float *Y = nullptr, *T = nullptr, *w = nullptr, *yout = nullptr, *xin = nullptr, *loss = nullptr;
     int32_t *idx = nullptr;
+    float *lpart = nullptr;          // k_loss: per-block partial sums (LOSS_BLOCKS)
+    unsigned *lticket = nullptr;     // k_loss: blocks finished (0 between launches)
     float *xd = nullptr, *td = nullptr;
     int64_t n_data = 0;
     double last_ms = 0.0;
@@ -507,15 +540,17 @@ static void forward_pass(mcs_mlp *c, int rows)
     gemm<false, false, EPI_F32>(c, g);
 }
 
-// loss + backward for `rows` rows (X, T in place); gradients into c->gr
-static void backward_pass(mcs_mlp *c, int rows)
+// loss (into *loss_out) + backward for `rows` rows (X, T in place);
+// gradients into c->gr
+static void backward_pass(mcs_mlp *c, int rows, float *loss_out)
 {
     const int L = c->d.n_hidden_layers;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, c->stream, c->loss, (size_t)1, 0.f);
     // weight and bias gradients are sums of split-K / row-slice partials
     hipLaunchKernelGGL(k_fill_f32, dim3(256), dim3(256), 0, c->stream, c->gr, c->n_pad, 0.f);
     const int colsum_blocks_y = (rows + COLSUM_ROWS - 1) / COLSUM_ROWS;
-    hipLaunchKernelGGL(k_loss, dim3(std::min(rows, 1024)), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY, c->loss);
+    const int loss_blocks = (int)std::min<int64_t>(LOSS_BLOCKS, ((int64_t)rows * (c->np / 4) + 255) / 256);
+    hipLaunchKernelGGL(k_loss, dim3(loss_blocks), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY,
+                       c->lpart, c->lticket, loss_out);
     // output layer
     const Layer &lo = c->layers[L];
     const bf16 *Aprev = c->A[L - 1];
@@ -613,7 +648,9 @@ int mcs_mlp_create(int device, const mcs_mlp_dims *dims, mcs_mlp **out)
               dalloc(&c->dZb, MB * H) == hipSuccess && dalloc(&c->Y, MB * c->np) == hipSuccess &&
               dalloc(&c->T, MB * c->np) == hipSuccess && dalloc(&c->w, c->np) == hipSuccess &&
               dalloc(&c->yout, MB * dims->n_out) == hipSuccess && dalloc(&c->xin, MB * dims->n_in) == hipSuccess &&
-              dalloc(&c->loss, 1024) == hipSuccess && dalloc(&c->idx, MB) == hipSuccess;
+              dalloc(&c->loss, 1024) == hipSuccess && dalloc(&c->idx, MB) == hipSuccess &&
+              dalloc(&c->lpart, LOSS_BLOCKS) == hipSuccess && dalloc(&c->lticket, 1) == hipSuccess &&
+              hipMemset(c->lticket, 0, sizeof(unsigned)) == hipSuccess;
     for (int l = 0; ok && l < L; ++l) {
         float *z = nullptr;
         bf16 *a = nullptr;
@@ -644,7 +681,7 @@ int mcs_mlp_destroy(mcs_mlp *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->p, c->gr, c->m, c->v, c->mask, c->pb_index, c->wb, c->X, c->dY, c->dZa, c->dZb,
                     c->Y, c->T, c->w, c->yout, c->xin, c->loss, c->idx, c->xd, c->td,
-                    c->fmu, c->fsd, c->oscale, c->oshift, c->ooff};
+                    c->fmu, c->fsd, c->oscale, c->oshift, c->ooff, c->lpart, c->lticket};
     for (void *b : bufs) if (b) (void)hipFree(b);
     for (float *z : c->Z) if (z) (void)hipFree(z);
     for (bf16 *a : c->A) if (a) (void)hipFree(a);
@@ -839,7 +876,7 @@ int mcs_mlp_grads(mcs_mlp *c, const float *x, const float *t, const float *w, in
     MCHK(c, hipMemcpy(c->T, tp.data(), tp.size() * 4, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(k_cast_x, dim3((n * K0 + 255) / 256), dim3(256), 0, c->stream, c->xin, ni, n, c->X);
     forward_pass(c, n);
-    backward_pass(c, n);
+    backward_pass(c, n, c->loss);
     MCHK(c, hipGetLastError());
     MCHK(c, hipStreamSynchronize(c->stream));
     std::vector<float> pad(c->n_pad);
@@ -882,14 +919,11 @@ int mcs_mlp_train_steps(mcs_mlp *c, const int32_t *idx, int32_t n_steps, int32_t
     MCHK(c, dalloc(&didx, (size_t)n_steps * batch));
     hipError_t e = hipMemcpy(didx, idx, sizeof(int32_t) * (size_t)n_steps * batch, hipMemcpyHostToDevice);
     if (e != hipSuccess) { (void)hipFree(didx); return merr(c, MCS_ERR_HIP, "idx upload: %s", hipGetErrorString(e)); }
-    hipLaunchKernelGGL(k_fill_f32, dim3(4), dim3(256), 0, c->stream, c->loss, (size_t)1024, 0.f);
     for (int s = 0; s < n_steps; ++s) {
         hipLaunchKernelGGL(k_gather, dim3(batch), dim3(64), 0, c->stream, c->xd, c->td, didx + (size_t)s * batch,
                            c->d.n_in, c->d.n_out, c->np, batch, c->n_data, c->X, c->T);
         forward_pass(c, batch);
-        backward_pass(c, batch);
-        // k_loss accumulates into loss[0]; keep step s's value in loss[1 + s]
-        (void)hipMemcpyAsync(c->loss + 1 + s, c->loss, 4, hipMemcpyDeviceToDevice, c->stream);
+        backward_pass(c, batch, c->loss + 1 + s);   // step s's loss in loss[1 + s]
         const int t = a->step + s + 1;
         const float lr_t = a->lr * std::sqrt(1.f - std::pow(a->beta2, (float)t)) / (1.f - std::pow(a->beta1, (float)t));
         hipLaunchKernelGGL(k_adam, dim3(1024), dim3(256), 0, c->stream, c->p, c->gr, c->m, c->v, c->mask, c->n_pad,

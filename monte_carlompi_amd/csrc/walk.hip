@@ -499,6 +499,52 @@ __global__ void k_scatter_records(const uint32_t *gid, const uint32_t *nscat, co
 // (retirement order), so the block gathers them into LDS by id and writes
 // them out contiguously: coalesced reads and writes.  LDS: chunk x (8 + 3
 // sizeof(R)) bytes.
+// Host-layout record columns (mcs_records), written on the device so that
+// read-back is plain copies.  Every pointer is nullable.
+struct WideRecords {
+    int8_t *condition;
+    uint8_t *band;
+    int64_t *n_scat;
+    double *theta_n, *phi_n, *path_length, *wvn, *snow_depth;
+};
+
+// Photon-order compact records (n_scat, condition | band << 8, theta, phi,
+// path in R) -> WideRecords.  The case of photon j is found by bisection on
+// the cases' first global ids; wvn and snow_depth come from the photon's band.
+template <class R>
+__global__ void k_widen_records(const DevCase *cases, const DevBand *bands, int n_cases, uint64_t n,
+                                const uint32_t *nscat, const uint32_t *cb, const R *th, const R *ph, const R *pa,
+                                WideRecords o)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t v = cb[j];
+        if (o.condition) o.condition[j] = (int8_t)(v & 0xffu);
+        if (o.band) o.band[j] = (uint8_t)(v >> 8);
+        if (o.n_scat) o.n_scat[j] = (int64_t)nscat[j];
+        if (o.theta_n) o.theta_n[j] = (double)th[j];
+        if (o.phi_n) o.phi_n[j] = (double)ph[j];
+        if (o.path_length) o.path_length[j] = (double)pa[j];
+        if (o.wvn || o.snow_depth) {
+            int lo = 0, hi = n_cases;   // cases[lo].gid_base <= j < cases[hi].gid_base
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if ((uint64_t)cases[mid].gid_base <= j) lo = mid; else hi = mid;
+            }
+            const DevBand &B = bands[cases[lo].band_begin + (int)(v >> 8)];
+            if (o.wvn) o.wvn[j] = B.wvn;
+            if (o.snow_depth) o.snow_depth[j] = B.snow_depth;
+        }
+    }
+}
+
+// Stokes planes (3 x n in R) -> float64 planes.
+template <class R>
+__global__ void k_widen_stokes(const R *in, uint64_t n3, double *out)
+{
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < n3; j += (uint64_t)gridDim.x * blockDim.x)
+        out[j] = (double)in[j];
+}
+
 template <class R>
 __global__ void k_order_records(const DevCase *cases, const int32_t *chunk0, int n_cases, int chunk,
                                 uint32_t n_chunks, const uint32_t *gid, const uint32_t *nscat, const uint32_t *cb,
@@ -642,7 +688,6 @@ struct mcs_ctx {
     int cus = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    hipEvent_t ev_rb[4] = {};   // read-back pipeline: one per chunk in flight
     char err[512] = {0};
     mcs_launch launch{256, 4, 2048, 8, 16};
     // host copies
@@ -660,6 +705,7 @@ struct mcs_ctx {
     int32_t pm_n = 0;
     size_t pm_floats = 0;        // FP32 table floats (n_tables x pm_stride)
     DevBuf d_pm, d_pmf, d_stokes;
+    DevBuf d_wide;                  // host-layout record columns for read-back
     bool last_stokes = false;
     int last_mode = MCS_NATIVE;
     bool last_records = false;
@@ -848,6 +894,60 @@ static void parallel_rows(int64_t n, F f)
     for (auto &x : th) x.join();
 }
 
+// [p, p + bytes) lies in one pinned (page-locked, hipHostMalloc) host
+// allocation: the DMA engine can write it directly.
+static bool host_pinned(const void *p, size_t bytes)
+{
+    hipPointerAttribute_t a;
+    if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+        (void)hipGetLastError();   // pageable memory: the query reports invalid value
+        return false;
+    }
+    if (a.type != hipMemoryTypeHost) return false;
+    void *start = nullptr;
+    size_t size = 0;
+    if (hipPointerGetAttribute(&start, HIP_POINTER_ATTRIBUTE_RANGE_START_ADDR, (hipDeviceptr_t)p) != hipSuccess ||
+        hipPointerGetAttribute(&size, HIP_POINTER_ATTRIBUTE_RANGE_SIZE, (hipDeviceptr_t)p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return (const char *)p >= (const char *)start && (const char *)p + bytes <= (const char *)start + size;
+}
+
+// Copy k device columns (d_wide + off[k], n * esz[k] bytes) to the host
+// arrays dst[k] (nullable): pinned destinations by one DMA each, the others
+// into pinned staging and from there by host threads.  Waits for the stream.
+static int copy_columns(mcs_ctx *c, void *const *dst, const size_t *esz, const size_t *off, int k, size_t n)
+{
+    bool pinned[8] = {};
+    size_t soff[8] = {}, stot = 0;
+    for (int i = 0; i < k; ++i) {
+        if (!dst[i]) continue;
+        pinned[i] = host_pinned(dst[i], esz[i] * n);
+        if (!pinned[i]) { soff[i] = stot; stot += esz[i] * n; }
+    }
+    int r;
+    if (stot && (r = stage(c, stot))) return r;
+    const char *w = (const char *)c->d_wide.p;
+    for (int i = 0; i < k; ++i) {
+        if (!dst[i]) continue;
+        void *to = pinned[i] ? dst[i] : (void *)((char *)c->h_stage + soff[i]);
+        HIPCHK(c, hipMemcpyAsync(to, w + off[i], esz[i] * n, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!stot) return MCS_OK;
+    // staged columns as one byte range split over host threads
+    parallel_rows((int64_t)stot, [&](int64_t a, int64_t b) {
+        for (int i = 0; i < k; ++i) {
+            if (!dst[i] || pinned[i]) continue;
+            const int64_t s0 = (int64_t)soff[i], s1 = s0 + (int64_t)(esz[i] * n);
+            const int64_t lo = std::max(a, s0), hi = std::min(b, s1);
+            if (lo < hi) memcpy((char *)dst[i] + (lo - s0), (const char *)c->h_stage + lo, (size_t)(hi - lo));
+        }
+    });
+    return MCS_OK;
+}
+
 extern "C" {
 
 int mcs_create(int device, mcs_ctx **out)
@@ -866,10 +966,6 @@ int mcs_create(int device, mcs_ctx **out)
     c->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_rb[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_rb[1], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_rb[2], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_rb[3], hipEventDisableTiming) != hipSuccess ||
         hipHostMalloc((void **)&c->h_sum, 8 * sizeof(unsigned long long), hipHostMallocMapped) != hipSuccess ||
         hipHostGetDevicePointer((void **)&c->d_sum, c->h_sum, 0) != hipSuccess) {
         (void)mcs_destroy(c);
@@ -891,13 +987,12 @@ int mcs_destroy(mcs_ctx *c)
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     if (c->graph) (void)hipGraphDestroy(c->graph);
     DevBuf *all[] = {&c->d_cases, &c->d_bands, &c->d_chunk0, &c->d_tally_f, &c->d_tally_u,
-                     &c->d_ctl, &c->d_pband, &c->d_rdev, &c->d_roff, &c->d_pm, &c->d_pmf, &c->d_stokes};
+                     &c->d_ctl, &c->d_pband, &c->d_rdev, &c->d_roff, &c->d_pm, &c->d_pmf, &c->d_stokes, &c->d_wide};
     for (DevBuf *b : all) if (b->p) (void)hipFree(b->p);
     for (auto &b : c->d_rec) if (b.p) (void)hipFree(b.p);
     for (auto &b : c->d_out) if (b.p) (void)hipFree(b.p);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
-    for (auto &e : c->ev_rb) if (e) (void)hipEventDestroy(e);
     if (c->h_sum) (void)hipHostFree(c->h_sum);
     if (c->h_stage) (void)hipHostFree(c->h_stage);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -936,6 +1031,7 @@ int mcs_set_cases(mcs_ctx *c, const mcs_case *cases, int32_t n_cases, const mcs_
         DevBand &D = db[b];
         D.wvn = B.wvn; D.g = B.g; D.ssa_ice = B.ssa_ice; D.ssa_imp = B.ssa_imp;
         D.ext_cff = B.ext_cff; D.p_ext_imp = B.p_ext_imp; D.cdf = B.cdf;
+        D.snow_depth = B.snow_depth;
         D.iso = (B.g == 0.0);
         const double g = D.iso ? 0.5 : B.g;
         D.hg_a = (float)(1.0 / (2.0 * g)); D.hg_b = (float)(1.0 + g * g); D.hg_c = (float)(1.0 - g * g);
@@ -1545,54 +1641,41 @@ int mcs_read_records(mcs_ctx *c, const mcs_records *o, int64_t n)
     if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
     if (n == 0) return MCS_OK;
     (void)hipSetDevice(c->device);
-    const bool rep = c->last_mode == MCS_REPLAY;
-    const size_t rsz = rep ? 8 : 4;
-    // photon-order device arrays (n_scat, cond | band << 8, theta, phi, path)
-    // -> pinned staging, in up to 4 chunks whose copies run while host
-    // threads widen the chunks before them into the caller's arrays
+    // the device writes every requested column in its host layout, then each
+    // column is one copy: straight into the caller's array when that is
+    // pinned host memory, else through pinned staging
+    void *dst[8] = {o->condition, o->band, o->n_scat, o->theta_n, o->phi_n, o->path_length, o->wvn, o->snow_depth};
+    const size_t esz[8] = {1, 1, 8, 8, 8, 8, 8, 8};
+    size_t off[8] = {}, total = 0;
+    for (int k = 0; k < 8; ++k)
+        if (dst[k]) { off[k] = total; total += (esz[k] * (size_t)n + 255) & ~(size_t)255; }
+    if (!total) return MCS_OK;
     int r;
-    if ((r = stage(c, (size_t)n * (8 + 3 * rsz)))) return r;
-    uint32_t *s_ns = (uint32_t *)c->h_stage, *s_cb = s_ns + n;
-    char *s_f = (char *)(s_cb + n);
-    double *dst[3] = {o->theta_n, o->phi_n, o->path_length};
-    const int nch = n >= (1 << 18) ? 4 : 1;
-    const int64_t per = (n + nch - 1) / nch;
-    for (int q = 0; q < nch; ++q) {
-        const int64_t a = std::min(n, q * per), e = std::min(n, a + per);
-        if (e <= a) continue;
-        const size_t m = (size_t)(e - a);
-        HIPCHK(c, hipMemcpyAsync(s_ns + a, (const uint32_t *)c->d_out[0].p + a, 4 * m, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipMemcpyAsync(s_cb + a, (const uint32_t *)c->d_out[1].p + a, 4 * m, hipMemcpyDeviceToHost, c->stream));
-        for (int k = 0; k < 3; ++k)
-            if (dst[k])
-                HIPCHK(c, hipMemcpyAsync(s_f + ((size_t)k * n + a) * rsz, (const char *)c->d_out[2 + k].p + a * rsz,
-                                         rsz * m, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipEventRecord(c->ev_rb[q], c->stream));
-    }
-    for (int q = 0; q < nch; ++q) {
-        const int64_t a0 = std::min(n, q * per), e0 = std::min(n, a0 + per);
-        if (e0 <= a0) continue;
-        HIPCHK(c, hipEventSynchronize(c->ev_rb[q]));
-        parallel_rows(e0 - a0, [&](int64_t a, int64_t b) {
-            a += a0;
-            b += a0;
-            for (int64_t j = a; j < b; ++j) {
-                if (o->n_scat) o->n_scat[j] = s_ns[j];
-                if (o->condition) o->condition[j] = (int8_t)(s_cb[j] & 0xff);
-                if (o->band) o->band[j] = (uint8_t)(s_cb[j] >> 8);
-            }
-            for (int k = 0; k < 3; ++k) {
-                if (!dst[k]) continue;
-                if (rep) {
-                    memcpy(dst[k] + a, (const double *)(s_f + (size_t)k * n * rsz) + a, sizeof(double) * (size_t)(b - a));
-                } else {
-                    const float *src = (const float *)(s_f + (size_t)k * n * rsz);
-                    for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
-                }
-            }
-        });
-    }
-    return MCS_OK;
+    if ((r = ensure(c, c->d_wide, total))) return r;
+    char *w = (char *)c->d_wide.p;
+    WideRecords W;
+    W.condition = dst[0] ? (int8_t *)(w + off[0]) : nullptr;
+    W.band = dst[1] ? (uint8_t *)(w + off[1]) : nullptr;
+    W.n_scat = dst[2] ? (int64_t *)(w + off[2]) : nullptr;
+    W.theta_n = dst[3] ? (double *)(w + off[3]) : nullptr;
+    W.phi_n = dst[4] ? (double *)(w + off[4]) : nullptr;
+    W.path_length = dst[5] ? (double *)(w + off[5]) : nullptr;
+    W.wvn = dst[6] ? (double *)(w + off[6]) : nullptr;
+    W.snow_depth = dst[7] ? (double *)(w + off[7]) : nullptr;
+    const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
+    const int nc = (int)c->cases.size();
+    if (c->last_mode == MCS_REPLAY)
+        hipLaunchKernelGGL(k_widen_records<double>, dim3(g), dim3(256), 0, c->stream, (const DevCase *)c->d_cases.p,
+                           (const DevBand *)c->d_bands.p, nc, (uint64_t)n, (const uint32_t *)c->d_out[0].p,
+                           (const uint32_t *)c->d_out[1].p, (const double *)c->d_out[2].p,
+                           (const double *)c->d_out[3].p, (const double *)c->d_out[4].p, W);
+    else
+        hipLaunchKernelGGL(k_widen_records<float>, dim3(g), dim3(256), 0, c->stream, (const DevCase *)c->d_cases.p,
+                           (const DevBand *)c->d_bands.p, nc, (uint64_t)n, (const uint32_t *)c->d_out[0].p,
+                           (const uint32_t *)c->d_out[1].p, (const float *)c->d_out[2].p,
+                           (const float *)c->d_out[3].p, (const float *)c->d_out[4].p, W);
+    HIPCHK(c, hipGetLastError());
+    return copy_columns(c, dst, esz, off, 8, (size_t)n);
 }
 
 int mcs_read_stokes(mcs_ctx *c, double *q, double *u, double *v, int64_t n)
@@ -1603,25 +1686,25 @@ int mcs_read_stokes(mcs_ctx *c, double *q, double *u, double *v, int64_t n)
     if (n != c->last_n) return set_err(c, MCS_ERR_ARG, "n = %lld, last walk had %lld photons", (long long)n, (long long)c->last_n);
     if (n == 0) return MCS_OK;
     (void)hipSetDevice(c->device);
-    const bool rep = c->last_mode == MCS_REPLAY;
-    const size_t rsz = rep ? 8 : 4;
     int r;
-    if ((r = stage(c, 3 * rsz * (size_t)n))) return r;
-    HIPCHK(c, hipMemcpyAsync(c->h_stage, c->d_stokes.p, 3 * rsz * (size_t)n, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    double *dst[3] = {q, u, v};
-    parallel_rows(n, [&](int64_t a, int64_t b) {
-        for (int k = 0; k < 3; ++k) {
-            if (!dst[k]) continue;
-            if (rep) {
-                memcpy(dst[k] + a, (const double *)c->h_stage + (size_t)k * n + a, sizeof(double) * (size_t)(b - a));
-            } else {
-                const float *src = (const float *)c->h_stage + (size_t)k * n;
-                for (int64_t j = a; j < b; ++j) dst[k][j] = src[j];
-            }
-        }
-    });
-    return MCS_OK;
+    const size_t plane = (8 * (size_t)n + 255) & ~(size_t)255;
+    if ((r = ensure(c, c->d_wide, 3 * plane))) return r;
+    // the float64 planes are written plane-padded: widen each plane
+    for (int k = 0; k < 3; ++k) {
+        const unsigned g = (unsigned)std::min<int64_t>((n + 255) / 256, 2048);
+        double *o = (double *)((char *)c->d_wide.p + k * plane);
+        if (c->last_mode == MCS_REPLAY)
+            hipLaunchKernelGGL(k_widen_stokes<double>, dim3(g), dim3(256), 0, c->stream,
+                               (const double *)c->d_stokes.p + (size_t)k * n, (uint64_t)n, o);
+        else
+            hipLaunchKernelGGL(k_widen_stokes<float>, dim3(g), dim3(256), 0, c->stream,
+                               (const float *)c->d_stokes.p + (size_t)k * n, (uint64_t)n, o);
+    }
+    HIPCHK(c, hipGetLastError());
+    void *dst[3] = {q, u, v};
+    const size_t esz[3] = {8, 8, 8};
+    const size_t off[3] = {0, plane, 2 * plane};
+    return copy_columns(c, dst, esz, off, 3, (size_t)n);
 }
 
 int mcs_read_tallies(mcs_ctx *c, double *out, int32_t n_cases)

@@ -70,6 +70,7 @@ struct DevBand {
     // physical use the two-area rejection (DevCase::s0_exact)
     int32_t pm_exact;
     float fM;
+    double snow_depth;                   // record column of the band's photons [m]
 };
 
 // probability -> 32-bit threshold (bits < t has probability p)

@@ -117,6 +117,13 @@ def pack_polar(cases):
     return parr, np.ascontiguousarray(np.stack(tabs)), n
 
 
+class _HeldArrays(dict):
+    """A dict of numpy views that keeps the tensors owning their memory."""
+    def __init__(self):
+        super().__init__()
+        self.hold = []
+
+
 class GpuWalker:
     """One library context on ``device``."""
 
@@ -166,8 +173,6 @@ class GpuWalker:
                                             tabs.shape[0], n))
         self.cases = list(cases)
         self.n_photons = int(sum(c.n_photons for c in cases))
-        self._band_wvn = [np.array([1.0 / b.wvl for b in c.bands]) for c in cases]
-        self._band_depth = [np.array([c.tau_tot / b.ext_cff for b in c.bands]) for c in cases]
 
     def set_launch(self, block=256, blocks_per_cu=4, chunk=2048, refill_min=8,
                    coop_live=_lib.DEFAULT_COOP_LIVE):
@@ -243,11 +248,28 @@ class GpuWalker:
                      ('phi_n', np.float64), ('path_length', np.float64), ('wvn', np.float64),
                      ('snow_depth', np.float64))
 
+    def record_buffers(self, pinned=True):
+        """Empty record arrays for ``records(out=...)``, one per column of
+        RECORD_DTYPES, n_photons long.  ``pinned``: in page-locked host memory
+        (torch pin_memory), which the library fills by DMA straight from the
+        device; keep the returned dict alive as long as its arrays are used."""
+        n = self.n_photons
+        if not pinned:
+            return {k: np.empty(n, t) for k, t in self.RECORD_DTYPES}
+        import torch
+        of = {np.int8: torch.int8, np.uint8: torch.uint8, np.int64: torch.int64, np.float64: torch.float64}
+        out = _HeldArrays()
+        for k, t in self.RECORD_DTYPES:
+            ten = torch.empty(n, dtype=of[t], pin_memory=True)
+            out.hold.append(ten)
+            out[k] = ten.numpy()
+        return out
+
     def records(self, out=None):
         """Per-photon records in photon order: the CSV columns of mc3D:1626-1636
         plus the band index.  ``out``: a dict of arrays of those dtypes and
         the photon count to fill (a consumer that reads walk after walk can
-        keep its buffers; new arrays otherwise)."""
+        keep its buffers, best from record_buffers(); new arrays otherwise)."""
         n = self.n_photons
         if out is None:
             out = {k: np.empty(n, t) for k, t in self.RECORD_DTYPES}
@@ -257,19 +279,8 @@ class GpuWalker:
                 if a.dtype != t or a.shape != (n,) or not a.flags.c_contiguous:
                     raise ValueError('out[%r] must be a contiguous %s array of %d entries' % (k, np.dtype(t), n))
         R = McsRecords(*[out[k].ctypes.data for k in
-                         ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length')])
+                         ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'wvn', 'snow_depth')])
         self._check(lib().mcs_read_records(self._ctx, ctypes.byref(R), n))
-        wvn, depth = out['wvn'], out['snow_depth']
-        base = 0
-        for k, c in enumerate(self.cases):
-            sl = slice(base, base + c.n_photons)
-            if self._band_wvn[k].size == 1:          # one band: constant columns
-                wvn[sl] = self._band_wvn[k][0]
-                depth[sl] = self._band_depth[k][0]
-            else:
-                np.take(self._band_wvn[k], out['band'][sl], out=wvn[sl])
-                np.take(self._band_depth[k], out['band'][sl], out=depth[sl])
-            base += c.n_photons
         return out
 
     def stokes(self):

@@ -67,7 +67,7 @@ def test_struct_layouts_match_the_c_compiler(tmp_path):
         assert int(got[cname]) == ctypes.sizeof(cls), cname
         for f, _ in cls._fields_:
             assert int(got['%s.%s' % (cname, f)]) == getattr(cls, f).offset, (cname, f)
-    assert ctypes.sizeof(_lib.McsRecords) == 48
+    assert ctypes.sizeof(_lib.McsRecords) == 64
     assert ctypes.sizeof(_lib.McsStats) == 40
     assert ctypes.sizeof(_lib.McsDevOut) == 64
 

@@ -42,6 +42,35 @@ def test_batched_counts_are_consistent(walker):
         assert rec['band'][sl].max() < len(cases[k].bands)
 
 
+def test_record_columns_pinned_and_pageable_agree(walker):
+    """Every record column read into pinned buffers (one DMA per column)
+    equals the same read into ordinary numpy arrays (staged copies), and the
+    wvn / snow_depth columns are the photon's band values: 1 / wvl and
+    tau_tot / ext_cff of its case (mc3D:1489-1490, 1612)."""
+    cases, _ = small_sweep()
+    walker.set_cases(cases)
+    walker.walk(mode='native', seed=33, records=True)
+    a = walker.records()
+    buf = walker.record_buffers()
+    b = walker.records(out=buf)
+    for k, _ in walker.RECORD_DTYPES:
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+    n = np.array([c.n_photons for c in cases])
+    starts = np.concatenate([[0], np.cumsum(n)])
+    for k, c in enumerate(cases):
+        sl = slice(starts[k], starts[k + 1])
+        wv = np.array([1.0 / x.wvl for x in c.bands])
+        dp = np.array([c.tau_tot / x.ext_cff for x in c.bands])
+        np.testing.assert_array_equal(a['wvn'][sl], wv[a['band'][sl]])
+        np.testing.assert_array_equal(a['snow_depth'][sl], dp[a['band'][sl]])
+    # a second walk into the same pinned buffers overwrites every entry
+    walker.walk(mode='native', seed=34, records=True)
+    c2 = walker.records(out=buf)
+    d2 = walker.records()
+    for k, _ in walker.RECORD_DTYPES:
+        np.testing.assert_array_equal(c2[k], d2[k], err_msg=k)
+
+
 def test_first_case_of_batch_equals_lone_launch(walker):
     """Philox streams are keyed by (case, photon): case 0 of a batch and the
     same case launched alone walk identical photons (tail mode off, so that
