@@ -212,14 +212,15 @@ __device__ __forceinline__ void coop_local(const u32x4 &cur, float ca, float cb,
     L = Quat{ha * hc, -hb * hs, ha * hs, hb * hc};
 }
 
-// Steps per lane and round of the cooperative mode: the frame and depth scans
-// run once per round, so more steps per lane spread their cost (and give a
-// lane more independent work while the round's dependent chain runs).  Four
-// in the 96-VGPR build with the hand-over (the single-case launches: 2-5 %
-// faster than two on MI355X); two in the 80-VGPR build, where four would
-// spill in the cooperative rounds.
-template <bool HAND>
-constexpr int coop_k() { return HAND ? 4 : 2; }
+// Steps per lane and round of the cooperative mode, by the build's waves per
+// SIMD: the frame and depth scans run once per round, so more steps per lane
+// spread their cost (and give a lane more independent work while the round's
+// dependent chain runs), as far as the register budget holds them.  Measured
+// on MI355X for the 1e6-photon single case at 4 waves/SIMD (128 VGPRs):
+// 5 steps 0.704 ms, 4 steps 0.712, 6 steps 0.725 (spills), 8 steps slower
+// still; the 96-VGPR build (5 waves) with 4 steps 0.746.
+template <int WPE>
+constexpr int coop_k() { return WPE <= 4 ? 5 : (WPE == 5 ? 4 : 2); }
 
 __device__ __forceinline__ Quat qconj(const Quat &q) { return Quat{q.w, -q.x, -q.y, -q.z}; }
 __device__ __forceinline__ float frame_uz(const Quat &q) { return fmaf(-2.f, fmaf(q.x, q.x, q.y * q.y), 1.f); }
@@ -454,7 +455,7 @@ __device__ __forceinline__ void coop_donate(BlockState &bs, NativeWalker &w, int
 // The cooperative phase of a wave: every photon the wave still holds is
 // finished (or, with HAND, handed over) when it returns.  `tab`: coop_live
 // entries; `map`: coop_live group slots.
-template <bool HAND>
+template <bool HAND, int K>
 __device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int &state, const KParams &P,
                                            CoopEntry *tab, int *map, unsigned long long *coop_steps,
                                            unsigned long long *moved, int wave)
@@ -472,7 +473,7 @@ __device__ __forceinline__ void coop_phase(BlockState &bs, NativeWalker &w, int 
     cg.lg = 0;
     cg.ent = 0;
     for (uint32_t round = 1; __ballot(state == LIVE); ++round) {
-        coop_round<coop_k<HAND>()>(w, state, P, tab, map, me, cg, coop_steps);
+        coop_round<K>(w, state, P, tab, map, me, cg, coop_steps);
         if constexpr (HAND) {
             if ((round & 1u) == 0u) coop_donate(bs, w, state, tab, me, cg, coop_steps, moved, wave);
         }

@@ -279,17 +279,16 @@ __global__ void k_gather(const float *xd, const float *td, const int32_t *idx, i
         T[(size_t)r * np + c] = c < n_out ? td[s * n_out + c] : 0.f;
 }
 
-// dY = 2 w (y - t) / rows (bf16) and *loss_out = sum w (y - t)^2 / rows.
-// Four columns per thread (np is a multiple of 64); each block leaves its
-// partial sum in part[], and the last block to finish (ticket) adds them in
-// block order and re-arms the ticket, so the loss is deterministic.
+// dY = 2 w (y - t) / rows (bf16); part[block] = that block's share of
+// sum w (y - t)^2.  Four columns per thread (np is a multiple of 64).
+// k_loss_sum then adds the partials in block order (deterministic; no
+// cross-block fence, which costs an L2 write-back on a multi-XCD part).
 constexpr int LOSS_BLOCKS = 1024;
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 __global__ void __launch_bounds__(256) k_loss(const float *Y, const float *T, const float *w, int rows, int np,
-                                              bf16 *dY, float *part, unsigned *ticket, float *loss_out)
+                                              bf16 *dY, float *part)
 {
     __shared__ float red[256];
-    __shared__ bool last;
     const float inv = 1.f / rows;
     const uint32_t np4 = (uint32_t)np >> 2, n4 = (uint32_t)rows * np4;
     float acc = 0.f;
@@ -312,27 +311,22 @@ __global__ void __launch_bounds__(256) k_loss(const float *Y, const float *T, co
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&part[blockIdx.x], red[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        last = atomicAdd(ticket, 1u) == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    float sum = 0.f;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x)
-        sum += __hip_atomic_load(&part[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    red[threadIdx.x] = sum;
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// *loss_out = (sum of part[0..nb)) / rows, one block of 256.
+__global__ void __launch_bounds__(256) k_loss_sum(const float *part, int nb, int rows, float *loss_out)
+{
+    __shared__ float red[256];
+    float s0 = 0.f;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) s0 += part[b];
+    red[threadIdx.x] = s0;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        *loss_out = red[0] * inv;
-        __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    if (threadIdx.x == 0) *loss_out = red[0] / rows;
 }
 
 // db[c] += column sums of dZ (db zeroed first).  Block (64, 4): a wave per
@@ -420,7 +414,6 @@ struct mcs_mlp {
     float *Y = nullptr, *T = nullptr, *w = nullptr, *yout = nullptr, *xin = nullptr, *loss = nullptr;
     int32_t *idx = nullptr;
     float *lpart = nullptr;          // k_loss: per-block partial sums (LOSS_BLOCKS)
-    unsigned *lticket = nullptr;     // k_loss: blocks finished (0 between launches)
     float *xd = nullptr, *td = nullptr;
     int64_t n_data = 0;
     double last_ms = 0.0;
@@ -550,7 +543,8 @@ static void backward_pass(mcs_mlp *c, int rows, float *loss_out)
     const int colsum_blocks_y = (rows + COLSUM_ROWS - 1) / COLSUM_ROWS;
     const int loss_blocks = (int)std::min<int64_t>(LOSS_BLOCKS, ((int64_t)rows * (c->np / 4) + 255) / 256);
     hipLaunchKernelGGL(k_loss, dim3(loss_blocks), dim3(256), 0, c->stream, c->Y, c->T, c->w, rows, c->np, c->dY,
-                       c->lpart, c->lticket, loss_out);
+                       c->lpart);
+    hipLaunchKernelGGL(k_loss_sum, dim3(1), dim3(256), 0, c->stream, c->lpart, loss_blocks, rows, loss_out);
     // output layer
     const Layer &lo = c->layers[L];
     const bf16 *Aprev = c->A[L - 1];
@@ -649,8 +643,7 @@ int mcs_mlp_create(int device, const mcs_mlp_dims *dims, mcs_mlp **out)
               dalloc(&c->T, MB * c->np) == hipSuccess && dalloc(&c->w, c->np) == hipSuccess &&
               dalloc(&c->yout, MB * dims->n_out) == hipSuccess && dalloc(&c->xin, MB * dims->n_in) == hipSuccess &&
               dalloc(&c->loss, 1024) == hipSuccess && dalloc(&c->idx, MB) == hipSuccess &&
-              dalloc(&c->lpart, LOSS_BLOCKS) == hipSuccess && dalloc(&c->lticket, 1) == hipSuccess &&
-              hipMemset(c->lticket, 0, sizeof(unsigned)) == hipSuccess;
+              dalloc(&c->lpart, LOSS_BLOCKS) == hipSuccess;
     for (int l = 0; ok && l < L; ++l) {
         float *z = nullptr;
         bf16 *a = nullptr;
@@ -681,7 +674,7 @@ int mcs_mlp_destroy(mcs_mlp *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     void *bufs[] = {c->p, c->gr, c->m, c->v, c->mask, c->pb_index, c->wb, c->X, c->dY, c->dZa, c->dZb,
                     c->Y, c->T, c->w, c->yout, c->xin, c->loss, c->idx, c->xd, c->td,
-                    c->fmu, c->fsd, c->oscale, c->oshift, c->ooff, c->lpart, c->lticket};
+                    c->fmu, c->fsd, c->oscale, c->oshift, c->ooff, c->lpart};
     for (void *b : bufs) if (b) (void)hipFree(b);
     for (float *z : c->Z) if (z) (void)hipFree(z);
     for (bf16 *a : c->A) if (a) (void)hipFree(a);

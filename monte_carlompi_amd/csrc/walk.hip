@@ -355,9 +355,9 @@ __device__ __forceinline__ void set_priority(uint32_t i)
 }
 
 // WPE: waves per SIMD the register allocation targets (the launch picks the
-// instance that fits its shape).  The hand-over of cooperative-tail photons
-// between waves is left out of the 6-wave build: at 80 VGPRs its code would
-// spill in the step loop.
+// instance that fits its shape: 4 -> 128 VGPRs, no spills; 5 -> 96; 6 -> 80).
+// The hand-over of cooperative-tail photons between waves is left out of the
+// 6-wave build: at 80 VGPRs its code would spill in the step loop.
 template <class W, int WPE, bool DIRECT = false>
 __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_walk(const KParams P)
@@ -418,7 +418,7 @@ k_walk(const KParams P)
                     to_coop = true;   // below, with the hand-over
                 } else {
                     __builtin_amdgcn_s_setprio(3);
-                    coop_phase<false>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
+                    coop_phase<false, coop_k<WPE>()>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
                                       &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
                     retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
                 }
@@ -464,7 +464,7 @@ k_walk(const KParams P)
                 if (to_coop) {
                     __builtin_amdgcn_s_setprio(3);
                     await_own_posts(bs, P, wave);
-                    coop_phase<true>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
+                    coop_phase<true, coop_k<WPE>()>(bs, w, state, P, coop_tab + wave * P.coop_live, coop_map + wave * P.coop_live,
                                      &bs.tal[my_slot].u[U_COOP], &bs.tal[my_slot].u[U_MOVED], wave);
                     // every photon the wave held is finished or handed over
                     retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
@@ -1300,13 +1300,18 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
             hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
     } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
-        // shapes asking for more than 5 waves per SIMD get the 80-VGPR instance
-        // (6 waves per SIMD; its spills sit outside the step loop)
+        // the instance whose register budget fits the shape's waves per SIMD:
+        // <= 4 (1024-thread blocks, one per CU) 128 VGPRs, 5 96, more 80 (the
+        // 6-wave build's spills sit outside the step loop)
         const int waves_per_simd = (c->launch.blocks_per_cu * (block / 64) + 3) / 4;
         if (waves_per_simd > 5 && P.rec_direct)
             hipLaunchKernelGGL((k_walk<NativeWalker, 6, true>), dim3(grid), dim3(block), tab, c->stream, P);
         else if (waves_per_simd > 5)
             hipLaunchKernelGGL((k_walk<NativeWalker, 6>), dim3(grid), dim3(block), tab, c->stream, P);
+        else if (waves_per_simd <= 4 && P.rec_direct)
+            hipLaunchKernelGGL((k_walk<NativeWalker, 4, true>), dim3(grid), dim3(block), tab, c->stream, P);
+        else if (waves_per_simd <= 4)
+            hipLaunchKernelGGL((k_walk<NativeWalker, 4>), dim3(grid), dim3(block), tab, c->stream, P);
         else if (P.rec_direct)
             hipLaunchKernelGGL((k_walk<NativeWalker, 5, true>), dim3(grid), dim3(block), tab, c->stream, P);
         else

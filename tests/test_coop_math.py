@@ -214,7 +214,7 @@ def test_two_steps_per_lane_give_every_frame_and_depth():
                 np.testing.assert_allclose(zb_k - dtb[j] * uzb[j], za, atol=1e-10)
 
 
-@pytest.mark.parametrize('K', [2, 3, 4])
+@pytest.mark.parametrize('K', [2, 3, 4, 5])
 def test_k_steps_per_lane_give_every_frame_and_depth(K):
     """The kernel's round with K steps per lane (coop_round<K>): lane j of a
     group walks steps K j + 1 .. K j + K; the scan runs over the product of

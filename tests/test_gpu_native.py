@@ -153,7 +153,9 @@ def test_records_independent_of_launch_shape_and_graph(walker):
     n = 100_000
     walker.set_cases([gpu_case(n, 20.0, 30.0, True, 0.5, g=0.87, ssa=0.995)])
     ref = None
-    shapes = [(256, 4, 2048, 8, 0), (64, 8, 64, 1, 0), (512, 2, 8192, 32, 0), (1024, 1, 4096, 64, 0)]
+    # 4, 2, 4, 4, 5 and 6 waves per SIMD: every register-budget instance
+    shapes = [(256, 4, 2048, 8, 0), (64, 8, 64, 1, 0), (512, 2, 8192, 32, 0), (1024, 1, 4096, 64, 0),
+              (256, 5, 2048, 8, 0), (256, 6, 1024, 8, 0)]
     for shape in shapes:
         walker.set_launch(*shape)
         for graph in (False, True):
