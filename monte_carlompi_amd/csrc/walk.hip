@@ -288,9 +288,12 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
 // LDS after the tables: 16 owner lanes and 64 packets per wave.
 __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wave)
 {
-    extern __shared__ float pm_lds_tab[];
+    extern __shared__ __align__(16) float pm_lds_tab[];
     int *const owners = reinterpret_cast<int *>(pm_lds_tab + P.pm_lds_floats);
-    PolarPacket *const pk = reinterpret_cast<PolarPacket *>(owners + (blockDim.x >> 6) * 16) + wave * 64;
+    // packets 16-byte aligned after the tables and owner lanes (the launch
+    // reserves 16 bytes for this)
+    const int pk_off = (P.pm_lds_floats + (int)(blockDim.x >> 6) * 16 + 3) & ~3;
+    PolarPacket *const pk = reinterpret_cast<PolarPacket *>(pm_lds_tab + pk_off) + wave * 64;
     int *const own = owners + wave * 16;
     const int lane = lane_id();
     for (;;) {
@@ -325,13 +328,22 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
             // one exit is an event (rare), the step cap or the last packet.
             // After an event the next round builds packets from the photon's
             // step count then (a bottom re-emission walks a step of its own).
+            // The next step's packet is loaded while this one's step runs
+            // (two packets in registers, their roles alternating).
             const PolarPacket *const mine = pk + lane_rank(live) * G;
+            PolarPacket a = mine[0];
             int k = 0;
             bool ev;
-            do {
-                ev = w.polar_move(P, mine[k]);
+            for (;;) {
+                const PolarPacket b = mine[min(k + 1, G - 1)];
+                ev = w.polar_move(P, a);
                 ++k;
-            } while (!ev && k < G && w.i < P.max_steps);
+                if (ev || k >= G || w.i >= P.max_steps) break;
+                a = mine[min(k + 1, G - 1)];
+                ev = w.polar_move(P, b);
+                ++k;
+                if (ev || k >= G || w.i >= P.max_steps) break;
+            }
             const bool d = ev && w.polar_event(P, mine[k - 1].dtau, mine[k - 1].w);
             if (d || w.i >= P.max_steps) {
                 if (!d) w.cond = 0;
@@ -376,7 +388,7 @@ k_walk(const KParams P)
     if constexpr (W::STOKES && W::COOP == false && sizeof(typename W::Real) == 4) {
         // polarised native walk: phase-matrix tables into dynamic LDS
         if (P.pm_lds_floats > 0) {
-            extern __shared__ float pm_lds_tab[];
+            extern __shared__ __align__(16) float pm_lds_tab[];
             for (int k = threadIdx.x; k < P.pm_lds_floats; k += blockDim.x) pm_lds_tab[k] = P.pmf[k];
         }
     }
@@ -1293,7 +1305,7 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
         // polar_tail; tables up to 16 KB ride in LDS
         // + the tail mode's owner lanes and packets (16 ints and 64 packets a wave)
         const size_t lds = (size_t)P.pm_lds_floats * sizeof(float) +
-                           (size_t)(block / 64) * (16 * sizeof(int) + 64 * sizeof(PolarPacket));
+                           (size_t)(block / 64) * (16 * sizeof(int) + 64 * sizeof(PolarPacket)) + 16;
         if (P.rec_direct)
             hipLaunchKernelGGL((k_walk<PolarWalker, 3, true>), dim3(grid), dim3(block), lds, c->stream, P);
         else

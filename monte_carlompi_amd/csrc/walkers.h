@@ -45,15 +45,18 @@ __device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v
 // sampler): free path and absorption word from the step's main Philox block,
 // cos theta, sin theta and the phase matrix there, the first 2 phi candidate
 // (with the sines and cosines of phi and 2 phi) and its acceptance uniform.
-// One lane can build it for any step of any photon.
-struct PolarPacket {
+// One lane can build it for any step of any photon.  80 bytes, 16-aligned:
+// the tail mode reads a packet from LDS as five 16-byte loads.
+struct alignas(16) PolarPacket {
     float dtau;
     uint32_t w;
     float ct, r2, acc;
     float p11, p12, p22, p33, p43, p44;
     int seg;
     float sint, sinp, cosp, c2r, s2r;
+    float pad;
 };
+static_assert(sizeof(PolarPacket) == 80, "PolarPacket: five 16-byte LDS loads");
 
 // sin and cos of phi and of 2 phi for 2 phi = r2 revolutions (one
 // transcendental pair; no implicit contraction, so every caller rounds alike)
@@ -512,13 +515,13 @@ struct NativeWalkerT {
         // instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1 (scattering
         // plane close to the meridian plane) would turn an FP32 rounding of
         // cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 = 0.
-        float ci = 0.f, si = 1.f;
-        if (den != 0.f) {
-            const float rd = __builtin_amdgcn_rcpf(den);
-            const float sg = (rev2 == 1.f) ? 1.f : -1.f;
-            ci = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
-            si = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
-        }
+        // (straight-line: both values are formed and den = 0 selects 0 and 1)
+        const float rd = __builtin_amdgcn_rcpf(den);
+        const float sg = (rev2 == 1.f) ? 1.f : -1.f;
+        const float ci0 = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
+        const float si0 = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
+        const float ci = (den != 0.f) ? ci0 : 0.f;
+        const float si = (den != 0.f) ? si0 : 1.f;
         const float cr = fmaf(-2.f * si, si, 1.f);     // cos(-2 i_2) = 1 - 2 sin^2 i_2
         const float sr = -2.f * ci * si;               // sin(-2 i_2)
         const float inv = __builtin_amdgcn_rcpf(Ic);
