@@ -331,18 +331,18 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
             // The next step's packet is loaded while this one's step runs
             // (two packets in registers, their roles alternating).
             const PolarPacket *const mine = pk + lane_rank(live) * G;
+            // (each step adds one to i: the step cap bounds the steps too)
+            const int kmax = (int)min((uint32_t)G, P.max_steps > w.i ? P.max_steps - w.i : 0u);
             PolarPacket a = mine[0];
             int k = 0;
-            bool ev;
-            for (;;) {
+            bool ev = false;
+            while (k < kmax) {
                 const PolarPacket b = mine[min(k + 1, G - 1)];
                 ev = w.polar_move(P, a);
-                ++k;
-                if (ev || k >= G || w.i >= P.max_steps) break;
+                if (++k >= kmax || ev) break;
                 a = mine[min(k + 1, G - 1)];
                 ev = w.polar_move(P, b);
-                ++k;
-                if (ev || k >= G || w.i >= P.max_steps) break;
+                if (++k >= kmax || ev) break;
             }
             const bool d = ev && w.polar_event(P, mine[k - 1].dtau, mine[k - 1].w);
             if (d || w.i >= P.max_steps) {

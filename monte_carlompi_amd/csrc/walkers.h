@@ -54,7 +54,7 @@ struct alignas(16) PolarPacket {
     float p11, p12, p22, p33, p43, p44;
     int seg;
     float sint, sinp, cosp, c2r, s2r;
-    float pad;
+    float omc2;                          // max(1 - ct^2, 0): the rotation back uses it
 };
 static_assert(sizeof(PolarPacket) == 80, "PolarPacket: five 16-byte LDS loads");
 
@@ -486,14 +486,14 @@ struct NativeWalkerT {
         const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        stokes_apply(cost, rev2, __builtin_amdgcn_cosf(rev2), __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12,
-                     p22, p33, p43, p44);
+        stokes_apply(cost, fmaxf(fmaf(-cost, cost, 1.f), 0.f), rev2, __builtin_amdgcn_cosf(rev2),
+                     __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12, p22, p33, p43, p44);
     }
 
     // (mx0, my0, mz0): the direction before the scattering; the photon's
     // direction members hold the one after it
-    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions)
-    __device__ __forceinline__ void stokes_apply(float cost, float rev2, float c2r, float s2r, float mx0, float my0,
+    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions); omc2 = max(1 - cost^2, 0)
+    __device__ __forceinline__ void stokes_apply(float cost, float omc2, float rev2, float c2r, float s2r, float mx0, float my0,
                                                  float mz0, float p11, float p12, float p22, float p33, float p43,
                                                  float p44)
     {
@@ -507,8 +507,7 @@ struct NativeWalkerT {
         const float Uc = fmaf(Us, p33, -sV * p43);
         const float Vc = fmaf(Us, p43, sV * p44);
         const float muz_n = muz;
-        const float den = __builtin_amdgcn_sqrtf(fmaxf(fmaf(-cost, cost, 1.f), 0.f) *
-                                                 fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
+        const float den = __builtin_amdgcn_sqrtf(omc2 * fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
         // i_2 in [0, pi]: cos i_2 as the reference forms it; sin i_2 >= 0 from
         // its exact identity (den sin i_2)^2 = (e_z . (u_0 x u_n))^2 -- the
         // Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2 - cos^2 --
@@ -576,6 +575,7 @@ struct NativeWalkerT {
         o.r2 = 2.f * u01f(pb.y);
         o.acc = u01f(pb.z);
         o.sint = __builtin_amdgcn_sqrtf(fmaf(-ct, ct, 1.f));
+        o.omc2 = fmaxf(fmaf(-ct, ct, 1.f), 0.f);
         phi_trig(o.r2, o.sinp, o.cosp, o.c2r, o.s2r);
     }
 
@@ -646,7 +646,7 @@ struct NativeWalkerT {
             muy = sg * sint * sinp;
             muz = sg * cost;
         }
-        stokes_apply(cost, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        stokes_apply(cost, k.omc2, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
         const float dtau = k.dtau;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;

@@ -196,6 +196,32 @@ def traffic(kernel, fetch, write, cal_fetch, cal_write, mib=256):
                 hbm_bytes_per_launch=fetch_b + write_b)
 
 
+def kernel_table(stats_csv, fetch, write, mfma, cal_fetch, cal_write, mib=256, top=12):
+    """Per kernel of a kernel-trace run: calls and average time, HBM bytes per
+    dispatch (FETCH_SIZE / WRITE_SIZE scaled by a copy of known size), BF16
+    MFMA work (SQ_INSTS_VALU_MFMA_MOPS_BF16, 512 flops each) and its rate over
+    the average time."""
+    known_kib = mib * 1024.0
+    f_scale = known_kib / _pick(counter_means(cal_fetch, 'FETCH_SIZE'), 'copyBuffer')[1][0]
+    w_scale = known_kib / _pick(counter_means(cal_write, 'WRITE_SIZE'), 'copyBuffer')[1][0]
+    cf, cw = counter_means(fetch, 'FETCH_SIZE'), counter_means(write, 'WRITE_SIZE')
+    mo = counter_means(mfma, 'SQ_INSTS_VALU_MFMA_MOPS_BF16')
+    rows = []
+    for r in sorted(kernel_stats(stats_csv), key=lambda r: -r['pct'])[:top]:
+        name = r['name']
+        def get(m, sc=1.0):
+            hit = [v for k, v in m.items() if k == name or k.startswith(name) or name.startswith(k)]
+            return hit[0][0] * sc if hit else None
+        fb, wb = get(cf, f_scale * 1024), get(cw, w_scale * 1024)
+        mops = get(mo)
+        tf = mops * 512.0 / (r['avg_ns'] * 1e-9) / 1e12 if mops else None
+        rows.append(dict(kernel=name, calls=r['calls'], avg_us=r['avg_ns'] / 1e3, share=r['pct'],
+                         fetch_mb=fb / 1e6 if fb is not None else None,
+                         write_mb=wb / 1e6 if wb is not None else None,
+                         bf16_tflops=tf, hbm_gbs=((fb or 0) + (wb or 0)) / (r['avg_ns'] * 1e-9) / 1e9))
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest='cmd', required=True)
@@ -216,6 +242,10 @@ def main():
         t.add_argument('--' + k.replace('_', '-'), dest=k, required=True)
     t.add_argument('--mib', type=int, default=256)
     t.add_argument('--json')
+    kt = sub.add_parser('kernels')
+    for k in ('stats', 'fetch', 'write', 'mfma', 'cal_fetch', 'cal_write'):
+        kt.add_argument('--' + k.replace('_', '-'), dest=k, required=True)
+    kt.add_argument('--json')
     a = ap.parse_args()
     if a.cmd == 'copy':
         copy_kernel(a.mib)
@@ -244,6 +274,17 @@ def main():
                 med['launch'] = [int(v) for v in a.launch.split(',')]
             with open(a.json, 'w') as fh:
                 json.dump(med, fh, indent=1)
+    elif a.cmd == 'kernels':
+        rows = kernel_table(a.stats, a.fetch, a.write, a.mfma, a.cal_fetch, a.cal_write)
+        for r in rows:
+            print('%-48s %5d calls %10.1f us %6.2f %%  fetch %s MB  write %s MB  %s TFLOP/s  %7.0f GB/s' % (
+                r['kernel'][:48], r['calls'], r['avg_us'], r['share'],
+                '%.2f' % r['fetch_mb'] if r['fetch_mb'] is not None else '-',
+                '%.2f' % r['write_mb'] if r['write_mb'] is not None else '-',
+                '%.1f' % r['bf16_tflops'] if r['bf16_tflops'] else '-', r['hbm_gbs']))
+        if a.json:
+            with open(a.json, 'w') as fh:
+                json.dump(rows, fh, indent=1)
     elif a.cmd == 'stats':
         for r in kernel_stats(a.csv):
             print('%-60s calls %4d  avg %12.1f ns  min %12.1f  max %12.1f  %6.2f%%' % (
