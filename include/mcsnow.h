@@ -191,6 +191,34 @@ int mcs_sync(mcs_ctx *ctx);
 /* replaces: comm.gather + flatten_list (parallelize.py:17-26, mc3D:1618) */
 int mcs_read_records(mcs_ctx *ctx, const mcs_records *out, int64_t n_photons);
 
+/* Host record columns for the record file (photon order, n each; q, u, v
+ * all set for the aspherical record or all NULL). */
+typedef struct {
+    const int8_t *condition;
+    const double *wvn;
+    const double *theta_n;
+    const double *phi_n;
+    const int64_t *n_scat;
+    const double *path_length;
+    const double *snow_depth;
+    const double *q, *u, *v;
+} mcs_text_records;
+
+/* replaces: the record-file loop of run() (mc3D:1621-1648): `header`, then
+ * per photon '%d %r %r %r %d %r %r\n' (condition, wvn, theta_n, phi_n,
+ * n_scat, path_length, snow_depth; aspherical: then ' %r %r %r' of Q_n, U_n,
+ * V_n before the newline), each float as Python's shortest round-trip repr
+ * (numpy float64's %r before numpy 2).  Formats on up to 16 host threads;
+ * MCS_ERR_ARG for bad arguments or a file that cannot be written. */
+int mcs_write_records_text(const char *path, const char *header, const mcs_text_records *records,
+                           int64_t n_photons);
+
+/* Python repr of n doubles, each followed by '\n', into out (cap bytes; 25
+ * per value always suffice).  Returns the bytes written, -1 if cap is short.
+ * No counterpart in the reference: the float formatting of the record file,
+ * exposed for checking against CPython. */
+int64_t mcs_repr_doubles(const double *x, int64_t n, char *out, int64_t cap);
+
 /* Caller-owned DEVICE arrays that mcs_walk_noalloc writes (photon order). */
 typedef struct {
     double *tallies;           /* n_cases x MCS_TALLY_F64 (mcs_read_tallies)  */

@@ -30,7 +30,7 @@ EXPORTS = ['mcs_create', 'mcs_destroy', 'mcs_last_error', 'mcs_device_cus', 'mcs
            'mcs_mlp_create', 'mcs_mlp_destroy', 'mcs_mlp_last_error', 'mcs_mlp_param_count',
            'mcs_mlp_set_params', 'mcs_mlp_get_params', 'mcs_mlp_forward', 'mcs_mlp_last_ms',
            'mcs_mlp_grads', 'mcs_mlp_set_dataset', 'mcs_mlp_train_steps', 'mcs_mlp_set_serving',
-           'mcs_mlp_serve']
+           'mcs_mlp_serve', 'mcs_write_records_text', 'mcs_repr_doubles']
 MCS_FEATURES_IDENTITY = 0
 MCS_FEATURES_SNOW_BRF = 1
 
@@ -83,6 +83,12 @@ class McsPolarCase(ctypes.Structure):
 class McsRecords(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in
                 ('condition', 'band', 'n_scat', 'theta_n', 'phi_n', 'path_length', 'wvn', 'snow_depth')]
+
+
+class McsTextRecords(ctypes.Structure):
+    """Host record columns for mcs_write_records_text (mcs_text_records)."""
+    _fields_ = [(k, ctypes.c_void_p) for k in
+                ('condition', 'wvn', 'theta_n', 'phi_n', 'n_scat', 'path_length', 'snow_depth', 'q', 'u', 'v')]
 
 
 class McsDevOut(ctypes.Structure):
@@ -173,6 +179,9 @@ def lib():
         'mcs_mlp_train_steps': ([P, P, ctypes.c_int32, ctypes.c_int32, P, ctypes.POINTER(McsAdam), P], I),
         'mcs_mlp_set_serving': ([P, ctypes.c_int32, P, P, P, P, P], I),
         'mcs_mlp_serve': ([P, P, ctypes.c_int64, P], I),
+        'mcs_write_records_text': ([ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(McsTextRecords),
+                                    ctypes.c_int64], I),
+        'mcs_repr_doubles': ([P, ctypes.c_int64, P, ctypes.c_int64], ctypes.c_int64),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)

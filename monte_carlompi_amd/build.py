@@ -17,7 +17,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, 'csrc')
 INCLUDE = os.path.join(ROOT, 'include')
 LIB = os.path.join(HERE, 'libmcsnow.so')
-SOURCES = ['walk.hip', 'mlp.hip']
+SOURCES = ['walk.hip', 'mlp.hip', 'records_text.hip']
 ARCH = os.environ.get('MCS_OFFLOAD_ARCH', 'gfx950')
 
 HIPCC_FLAGS = ['--offload-arch=' + ARCH, '-O3', '-std=c++17', '-fPIC', '-shared',

@@ -31,14 +31,14 @@ constexpr int K0 = 32;                                    // padded input width
 
 enum { EPI_F32 = 0, EPI_SILU = 1, EPI_DSILU = 2, EPI_SILU_NOZ = 3, EPI_ADD_F32 = 4, EPI_EXP = 5 };
 
-__device__ __forceinline__ float sigm(float z) { return 1.f / (1.f + __expf(-z)); }
+__device__ __forceinline__ float sigm(float z) { return __builtin_amdgcn_rcpf(1.f + __expf(-z)); }
 
 struct GemmArgs {
     const bf16 *A;  // logical A [M x K]; stored [M][lda] or (A_T) [K][lda]
     const bf16 *B;  // logical B [N x K] (C = A B^T); stored [N][ldb] or (B_T) [K][ldb]
     int M, N, K, lda, ldb;
     const float *bias;  // [N] or null
-    const float *oscale, *oshift, *ooff;  // EPI_EXP: C = exp(oscale v + oshift) - ooff per column
+    const float *oscale, *oshift, *ooff;  // EPI_EXP: C = 2^(oscale v + oshift) - ooff per column (log2 e folded in)
     float *C;           // EPI_F32: C [M][ldc]; EPI_SILU: Z [M][ldc]
     bf16 *Cb;           // EPI_SILU / EPI_SILU_NOZ: silu(z) [M][ldcb]; EPI_DSILU: dZ [M][ldcb]
     const float *Zin;   // EPI_DSILU: pre-activation [M][ldc]
@@ -159,7 +159,18 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(const GemmArgs g)
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;        // quadrant of the 128 x 128 tile
-    const int m0 = blockIdx.y * FM, n0 = blockIdx.x * FN;
+    // XCD-aware tiles: workgroups go to the 8 XCDs round-robin by linear id,
+    // each XCD with its own L2, so the column tiles of one row block are
+    // given ids on the same XCD (else each of them fetches the block's rows
+    // of A from HBM again)
+    int bx = blockIdx.x, by = blockIdx.y;
+    if ((gridDim.y & 7u) == 0u) {
+        const int b = blockIdx.y * gridDim.x + blockIdx.x;
+        const int s = b >> 3;
+        by = (s / (int)gridDim.x) * 8 + (b & 7);
+        bx = s % (int)gridDim.x;
+    }
+    const int m0 = by * FM, n0 = bx * FN;
     // each thread moves 2 x 16 B of A and 2 x 16 B of B per stage:
     // rows r = t / 4 and t / 4 + 64, k chunk (t % 4) * 8
     const int lr = t >> 2, lk = (t & 3) * 8;
@@ -221,7 +232,7 @@ __global__ void __launch_bounds__(256) k_gemm_fwd(const GemmArgs g)
                 if (m >= g.M) continue;
                 const float v = acc[i][j][r] + bias;
                 if (EPI == EPI_F32) g.C[(size_t)m * g.ldc + n] = v;
-                else if (EPI == EPI_EXP) g.C[(size_t)m * g.ldc + n] = expf(fmaf(osc, v, osh)) - oof;
+                else if (EPI == EPI_EXP) g.C[(size_t)m * g.ldc + n] = __builtin_amdgcn_exp2f(fmaf(osc, v, osh)) - oof;
                 else g.Cb[(size_t)m * g.ldcb + n] = (bf16)(v * sigm(v));
             }
         }
@@ -800,8 +811,16 @@ int mcs_mlp_set_serving(mcs_mlp *c, int32_t feature_map, const float *feat_mu, c
     for (int k = 0; k < nf && feature_map != MCS_FEATURES_IDENTITY; ++k) { mu[k] = feat_mu[k]; sd[k] = feat_sd[k]; }
     MCHK(c, hipMemcpy(c->fmu, mu.data(), K0 * 4, hipMemcpyHostToDevice));
     MCHK(c, hipMemcpy(c->fsd, sd.data(), K0 * 4, hipMemcpyHostToDevice));
-    MCHK(c, hipMemcpy(c->oscale, out_scale, (size_t)no * 4, hipMemcpyHostToDevice));
-    MCHK(c, hipMemcpy(c->oshift, out_shift, (size_t)no * 4, hipMemcpyHostToDevice));
+    // exp(s v + h) = 2^(s' v + h') with s' = s log2(e), h' = h log2(e): the
+    // epilogue is one FMA and the hardware base-2 exponential
+    std::vector<float> s2(no), h2(no);
+    const double log2e = 1.0 / std::log(2.0);
+    for (int k = 0; k < no; ++k) {
+        s2[k] = (float)((double)out_scale[k] * log2e);
+        h2[k] = (float)((double)out_shift[k] * log2e);
+    }
+    MCHK(c, hipMemcpy(c->oscale, s2.data(), (size_t)no * 4, hipMemcpyHostToDevice));
+    MCHK(c, hipMemcpy(c->oshift, h2.data(), (size_t)no * 4, hipMemcpyHostToDevice));
     MCHK(c, hipMemcpy(c->ooff, out_offset, (size_t)no * 4, hipMemcpyHostToDevice));
     c->feature_map = feature_map;
     c->serving = true;

@@ -640,11 +640,15 @@ struct NativeWalkerT {
         const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
         muz = fmaf(-s2 * st, cosp, muz * cost);
         mux = gx; muy = gy;
-        if (s2 < 1e-7f) {
-            const float sg = mz0 > 0.f ? 1.f : -1.f;
-            mux = sint * cosp;
-            muy = sg * sint * sinp;
-            muz = sg * cost;
+        // near-vertical incoming directions: the axis form, behind a
+        // wave-uniform branch (the common path issues none of it)
+        if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
+            if (s2 < 1e-7f) {
+                const float sg = mz0 > 0.f ? 1.f : -1.f;
+                mux = sint * cosp;
+                muy = sg * sint * sinp;
+                muz = sg * cost;
+            }
         }
         stokes_apply(cost, k.omc2, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
         const float dtau = k.dtau;

@@ -133,26 +133,37 @@ class MonteCarlo(object):
             output_file = os.path.join(save_dir, '%s_%d.txt' % (base, i))
         return output_file
 
+    HEADER = 'condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m]\n'
+    HEADER_STOKES = ('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m], Q_n, '
+                     'U_n, V_n\n')
+
     @staticmethod
     def write_records(path, rec):
-        """Header and rows of mc3D:1626-1636.  Floats are written with
-        Python's shortest round-trip repr -- the text ``%r`` produced for
-        numpy float64 before numpy 2 (numpy >= 2 would print
-        ``np.float64(...)``, which post_processing.py cannot parse)."""
-        cond = rec['condition'].tolist()
-        nsc = rec['n_scat'].tolist()
-        cols = [rec[k].tolist() for k in ('wvn', 'theta_n', 'phi_n', 'path_length', 'snow_depth')]
-        with open(path, 'w') as fh:
-            if 'Q_n' not in rec:
-                fh.write('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m]\n')
-                fh.writelines('%d %r %r %r %d %r %r\n' % (c, w, t, p, n, pl, sd) for c, w, t, p, n, pl, sd in
-                              zip(cond, cols[0], cols[1], cols[2], nsc, cols[3], cols[4]))
-                return
-            st = [rec[k].tolist() for k in ('Q_n', 'U_n', 'V_n')]     # mc3D:1637-1646
-            fh.write('condition wvn[um^-1] theta_n phi_n n_scat path_length[m], snow_depth[m], Q_n, '
-                     'U_n, V_n\n')
-            fh.writelines('%d %r %r %r %d %r %r %r %r %r\n' % r for r in
-                          zip(cond, cols[0], cols[1], cols[2], nsc, cols[3], cols[4], st[0], st[1], st[2]))
+        """Header and rows of mc3D:1626-1646, written by the library
+        (mcs_write_records_text: host threads, floats as Python's shortest
+        round-trip repr -- the text ``%r`` produced for numpy float64 before
+        numpy 2; numpy >= 2 would print ``np.float64(...)``, which
+        post_processing.py cannot parse)."""
+        import ctypes
+        from ._lib import McsTextRecords, check, lib
+        cols = dict(condition=np.ascontiguousarray(rec['condition'], dtype=np.int8),
+                    n_scat=np.ascontiguousarray(rec['n_scat'], dtype=np.int64))
+        for k in ('wvn', 'theta_n', 'phi_n', 'path_length', 'snow_depth'):
+            cols[k] = np.ascontiguousarray(rec[k], dtype=np.float64)
+        stokes = 'Q_n' in rec
+        if stokes:
+            for k, c in (('Q_n', 'q'), ('U_n', 'u'), ('V_n', 'v')):
+                cols[c] = np.ascontiguousarray(rec[k], dtype=np.float64)
+        n = cols['condition'].size
+        if any(a.size != n for a in cols.values()):
+            raise ValueError('record columns differ in length')
+        R = McsTextRecords(*[cols[k].ctypes.data if k in cols else None for k in
+                             ('condition', 'wvn', 'theta_n', 'phi_n', 'n_scat', 'path_length', 'snow_depth',
+                              'q', 'u', 'v')])
+        header = MonteCarlo.HEADER_STOKES if stokes else MonteCarlo.HEADER
+        code = lib().mcs_write_records_text(os.fsencode(path), header.encode(), ctypes.byref(R), n)
+        if code != 0:
+            raise OSError('cannot write record file %s' % path)
 
     # ------------------------------------------------------------------ optics
     def get_optical_properties(self, wvls, rds_snw):
