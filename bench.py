@@ -53,7 +53,10 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
+    # the walk kernel settles after ~40 launches (clocks and power state:
+    # 0.76 ms averaged over launches 3-23, 0.70 ms from launch 43 on), so the
+    # default warm-up covers that before the timed steps
+    ap.add_argument('--warmup', type=int, default=50)
     ap.add_argument('--photons', type=int, default=N_PHOTONS)
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline leg')
     ap.add_argument('--cpu-photons', type=int, default=10_000)
@@ -176,7 +179,8 @@ def main():
             'basis': '48 FP32 flops per photon-step x photon-steps per launch / walk-kernel '
                      'HIP-event time (the kernel is bound by VALU issue, not FP32 flops: valu_issue = '
                      'VALU wave instructions per photon-step x photon-steps per launch / the same time, '
-                     'against one wave64 instruction per SIMD per 4 cycles); traffic (HBM bytes per '
+                     'against one wave64 instruction per SIMD per 4 cycles, which the SIMDs exceed '
+                     'when they dual-issue VALU work of two waves, ~26 % of quad-cycles here); traffic (HBM bytes per '
                      'launch), valu_issue_busy (share of SIMD quad-cycles issuing VALU work) and the '
                      'instructions per step from rocprofv3 runs of the same workload and launch shape, '
                      'profiles/',

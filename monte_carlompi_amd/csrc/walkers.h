@@ -783,15 +783,28 @@ struct ReplayWalker {
         ++i;
         double dtau, u_hg = 0.0, u_phi = 0.0, p_cos = 0.0, two_phi = 0.0;
         if (i >= 2) {
-            if (pm >= 0) {
-                polar_draws(P, p_cos, two_phi);
+            if (pm < 0 && pos + 5 <= end) {
+                // the step's five deviates are consecutive: five independent
+                // loads in flight at once instead of five dependent draws
+                const double *d = P.rdev + pos;
+                const double d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4];
+                pos += 5;
+                u_hg = d0;
+                u_phi = d1;
+                dtau = -log(d2);
+                u_ssa = d3;
+                u_ext = d4;
             } else {
-                u_hg = draw(P);
-                u_phi = draw(P);
+                if (pm >= 0) {
+                    polar_draws(P, p_cos, two_phi);
+                } else {
+                    u_hg = draw(P);
+                    u_phi = draw(P);
+                }
+                dtau = -log(draw(P));
+                u_ssa = draw(P);
+                u_ext = draw(P);
             }
-            dtau = -log(draw(P));
-            u_ssa = draw(P);
-            u_ext = draw(P);
         } else {
             dtau = surf ? 0.0 : dtau0;   // mc3D:1228-1229
         }
