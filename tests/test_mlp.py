@@ -95,3 +95,40 @@ def test_gpu_training_fits_a_smooth_function():
     mse = np.mean((y - t[:1000]) ** 2)
     assert abs(mse - losses[-1]) < 0.5 * losses[-1] + 1e-3
     net.close()
+
+
+@pytest.mark.gpu
+def test_fused_inference_equals_the_per_layer_kernels(monkeypatch):
+    """256-wide networks infer through one fused kernel (activations in LDS);
+    its MFMA operands, k order and epilogues are the per-layer GEMMs', so
+    forward and serving outputs are bit-identical to theirs, for row counts
+    that are not multiples of the fused kernel's 32-row tiles and over
+    several passes of max_batch rows."""
+    import torch
+    from monte_carlompi_amd import surrogate as S
+    from monte_carlompi_amd._lib import MCS_FEATURES_SNOW_BRF
+    from monte_carlompi_amd.mlp import Mlp
+    rng = np.random.default_rng(7)
+    shift = rng.normal(size=S.N_OUT) * 0.1
+    nets = []
+    for layered in ('0', '1'):
+        monkeypatch.setenv('MCS_MLP_LAYERED', layered)
+        net = Mlp(S.N_FEAT, 256, 4, S.N_OUT, max_batch=4096)
+        net.init_params(seed=3)
+        net.set_serving(MCS_FEATURES_SNOW_BRF, np.full(S.N_OUT, 1e-2), shift, np.full(S.N_OUT, 1e-3),
+                        S.FEAT_MU, S.FEAT_SD)
+        nets.append(net)
+    try:
+        x = rng.normal(size=(9001, S.N_FEAT)).astype(np.float32)
+        np.testing.assert_array_equal(nets[0].forward(x), nets[1].forward(x))
+        q = torch.from_numpy(S.sample_inputs(10_007, 5).astype(np.float32)).cuda()
+        ys = []
+        for net in nets:
+            y = torch.empty((q.shape[0], S.N_OUT), dtype=torch.float32, device='cuda')
+            net.serve(q, q.shape[0], y)
+            ys.append(y.cpu().numpy())
+        assert np.all(np.isfinite(ys[0]))
+        np.testing.assert_array_equal(ys[0], ys[1])
+    finally:
+        for net in nets:
+            net.close()
