@@ -454,7 +454,23 @@ k_walk(const KParams P)
         const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - P.coop_live : 64 - ptail)
                                                    : min(64, (64 - __popcll(live)) + (int)P.refill_min);
         // ---------------- hot: step every live photon
+        // Two steps per loop turn, the exit test after the second: the next
+        // step's Philox block, the direction and the step count then
+        // alternate between two register sets instead of being copied back
+        // at the end of every turn.  Each photon takes the same steps (a lane
+        // that ends in the first stays DONE).  Not in the 96-VGPR build,
+        // where it would add a spill reload to the step loop.
+        constexpr bool TWO = !W::STOKES && WPE != 5;
         for (uint32_t iter = 1;; ++iter) {
+            if constexpr (TWO) {
+                if (state == LIVE) {
+                    const bool d = w.step(P);
+                    if (d || w.i >= P.max_steps) {
+                        if (!d) w.cond = 0;
+                        state = DONE;
+                    }
+                }
+            }
             if (state == LIVE) {
                 const bool d = w.step(P);
                 if (d || w.i >= P.max_steps) {
@@ -464,7 +480,7 @@ k_walk(const KParams P)
             }
             const unsigned long long nl = __ballot(state == LIVE);
             if (nl == 0ull || 64 - __popcll(nl) >= thresh) break;
-            if ((iter & 255u) == 0u) set_priority(state == LIVE ? w.i : 0u);
+            if ((iter & (TWO ? 127u : 255u)) == 0u) set_priority(state == LIVE ? w.i : 0u);
         }
     }
     if constexpr (HAND) {
