@@ -355,6 +355,14 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
     }
 }
 
+// One step of the hot loop (the native walker's step form by build)
+template <int WPE, class W>
+__device__ __forceinline__ bool hot_walk_step(W &w, const KParams &P)
+{
+    if constexpr (W::COOP) return w.template step<(WPE <= 4)>(P);
+    else return w.step(P);
+}
+
 // Wave issue priority from the oldest live photon (i = its step count, 0 for
 // other lanes): long walks set the tail of a launch, so their waves win issue
 // slots over young ones.
@@ -464,16 +472,16 @@ k_walk(const KParams P)
         for (uint32_t iter = 1;; ++iter) {
             if constexpr (TWO) {
                 if (state == LIVE) {
-                    const bool d = w.step(P);
-                    if (d || w.i >= P.max_steps) {
+                    const bool d = hot_walk_step<WPE>(w, P);
+                    if (d || (!W::CAP_IN_STEP && w.i >= P.max_steps)) {
                         if (!d) w.cond = 0;
                         state = DONE;
                     }
                 }
             }
             if (state == LIVE) {
-                const bool d = w.step(P);
-                if (d || w.i >= P.max_steps) {
+                const bool d = hot_walk_step<WPE>(w, P);
+                if (d || (!W::CAP_IN_STEP && w.i >= P.max_steps)) {
                     if (!d) w.cond = 0;
                     state = DONE;
                 }

@@ -74,6 +74,9 @@ struct NativeWalkerT {
     using Real = float;
     static constexpr bool COOP = !POLAR;      // has the wave-cooperative tail mode below
     static constexpr bool STOKES = POLAR;     // carries a Stokes vector (aspherical particles)
+    // step() itself ends a walk at the step cap (condition 0); the hot loop
+    // tests the cap after each step for walkers without this
+    static constexpr bool CAP_IN_STEP = !POLAR;
     // the polarised walk is built for 3 waves per SIMD (168 VGPRs), which
     // caps its blocks at 512 threads
     static constexpr int MAX_BLOCK = POLAR ? 512 : 1024;
@@ -295,7 +298,11 @@ struct NativeWalkerT {
     }
 
     // one step (i >= 1: after the first); true when the photon has left or
-    // been absorbed
+    // been absorbed.  PIN_I: how the next Philox block is held back until the
+    // step's uniforms are read (below); the two forms differ only in register
+    // allocation (the 128-VGPR build issues one move less per step with the
+    // step count pinned; the 80-VGPR build then spills at the hot-loop entry)
+    template <bool PIN_I = false>
     __device__ __forceinline__ bool step(const KParams &P)
     {
         if constexpr (POLAR) {
@@ -310,15 +317,27 @@ struct NativeWalkerT {
         }
         ++i;
         const u32x4 cur = r;
-        r = philox_step(pre, i + 1u, P.key0, P.key1);
         const float dtau = free_path(cur);
-        // polar cosine: HG inverse CDF (mc3D:790-800) with the uniform's
-        // scale and offset folded into d and e; isotropic for g = 0
         const float xf = (float)cur.x;
-        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
-        float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
         // azimuth: v_sin/v_cos take revolutions, 2 pi u -> u
         float rev = u01f(cur.y);
+        // The next block's counter (or the step count) passes an empty asm
+        // that reads the three uniforms above, so the block is built after
+        // they are: the words of `cur` are then dead and the new block can
+        // take their registers (without this every step ended with register
+        // copies).
+        if constexpr (PIN_I) {
+            asm volatile("" : "+v"(i) : "v"(dtau), "v"(xf), "v"(rev));
+            r = philox_step(pre, i + 1u, P.key0, P.key1);
+        } else {
+            uint32_t ctr = i + 1u;
+            asm volatile("" : "+v"(ctr) : "v"(dtau), "v"(xf), "v"(rev));
+            r = philox_step(pre, ctr, P.key0, P.key1);
+        }
+        // polar cosine: HG inverse CDF (mc3D:790-800) with the uniform's
+        // scale and offset folded into d and e; isotropic for g = 0
+        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
+        float cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
         float rev2 = 0.f;
         int seg = -1;
         if constexpr (POLAR) {
@@ -353,9 +372,10 @@ struct NativeWalkerT {
         }
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
-        if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all)) return false;
-        // ---- rare: exits and absorption.  The depth passes an empty asm so
-        // that the tests below are issued here, not hoisted into every step.
+        if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all || (CAP_IN_STEP && i >= P.max_steps))) return false;
+        // ---- rare: exits, absorption and the step cap.  The depth passes an
+        // empty asm so that the tests below are issued here, not hoisted into
+        // every step.
         float ze = z;
         asm volatile("" : "+v"(ze));
         if (ze > 0.f) {                              // mc3D:1390-1397
@@ -366,14 +386,21 @@ struct NativeWalkerT {
         if (ze < -tau_tot) {                         // mc3D:1399-1459
             tau_acc -= fminf((z + tau_tot) / muz, dtau);
             z = -tau_tot;
-            if (lamb && reflects(P)) return reemit(P);
+            if (lamb && reflects(P)) {
+                if (reemit(P)) return true;
+                if (CAP_IN_STEP && i >= P.max_steps) {   // re-emitted at the cap
+                    cond = 0;
+                    return true;
+                }
+                return false;
+            }
             cond = 2;
             return true;
         }
         // absorbed: one uniform decides species and absorption jointly, with
         // P(cond 5) = P_imp (1 - omega_imp), P(cond 4) = (1 - P_imp)(1 - omega_ice)
-        // (mc3D:1374-1383, 1461-1466)
-        cond = (cur.w < t_abs5) ? 5 : 4;
+        // (mc3D:1374-1383, 1461-1466); otherwise the step cap (condition 0)
+        cond = (cur.w < t_abs5) ? 5 : (CAP_IN_STEP && cur.w >= t_abs_all) ? 0 : 4;
         return true;
     }
 
@@ -724,6 +751,7 @@ struct ReplayWalker {
     using Real = double;
     static constexpr bool COOP = false;
     static constexpr bool STOKES = true;
+    static constexpr bool CAP_IN_STEP = false;
     static constexpr int MAX_BLOCK = 256;
     static constexpr int WAVES_PER_EU = 2;
     double z, mux, muy, muz, muz2, path;
