@@ -244,10 +244,11 @@ def polar_block(w, no_cpu):
     case = snow_polar_case(n, d)
     w.set_cases([case])
     # 512-thread blocks (the polarised kernel's limit; 1024 requests are split),
-    # or three of 256 per CU (12 waves: all that 3 waves per SIMD allow);
+    # or three of 256 per CU (12 waves: all that 3 waves per SIMD allow; chunks
+    # of 1302 make 768 chunks for 1e6 photons, one per block);
     # coop_live is the tail mode's threshold (at most 16 photons per wave)
-    cands = [(512, 2, 1024, 16, 16), (256, 3, 1024, 16, 16), (256, 3, 2048, 16, 16), (1024, 1, 2048, 16, 16),
-             (512, 2, 2048, 16, 8)]
+    cands = [(512, 2, 1024, 16, 16), (256, 3, 1024, 16, 16), (256, 3, 1302, 16, 16), (256, 3, 2048, 16, 16),
+             (1024, 1, 2048, 16, 16), (512, 2, 2048, 16, 8)]
     shape, tune = w.autotune(cands, photons_cap=0, repeats=3, seed=7)
     for k in range(2):
         w.walk(mode='native', seed=900 + k, records=True)

@@ -49,7 +49,8 @@ constexpr uint32_t CUR_OFF_MASK = (1u << CUR_SLOT_SHIFT) - 1u;
 
 // Take the block's chunk number `seq` into free slot `slot` (whole wave calls).
 // Returns false when the global queue is empty.
-__device__ bool acquire_chunk(BlockState &bs, uint32_t seq, int slot, const KParams &P)
+template <class BS>
+__device__ bool acquire_chunk(BS &bs, uint32_t seq, int slot, const KParams &P)
 {
     const int lane = lane_id();
     unsigned int j = 0;
@@ -88,7 +89,8 @@ __device__ bool acquire_chunk(BlockState &bs, uint32_t seq, int slot, const KPar
 }
 
 // Flush slot s to its case's global tallies and free it (whole wave calls).
-__device__ void flush_slot(BlockState &bs, int s, const KParams &P)
+template <class BS>
+__device__ void flush_slot(BS &bs, int s, const KParams &P)
 {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     const int lane = lane_id();
@@ -127,8 +129,8 @@ __device__ __forceinline__ void write_record(const KParams &P, size_t pos, uint6
 }
 
 // Record + tally the DONE lanes of the wave; flush chunks that complete.
-template <class W, bool DIRECT = false>
-__device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, int &state,
+template <class W, bool DIRECT = false, class BS>
+__device__ __forceinline__ void retire(BS &bs, const KParams &P, W &w, int &state,
                                        int my_slot, uint64_t my_gid)
 {
     const int lane = lane_id();
@@ -200,8 +202,8 @@ __device__ __forceinline__ void retire(BlockState &bs, const KParams &P, W &w, i
 
 // Give photons of the block's current chunk to EMPTY lanes, advancing the
 // block to the next chunk of the global queue when the current one is used up.
-template <class W>
-__device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, int &state,
+template <class W, class BS>
+__device__ __forceinline__ void refill(BS &bs, const KParams &P, W &w, int &state,
                                       int &my_slot, uint64_t &my_gid)
 {
     const int lane = lane_id();
@@ -211,7 +213,7 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
         if (n == 0) break;
         const unsigned long long cur = lds_load(&bs.cur);
         uint32_t seq = (uint32_t)(cur >> 32);
-        int cslot = (int)((uint32_t)cur >> CUR_SLOT_SHIFT) & (SLOTS - 1);
+        int cslot = (int)((uint32_t)cur >> CUR_SLOT_SHIFT) & (BS::NSLOT - 1);
         uint32_t off = (uint32_t)cur & CUR_OFF_MASK;
         unsigned long long meta = lds_load(&bs.hdr[cslot].seqcount);
         uint32_t count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
@@ -221,7 +223,7 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
             if (lane == 0) old = atomicAdd(&bs.cur, (unsigned long long)n);
             old = bcast64(old, 0);
             seq = (uint32_t)(old >> 32);
-            cslot = (int)((uint32_t)old >> CUR_SLOT_SHIFT) & (SLOTS - 1);
+            cslot = (int)((uint32_t)old >> CUR_SLOT_SHIFT) & (BS::NSLOT - 1);
             off = (uint32_t)old & CUR_OFF_MASK;
             meta = lds_load(&bs.hdr[cslot].seqcount);
             count = ((uint32_t)(meta >> 32) == seq) ? (uint32_t)meta : 0u;
@@ -248,8 +250,8 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
         if (seq_now == seq && !lds_load(&bs.exhausted)) {
             // any free slot: a chunk whose last photons still walk holds only its own
             int ns = -1;
-            for (int d = 1; d <= SLOTS && ns < 0; ++d) {
-                const int s = (cslot + d) & (SLOTS - 1);
+            for (int d = 1; d <= BS::NSLOT && ns < 0; ++d) {
+                const int s = (cslot + d) & (BS::NSLOT - 1);
                 if (lds_load(&bs.hdr[s].state) == 0) ns = s;
             }
             if (ns >= 0 && acquire_chunk(bs, seq_now + 1, ns, P)) {
@@ -285,7 +287,9 @@ __device__ __forceinline__ void refill(BlockState &bs, const KParams &P, W &w, i
 // state), then each photon's own lane walks those G steps through
 // polar_step.  The serial chain of a step keeps only the phi acceptance,
 // rotation, Stokes update and events.  Records equal the one-lane walk's.
-// LDS after the tables: 16 owner lanes and 64 packets per wave.
+// LDS after the tables: 16 owner lanes and PK_SLOTS packets per wave (64 and
+// one that the last group's prefetch may read past its own).
+constexpr int PK_SLOTS = 65;
 __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wave)
 {
     extern __shared__ __align__(16) float pm_lds_tab[];
@@ -293,7 +297,7 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
     // packets 16-byte aligned after the tables and owner lanes (the launch
     // reserves 16 bytes for this)
     const int pk_off = (P.pm_lds_floats + (int)(blockDim.x >> 6) * 16 + 3) & ~3;
-    PolarPacket *const pk = reinterpret_cast<PolarPacket *>(pm_lds_tab + pk_off) + wave * 64;
+    PolarPacket *const pk = reinterpret_cast<PolarPacket *>(pm_lds_tab + pk_off) + wave * PK_SLOTS;
     int *const own = owners + wave * 16;
     const int lane = lane_id();
     for (;;) {
@@ -318,33 +322,48 @@ __device__ void polar_tail(PolarWalker &w, int &state, const KParams &P, int wav
             const u32x4 pb = philox4x32_10(u32x4{l0 * 0x991A7CDBu, ck0 ^ P.key0, st, 2u}, P.key0, P.key1);
             const float *T = (P.pm_lds_floats > 0 ? pm_lds_tab : P.pmf) + (size_t)pmo * (size_t)P.pm_stride;
             PolarPacket q;
-            PolarWalker::make_packet(mainb, pb, T, P.pm_n, mass, q);
+            float r2;
+            int seg;
+            PolarWalker::make_packet(mainb, pb, T, P.pm_n, mass, q, r2, seg);
             pk[lane] = q;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
         if (state == LIVE) {
-            // the moves run straight through the group's packets; the loop's
-            // one exit is an event (rare), the step cap or the last packet.
-            // After an event the next round builds packets from the photon's
-            // step count then (a bottom re-emission walks a step of its own).
-            // The next step's packet is loaded while this one's step runs
-            // (two packets in registers, their roles alternating).
+            // the moves run straight through the group's packets, all photons
+            // of the wave in step: the round ends for every lane after the
+            // step in which any photon has an event (rare) or reaches the step
+            // cap, or after the last packet -- a wave-uniform exit, so the
+            // loop issues no exec-mask changes.  A photon without an event
+            // simply goes on in the next round, whose packets are built from
+            // its step count then; after an event (a bottom re-emission walks
+            // a step of its own) likewise.  The next step's packet is loaded
+            // while this one's step runs (two packets in registers, their
+            // roles alternating; the last prefetch may read the slot after the
+            // group's packets, never used).  Photons in the tail have i <
+            // max_steps, so the first step is always taken.
             const PolarPacket *const mine = pk + lane_rank(live) * G;
-            // (each step adds one to i: the step cap bounds the steps too)
-            const int kmax = (int)min((uint32_t)G, P.max_steps > w.i ? P.max_steps - w.i : 0u);
+            // the step cap in a VGPR for the loop (as a kernel argument it
+            // would be reloaded from spilled scalar registers every step)
+            uint32_t cap = P.max_steps;
+            asm volatile("" : "+v"(cap));
             PolarPacket a = mine[0];
             int k = 0;
-            bool ev = false;
-            while (k < kmax) {
-                const PolarPacket b = mine[min(k + 1, G - 1)];
-                ev = w.polar_move(P, a);
-                if (++k >= kmax || ev) break;
-                a = mine[min(k + 1, G - 1)];
+            for (;;) {
+                PolarPacket b = mine[k + 1];
+                bool ev = w.polar_move(P, a);
+                ++k;
+                if (__ballot(ev || w.i >= cap) != 0ull || k >= G) break;
+                a = mine[k + 1];
                 ev = w.polar_move(P, b);
-                if (++k >= kmax || ev) break;
+                ++k;
+                if (__ballot(ev || w.i >= cap) != 0ull || k >= G) break;
             }
-            const bool d = ev && w.polar_event(P, mine[k - 1].dtau, mine[k - 1].w);
+            // the last step's event test, formed again from the state it left
+            // (as polar_move forms it), so the loop carries no per-lane flag
+            const PolarPacket &last = mine[k - 1];
+            const bool ev = w.z > 0.f || w.z < -w.tau_tot || last.w < w.t_abs_all;
+            const bool d = ev && w.polar_event(P, last.dtau, last.w);
             if (d || w.i >= P.max_steps) {
                 if (!d) w.cond = 0;
                 state = DONE;
@@ -377,13 +396,17 @@ __device__ __forceinline__ void set_priority(uint32_t i)
 // WPE: waves per SIMD the register allocation targets (the launch picks the
 // instance that fits its shape: 4 -> 128 VGPRs, no spills; 5 -> 96; 6 -> 80).
 // The hand-over of cooperative-tail photons between waves is left out of the
-// 6-wave build: at 80 VGPRs its code would spill in the step loop.
-template <class W, int WPE, bool DIRECT = false>
+// 6-wave build: at 80 VGPRs its code would spill in the step loop.  SL: chunk
+// slots of the block (tally slots in LDS, 5.3 KB each); the polarised walk
+// in blocks of up to 256 threads takes 2, so that with its tables and tail
+// packets three blocks fit in a CU's LDS (with 4, two do; larger blocks keep
+// 4, or their waves would wait for slots held by lingering walks).
+template <class W, int WPE, bool DIRECT = false, int SL = SLOTS>
 __global__ void __launch_bounds__(W::MAX_BLOCK) __attribute__((amdgpu_waves_per_eu(WPE)))
 k_walk(const KParams P)
 {
     constexpr bool HAND = W::COOP && WPE <= 5;
-    __shared__ BlockState bs;
+    __shared__ BlockStateT<SL> bs;
     // native mode: coop_live table entries per wave, then coop_live group slots per wave
     extern __shared__ CoopEntry coop_tab[];
     const int lane = lane_id();
@@ -391,7 +414,7 @@ k_walk(const KParams P)
     // ---- init: zero LDS, wave 0 takes the first chunk
     {
         unsigned long long *w64 = reinterpret_cast<unsigned long long *>(&bs);
-        for (int k = threadIdx.x; k < (int)(sizeof(BlockState) / 8); k += blockDim.x) w64[k] = 0ull;
+        for (int k = threadIdx.x; k < (int)(sizeof(bs) / 8); k += blockDim.x) w64[k] = 0ull;
     }
     if constexpr (W::STOKES && W::COOP == false && sizeof(typename W::Real) == 4) {
         // polarised native walk: phase-matrix tables into dynamic LDS
@@ -1327,13 +1350,21 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
     if (mode == MCS_NATIVE && c->polar) {
         // aspherical particles: one photon per lane, then the tail mode of
         // polar_tail; tables up to 16 KB ride in LDS
-        // + the tail mode's owner lanes and packets (16 ints and 64 packets a wave)
+        // + the tail mode's owner lanes and packets (16 ints and PK_SLOTS packets a wave)
         const size_t lds = (size_t)P.pm_lds_floats * sizeof(float) +
-                           (size_t)(block / 64) * (16 * sizeof(int) + 64 * sizeof(PolarPacket)) + 16;
-        if (P.rec_direct)
-            hipLaunchKernelGGL((k_walk<PolarWalker, 3, true>), dim3(grid), dim3(block), lds, c->stream, P);
-        else
-            hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
+                           (size_t)(block / 64) * (16 * sizeof(int) + PK_SLOTS * sizeof(PolarPacket)) + 16;
+        // blocks of up to 256 threads: two chunk slots (k_walk's SL)
+        if (P.rec_direct) {
+            if (block <= 256)
+                hipLaunchKernelGGL((k_walk<PolarWalker, 3, true, 2>), dim3(grid), dim3(block), lds, c->stream, P);
+            else
+                hipLaunchKernelGGL((k_walk<PolarWalker, 3, true>), dim3(grid), dim3(block), lds, c->stream, P);
+        } else {
+            if (block <= 256)
+                hipLaunchKernelGGL((k_walk<PolarWalker, 3, false, 2>), dim3(grid), dim3(block), lds, c->stream, P);
+            else
+                hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
+        }
     } else if (mode == MCS_NATIVE) {
         const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         // the instance whose register budget fits the shape's waves per SIMD:

@@ -136,7 +136,10 @@ struct SlotTally {
     unsigned long long u[TU];
 };
 
-struct BlockState {
+template <int S>
+struct BlockStateT {
+    static constexpr int NSLOT = S;
+    static_assert((S & (S - 1)) == 0, "slots: a power of two");
     unsigned long long cur;  // (seq << 32) | next photon offset in chunk
     int lock;
     int exhausted;
@@ -147,9 +150,12 @@ struct BlockState {
     int ib_src[MAX_WAVES];        //   posting wave (its coop table holds them)
     unsigned int ib_mask[MAX_WAVES];  //   their table entries
     int out_posts[MAX_WAVES];     // posts of each wave not yet taken
-    SlotHdr hdr[SLOTS];
-    SlotTally tal[SLOTS];
+    SlotHdr hdr[S];
+    SlotTally tal[S];
 };
+// the HG walk's block state (the polarised walk in blocks of up to 256
+// threads runs with two slots: k_walk's SL)
+using BlockState = BlockStateT<SLOTS>;
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 

@@ -44,17 +44,21 @@ __device__ __forceinline__ T interp_at(const T *x, const T *y, int n, int j, T v
 // The state-independent part of one step of the polarised walk (exact
 // sampler): free path and absorption word from the step's main Philox block,
 // cos theta, sin theta and the phase matrix there, the first 2 phi candidate
-// (with the sines and cosines of phi and 2 phi) and its acceptance uniform.
-// One lane can build it for any step of any photon.  80 bytes, 16-aligned:
-// the tail mode reads a packet from LDS as five 16-byte loads.
+// (with the sines and cosines of phi and 2 phi) and its acceptance uniform,
+// and the second candidate's two uniforms.  One lane can build it for any
+// step of any photon.  80 bytes, 16-aligned: the tail mode reads a packet
+// from LDS as five 16-byte loads.  (The candidate's 2 phi and the table
+// segment, which only the Lambertian azimuth needs, come out of make_packet
+// and phi_retry beside it.)
 struct alignas(16) PolarPacket {
     float dtau;
     uint32_t w;
-    float ct, r2, acc;
+    float ct, acc;
     float p11, p12, p22, p33, p43, p44;
-    int seg;
     float sint, sinp, cosp, c2r, s2r;
     float omc2;                          // max(1 - ct^2, 0): the rotation back uses it
+    float s2s, m2s;                      // the rotation signs of the candidate (phi_signs)
+    float r2b, accb;                     // the second candidate (phi_second)
 };
 static_assert(sizeof(PolarPacket) == 80, "PolarPacket: five 16-byte LDS loads");
 
@@ -67,6 +71,16 @@ __device__ __forceinline__ void phi_trig(float r2, float &sinp, float &cosp, flo
     cosp = __builtin_amdgcn_cosf(0.5f * r2);
     c2r = fmaf(cosp, cosp, -sinp * sinp);
     s2r = 2.f * sinp * cosp;
+}
+
+// The signs the reference's folded azimuth phi_sca = arccos(cos phi) puts on
+// the Stokes rotations (mc3D:1291-1337): s2s = sin 2 phi_sca (-sin 2 phi for
+// phi > pi) into the scattering plane, and m2s = -2 sign for the return
+// rotation (sign = +1 only at phi_sca = pi).  2 phi = r2 revolutions.
+__device__ __forceinline__ void phi_signs(float r2, float s2r, float &s2s, float &m2s)
+{
+    s2s = (r2 > 1.f ? -1.f : 1.f) * s2r;
+    m2s = (r2 == 1.f) ? -2.f : 2.f;
 }
 
 template <bool POLAR>
@@ -311,7 +325,9 @@ struct NativeWalkerT {
                 r = philox_step(pre, i + 2u, P.key0, P.key1);
                 const u32x4 pb = philox4x32_10(u32x4{photon_id(), case_id(P), i + 1u, 2u}, P.key0, P.key1);
                 PolarPacket k;
-                make_packet(cur, pb, pm_table(P), P.pm_n, pm_mass, k);
+                float r2;
+                int seg;
+                make_packet(cur, pb, pm_table(P), P.pm_n, pm_mass, k, r2, seg);
                 return polar_step(P, k);
             }
         }
@@ -445,16 +461,17 @@ struct NativeWalkerT {
             // the marginal S11 by inverting its piecewise-quadratic integral
             // (make_packet), then 2 phi from 1 + (S12 / S11) (Q cos 2phi +
             // U sin 2phi) by rejection against 1 + |S12 / S11| sqrt(Q^2 +
-            // U^2) (polar_phi: acceptance >= 1/2, no table reads).  Philox
+            // U^2) (phi_ok, phi_retry: acceptance >= 1/2, no table reads).  Philox
             // block (photon, case, i, 2 + k), k = 0, 1, ...
             const u32x4 q = philox4x32_10(u32x4{photon_id(), cs, i, 2u}, P.key0, P.key1);
             PolarPacket k;
-            make_packet(q, q, T, n, pm_mass, k);   // its free path and absorption word go unused
-            float r2, sinp, cosp, c2r, s2r;
-            polar_phi(P, k, r2, sinp, cosp, c2r, s2r);
+            float r2;
+            make_packet(q, q, T, n, pm_mass, k, r2, seg);   // its free path and absorption word go unused
+            // (no second candidate here: with one block for both, its words
+            // are the ones cos theta and the first 2 phi came from)
+            if (!phi_ok(k)) r2 = phi_retry(P, k);
             cost = k.ct;
             rev2 = r2;
-            seg = k.seg;
             return;
         }
         ++n_rej;
@@ -513,43 +530,43 @@ struct NativeWalkerT {
         const float p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         const float p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         const float p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        stokes_apply(cost, fmaxf(fmaf(-cost, cost, 1.f), 0.f), rev2, __builtin_amdgcn_cosf(rev2),
-                     __builtin_amdgcn_sinf(rev2), mx0, my0, mz0, p11, p12, p22, p33, p43, p44);
+        float s2s, m2s;
+        phi_signs(rev2, __builtin_amdgcn_sinf(rev2), s2s, m2s);
+        stokes_apply(cost, fmaxf(fmaf(-cost, cost, 1.f), 0.f), __builtin_amdgcn_cosf(rev2), s2s, m2s, mx0, my0,
+                     mz0, p11, p12, p22, p33, p43, p44);
     }
 
     // (mx0, my0, mz0): the direction before the scattering; the photon's
-    // direction members hold the one after it
-    // c2r, s2r: cos and sin of 2 phi (rev2 revolutions); omc2 = max(1 - cost^2, 0)
-    __device__ __forceinline__ void stokes_apply(float cost, float omc2, float rev2, float c2r, float s2r, float mx0, float my0,
-                                                 float mz0, float p11, float p12, float p22, float p33, float p43,
-                                                 float p44)
+    // direction members hold the one after it.  c2 = cos 2 phi, s2s and m2s
+    // from phi_signs; omc2 = max(1 - cost^2, 0)
+    __device__ __forceinline__ void stokes_apply(float cost, float omc2, float c2, float s2s, float m2s, float mx0,
+                                                 float my0, float mz0, float p11, float p12, float p22, float p33,
+                                                 float p43, float p44)
     {
 #pragma clang fp contract(off)
-        const float c2 = c2r;
-        const float s2 = (rev2 > 1.f ? -1.f : 1.f) * s2r;
-        const float Qs = fmaf(sQ, c2, sU * s2);
-        const float Us = fmaf(-sQ, s2, sU * c2);
+        const float Qs = fmaf(sQ, c2, sU * s2s);
+        const float Us = fmaf(-sQ, s2s, sU * c2);
         const float Ic = fmaf(sI, p11, Qs * p12);
         const float Qc = fmaf(sI, p12, Qs * p22);
         const float Uc = fmaf(Us, p33, -sV * p43);
         const float Vc = fmaf(Us, p43, sV * p44);
         const float muz_n = muz;
-        const float den = __builtin_amdgcn_sqrtf(omc2 * fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f));
-        // i_2 in [0, pi]: cos i_2 as the reference forms it; sin i_2 >= 0 from
-        // its exact identity (den sin i_2)^2 = (e_z . (u_0 x u_n))^2 -- the
-        // Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2 - cos^2 --
-        // instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1 (scattering
-        // plane close to the meridian plane) would turn an FP32 rounding of
-        // cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 = 0.
-        // (straight-line: both values are formed and den = 0 selects 0 and 1)
-        const float rd = __builtin_amdgcn_rcpf(den);
-        const float sg = (rev2 == 1.f) ? 1.f : -1.f;
-        const float ci0 = fminf(fmaxf(sg * fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
+        const float den2 = omc2 * fmaxf(fmaf(-muz_n, muz_n, 1.f), 0.f);
+        const float rd = __builtin_amdgcn_rsqf(den2);   // 1 / den
+        // i_2 in [0, pi]: cos i_2 as the reference forms it, up to its sign,
+        // which only the return rotation's sine uses (m2s; the clamp is odd);
+        // sin i_2 >= 0 from its exact identity (den sin i_2)^2 = (e_z . (u_0 x
+        // u_n))^2 -- the Gram determinant 1 + 2 mu_0 mu_n cos - mu_0^2 - mu_n^2
+        // - cos^2 -- instead of sqrt(1 - cos^2 i_2), which near |cos i_2| = 1
+        // (scattering plane close to the meridian plane) would turn an FP32
+        // rounding of cos i_2 into a rotation of ~1e-3 rad.  den = 0: cos i_2 =
+        // 0.  (straight-line: both values are formed and den = 0 selects 0 and 1)
+        const float ci0 = fminf(fmaxf(fmaf(muz_n, cost, -mz0) * rd, -1.f), 1.f);
         const float si0 = fminf(fabsf(fmaf(mx0, muy, -my0 * mux)) * rd, 1.f);
-        const float ci = (den != 0.f) ? ci0 : 0.f;
-        const float si = (den != 0.f) ? si0 : 1.f;
+        const float ci = (den2 != 0.f) ? ci0 : 0.f;
+        const float si = (den2 != 0.f) ? si0 : 1.f;
         const float cr = fmaf(-2.f * si, si, 1.f);     // cos(-2 i_2) = 1 - 2 sin^2 i_2
-        const float sr = -2.f * ci * si;               // sin(-2 i_2)
+        const float sr = m2s * ci * si;                // sin(-2 i_2) = -2 cos i_2 sin i_2
         const float inv = __builtin_amdgcn_rcpf(Ic);
         const float Qm = fmaf(Qc, cr, Uc * sr);
         const float Um = fmaf(-Qc, sr, Uc * cr);
@@ -564,7 +581,7 @@ struct NativeWalkerT {
     // theta by inverting the integral of the linear S11 (guided search), the
     // six elements there.
     __device__ static void make_packet(const u32x4 &mainb, const u32x4 &pb, const float *T, int n, float mass,
-                                       PolarPacket &o)
+                                       PolarPacket &o, float &r2, int &seg)
     {
         // no implicit contraction in the packet or the step that uses it: a
         // packet built in another lane and read from LDS must give the same
@@ -592,18 +609,23 @@ struct NativeWalkerT {
         const float ct = fminf(x0 + fminf(fmaxf(s, 0.f), h), T[n - 1]);
         const float dx = ct - x0;
         o.ct = ct;
-        o.seg = j;
+        seg = j;
         o.p11 = fmaf(slope, dx, y0);
         o.p12 = fmaf(T[9 * n + j], dx, T[2 * n + j]);
         o.p22 = fmaf(T[10 * n + j], dx, T[3 * n + j]);
         o.p33 = fmaf(T[11 * n + j], dx, T[4 * n + j]);
         o.p43 = fmaf(T[12 * n + j], dx, T[5 * n + j]);
         o.p44 = fmaf(T[13 * n + j], dx, T[6 * n + j]);
-        o.r2 = 2.f * u01f(pb.y);
+        r2 = 2.f * u01f(pb.y);
         o.acc = u01f(pb.z);
         o.sint = __builtin_amdgcn_sqrtf(fmaf(-ct, ct, 1.f));
         o.omc2 = fmaxf(fmaf(-ct, ct, 1.f), 0.f);
-        phi_trig(o.r2, o.sinp, o.cosp, o.c2r, o.s2r);
+        phi_trig(r2, o.sinp, o.cosp, o.c2r, o.s2r);
+        phi_signs(r2, o.s2r, o.s2s, o.m2s);
+        // the main block's first two words (the exact sampler draws its angles
+        // from the other block) give the second candidate
+        o.r2b = 2.f * u01f(mainb.x);
+        o.accb = u01f(mainb.y);
     }
 
     // 2 phi given cos theta: the first candidate r2 (revolutions), its
@@ -614,28 +636,41 @@ struct NativeWalkerT {
     // accepted r2 with the sine and cosine of the azimuth (r2 / 2) and of 2
     // phi.  The first candidate is tested outside the retry loop: it is
     // accepted with probability >= 1/2, so the common step is straight-line.
-    __device__ __forceinline__ void polar_phi(const KParams &P, const PolarPacket &k, float &r2, float &sinp,
-                                              float &cosp, float &c2r, float &s2r) const
+    // phi_ok tests the packet's candidate against the current Stokes vector;
+    // phi_second makes the packet's second candidate (from the step's main
+    // block) its candidate; phi_retry then draws until one is accepted
+    // (blocks (photon, case, i, 2 + k), k >= 1), replacing the packet's
+    // candidate (r2, its sines and cosines and signs) in place.
+    __device__ __forceinline__ bool phi_ok(const PolarPacket &k) const
     {
 #pragma clang fp contract(off)
         const float s11 = k.p11, s12 = k.p12;
         const float bound = fmaf(fabsf(sI), s11, fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU)));
-        r2 = k.r2;
-        sinp = k.sinp;
-        cosp = k.cosp;
-        c2r = k.c2r;
-        s2r = k.s2r;
-        if (k.acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r))) return;
+        return k.acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, k.c2r, sU * k.s2r));
+    }
+    __device__ __forceinline__ static void phi_second(PolarPacket &k)
+    {
+        k.acc = k.accb;
+        phi_trig(k.r2b, k.sinp, k.cosp, k.c2r, k.s2r);
+        phi_signs(k.r2b, k.s2r, k.s2s, k.m2s);
+    }
+    // (returns the accepted candidate's 2 phi in revolutions)
+    __device__ float phi_retry(const KParams &P, PolarPacket &k) const
+    {
+#pragma clang fp contract(off)
+        const float s11 = k.p11, s12 = k.p12;
+        const float bound = fmaf(fabsf(sI), s11, fabsf(s12) * __builtin_amdgcn_sqrtf(fmaf(sQ, sQ, sU * sU)));
         for (uint32_t kk = 1;; ++kk) {
-            if (kk >= POLAR_MAX_TRIES) {
-                atomicAdd(P.ctl + 2, 1ull);
-                return;
-            }
             const u32x4 q = philox4x32_10(u32x4{photon_id(), case_id(P), i, 2u + kk}, P.key0, P.key1);
-            r2 = 2.f * u01f(q.x);
+            const float r2 = 2.f * u01f(q.x);
             const float acc = u01f(q.y);
-            phi_trig(r2, sinp, cosp, c2r, s2r);
-            if (acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, c2r, sU * s2r))) return;
+            phi_trig(r2, k.sinp, k.cosp, k.c2r, k.s2r);
+            phi_signs(r2, k.s2r, k.s2s, k.m2s);
+            if (acc * bound <= fmaf(sI, s11, s12 * fmaf(sQ, k.c2r, sU * k.s2r))) return r2;
+            if (kk + 1 >= POLAR_MAX_TRIES) {
+                atomicAdd(P.ctl + 2, 1ull);
+                return r2;
+            }
         }
     }
 
@@ -645,19 +680,40 @@ struct NativeWalkerT {
     // then the events (polar_event).  The one-lane walk and the tail mode
     // both step through these two, so a photon's record does not depend on
     // which of them walked it.  Returns true when the walk has ended.
-    __device__ __forceinline__ bool polar_step(const KParams &P, const PolarPacket &k)
+    __device__ __forceinline__ bool polar_step(const KParams &P, PolarPacket &k)
     {
-        return polar_move(P, k) && polar_event(P, k.dtau, k.w);
+        return polar_move_seq(P, k) && polar_event(P, k.dtau, k.w);
     }
 
-    // The move of a step; true when an event follows (the photon left the
-    // slab through either face, or its absorption word says absorbed).
-    __device__ __forceinline__ bool polar_move(const KParams &P, const PolarPacket &k)
+    // The one-lane walk's move: the candidate is chosen first (the first of
+    // the packet's two, then retries, that is accepted), then the step is
+    // formed.  In a wave of 64 photons some lane rejects its first candidate
+    // at nearly every step, so the second is ready without a Philox block.
+    __device__ __forceinline__ bool polar_move_seq(const KParams &P, PolarPacket &k)
     {
 #pragma clang fp contract(off)
         ++i;
-        float r2, sinp, cosp, c2r, s2r;
-        polar_phi(P, k, r2, sinp, cosp, c2r, s2r);
+        bool ok = phi_ok(k);
+        if (!ok) {
+            phi_second(k);
+            ok = phi_ok(k);
+        }
+        if (__builtin_expect(__ballot(!ok) != 0ull, 0)) {
+            if (!ok) phi_retry(P, k);
+        }
+        polar_scatter(k);
+        const float dtau = k.dtau;
+        z = fmaf(dtau, muz, z);
+        tau_acc += dtau;
+        return z > 0.f || z < -tau_tot || k.w < t_abs_all;
+    }
+
+    // The scattering of a step with the packet's candidate: rotation of the
+    // direction (mc3D:1255-1281) and Stokes update, from the state before it.
+    __device__ __forceinline__ void polar_scatter(const PolarPacket &k)
+    {
+#pragma clang fp contract(off)
+        const float sinp = k.sinp, cosp = k.cosp;
         const float cost = k.ct;
         const float sint = k.sint;
         const float s2 = fmaf(-muz, muz, 1.f);
@@ -677,7 +733,35 @@ struct NativeWalkerT {
                 muz = sg * cost;
             }
         }
-        stokes_apply(cost, k.omc2, r2, c2r, s2r, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+        stokes_apply(cost, k.omc2, k.c2r, k.s2s, k.m2s, mx0, my0, mz0, k.p11, k.p12, k.p22, k.p33, k.p43, k.p44);
+    }
+
+    // The tail mode's move; true when an event follows (the photon left the
+    // slab through either face, or its absorption word says absorbed).  The
+    // step is formed with the packet's first candidate while that candidate's
+    // acceptance test (against the state before the step) runs beside it, so
+    // the direction and Stokes chains do not wait for the test; a rejected
+    // candidate (rare with a few photons per wave: behind a wave-uniform
+    // branch) restores the state, takes the second candidate or draws until
+    // one is accepted (replacing the packet's candidate) and forms the step
+    // again.  Results equal polar_move_seq's.
+    __device__ __forceinline__ bool polar_move(const KParams &P, PolarPacket &k)
+    {
+#pragma clang fp contract(off)
+        ++i;
+        const bool ok = phi_ok(k);
+        const float mx0 = mux, my0 = muy, mz0 = muz;
+        const float i0 = sI, q0 = sQ, u0 = sU, v0 = sV;
+        polar_scatter(k);
+        if (__builtin_expect(__ballot(!ok) != 0ull, 0)) {
+            if (!ok) {
+                mux = mx0; muy = my0; muz = mz0;
+                sI = i0; sQ = q0; sU = u0; sV = v0;
+                phi_second(k);
+                if (!phi_ok(k)) phi_retry(P, k);
+                polar_scatter(k);
+            }
+        }
         const float dtau = k.dtau;
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
