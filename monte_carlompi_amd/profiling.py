@@ -56,7 +56,7 @@ def run_sweep(repeats=2, launch=(256, 6, 8192, 24)):
                                                                       st['steps']), flush=True)
 
 
-def run_polar(repeats=3, launch=(256, 3, 2048, 16, 16), photons=1_000_000):
+def run_polar(repeats=3, launch=(256, 3, 1302, 16, 16), photons=1_000_000):
     """The aspherical-grain workload of bench.py (1e6 photons, droxtal phase
     matrices, polarised light) `repeats` times (a workload to profile on its
     own)."""
@@ -229,7 +229,7 @@ def main():
     sw = sub.add_parser('sweep'); sw.add_argument('--repeats', type=int, default=2)
     po = sub.add_parser('polar'); po.add_argument('--repeats', type=int, default=3)
     po.add_argument('--photons', type=int, default=1_000_000)
-    po.add_argument('--launch', default='256,3,2048,16,16')
+    po.add_argument('--launch', default='256,3,1302,16,16')
     ml = sub.add_parser('mlp'); ml.add_argument('--repeats', type=int, default=3)
     sn = sub.add_parser('snow1m'); sn.add_argument('--repeats', type=int, default=6)
     sn.add_argument('--launch', default='1024,1,1954,16,32'); sn.add_argument('--json')
