@@ -283,9 +283,9 @@ __device__ __forceinline__ void refill(BS &bs, const KParams &P, W &w, int &stat
 // at most 16 photons (all with the exact sampler), the wave works in rounds:
 // the G = 64 / 2^ceil(log2 L) lanes of group g build the packets of the next
 // G steps of the group's photon (Philox blocks, cos theta, phase matrix,
-// first phi candidate: everything that does not depend on the photon's
+// the phi candidates: everything that does not depend on the photon's
 // state), then each photon's own lane walks those G steps through
-// polar_step.  The serial chain of a step keeps only the phi acceptance,
+// polar_move.  The serial chain of a step keeps only the phi acceptance,
 // rotation, Stokes update and events.  Records equal the one-lane walk's.
 // LDS after the tables: 16 owner lanes and PK_SLOTS packets per wave (64 and
 // one that the last group's prefetch may read past its own).
