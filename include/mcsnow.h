@@ -118,14 +118,16 @@ typedef struct {
 typedef struct {
     int32_t block;             /* threads per block: 64..1024, multiple of 64 */
     int32_t blocks_per_cu;     /* grid = blocks_per_cu * CUs                  */
-    int32_t chunk;             /* photons per work-queue chunk, 64..2^24.  A
+    int32_t chunk;             /* photons per work-queue chunk, 64..2^24;
+                                  a walk raises it to `block` when smaller.  A
                                   block keeps 4 chunks in flight and a chunk
                                   holds its tally slot until its longest walk
-                                  ends: chunks under ~2x the block's threads
-                                  leave lanes idle while the queue still has
-                                  chunks (a block whose four slots all hold
-                                  walking chunks reads the queue counter to
-                                  enter the cooperative tail once it is dry) */
+                                  ends, so chunks much smaller than the block
+                                  would leave its lanes idle while the queue
+                                  still has chunks (a block whose slots all
+                                  hold walking chunks reads the queue counter
+                                  to enter the cooperative tail once it is
+                                  dry).  mcs_get_launch returns the value set */
     int32_t refill_min;        /* idle lanes that trigger a wave refill 1..64 */
     int32_t coop_live;         /* native mode, once the work queue is empty: a
                                   wave with at most this many live photons

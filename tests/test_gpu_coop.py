@@ -276,11 +276,12 @@ def test_hand_over_across_cases(walker):
 
 @pytest.mark.parametrize('shape', [(1024, 1, 977, 8, 32), (640, 2, 977, 8, 32), (256, 2, 300, 4, 16)])
 def test_small_chunks_reach_the_cooperative_tail(walker, shape):
-    """Chunks smaller than a block's lanes can fill all four tally slots with
-    walking chunks; such a block can acquire nothing, so it must learn from
-    the global counter that the queue ran dry -- else its long walks finish
-    one lane each (5 ms instead of 0.8 ms for the 1e6-photon case).  Checked
-    through the cooperative photon-steps (count slot 9) and the law."""
+    """Small chunks (a walk raises them to the block's threads) can fill all
+    four tally slots with walking chunks; such a block can acquire nothing,
+    so it must learn from the global counter that the queue ran dry -- else
+    its long walks finish one lane each (5 ms instead of 0.8 ms for the
+    1e6-photon case).  Checked through the cooperative photon-steps (count
+    slot 9) and the law."""
     from monte_carlompi_amd.workloads import snow_1m_case
     walker.set_cases([snow_1m_case(400_000)])
     walker.set_launch(*shape)
@@ -296,6 +297,34 @@ def test_small_chunks_reach_the_cooperative_tail(walker, shape):
         for cond in (1, 2, 4):
             z = two_sample_z(np.sum(r['condition'] == cond), r['condition'].size,
                              np.sum(r0['condition'] == cond), r0['condition'].size)
+            assert abs(z) < 4, (cond, z)
+    finally:
+        walker.set_launch()
+
+
+def test_chunks_under_the_block_do_not_starve_it(walker):
+    """A launch whose chunks are much smaller than its blocks (489 photons
+    for 1024 threads) walks with chunks of the block's threads: the
+    1e6-photon case takes about as long as at the benchmark's chunk (with
+    489-photon chunks its blocks waited for tally slots: ~8 x longer), and
+    the records keep the law."""
+    from monte_carlompi_amd.workloads import snow_1m_case
+    walker.set_cases([snow_1m_case(1_000_000)])
+    try:
+        t = {}
+        recs = {}
+        for shape in [(1024, 1, 1954, 16, 32), (1024, 1, 489, 16, 32)]:
+            walker.set_launch(*shape)
+            for k in range(4):
+                walker.walk(mode='native', seed=40 + k, records=True)
+            t[shape[2]] = walker.stats()['kernel_ms']
+            recs[shape[2]] = walker.records()['condition']
+            assert int(walker.counts()[0][0]) == 1_000_000
+        print('kernel ms by chunk', t, flush=True)
+        assert t[489] < 2.5 * t[1954], t
+        a, b = recs[489], recs[1954]
+        for cond in (1, 2, 4):
+            z = two_sample_z(int(np.sum(a == cond)), a.size, int(np.sum(b == cond)), b.size)
             assert abs(z) < 4, (cond, z)
     finally:
         walker.set_launch()
