@@ -628,19 +628,16 @@ struct NativeWalkerT {
         o.accb = u01f(mainb.y);
     }
 
-    // 2 phi given cos theta: the first candidate r2 (revolutions), its
-    // acceptance uniform and its sines and cosines come with the packet,
-    // retries draw blocks (photon, case, i, 2 + k); target I S11 + S12 (Q cos
-    // 2phi + U sin 2phi), bound |I| S11 + |S12| sqrt(Q^2 + U^2) (I = 1 after
-    // the first scattering; the initial vector keeps its own I).  Returns the
-    // accepted r2 with the sine and cosine of the azimuth (r2 / 2) and of 2
-    // phi.  The first candidate is tested outside the retry loop: it is
-    // accepted with probability >= 1/2, so the common step is straight-line.
-    // phi_ok tests the packet's candidate against the current Stokes vector;
-    // phi_second makes the packet's second candidate (from the step's main
-    // block) its candidate; phi_retry then draws until one is accepted
-    // (blocks (photon, case, i, 2 + k), k >= 1), replacing the packet's
-    // candidate (r2, its sines and cosines and signs) in place.
+    // 2 phi given cos theta, by rejection: target I S11 + S12 (Q cos 2phi + U
+    // sin 2phi), bound |I| S11 + |S12| sqrt(Q^2 + U^2) (I = 1 after the first
+    // scattering; the initial vector keeps its own I); acceptance >= 1/2.
+    // The packet carries the first candidate (r2 = 2 phi in revolutions, its
+    // acceptance uniform, the sines and cosines of phi and 2 phi and the
+    // rotation signs) and the second's two uniforms.  phi_ok tests the
+    // packet's candidate against the current Stokes vector; phi_second makes
+    // the second candidate the packet's; phi_retry then draws until one is
+    // accepted (blocks (photon, case, i, 2 + k), k >= 1), replacing the
+    // packet's candidate in place and returning its r2.
     __device__ __forceinline__ bool phi_ok(const PolarPacket &k) const
     {
 #pragma clang fp contract(off)
@@ -654,7 +651,6 @@ struct NativeWalkerT {
         phi_trig(k.r2b, k.sinp, k.cosp, k.c2r, k.s2r);
         phi_signs(k.r2b, k.s2r, k.s2s, k.m2s);
     }
-    // (returns the accepted candidate's 2 phi in revolutions)
     __device__ float phi_retry(const KParams &P, PolarPacket &k) const
     {
 #pragma clang fp contract(off)
@@ -674,12 +670,12 @@ struct NativeWalkerT {
         }
     }
 
-    // One step of an exact-sampler photon given its packet: 2 phi by
-    // rejection against the current Stokes vector (retries draw blocks
-    // (photon, case, i, 2 + k)), rotation, Stokes update, move (polar_move),
-    // then the events (polar_event).  The one-lane walk and the tail mode
-    // both step through these two, so a photon's record does not depend on
-    // which of them walked it.  Returns true when the walk has ended.
+    // One step of an exact-sampler photon given its packet: 2 phi, rotation,
+    // Stokes update and move (polar_move_seq), then the events (polar_event).
+    // The tail mode's polar_move takes the same candidate by the same tests
+    // and forms the step with the same arithmetic, so a photon's record does
+    // not depend on which of them walked it.  Returns true when the walk has
+    // ended.
     __device__ __forceinline__ bool polar_step(const KParams &P, PolarPacket &k)
     {
         return polar_move_seq(P, k) && polar_event(P, k.dtau, k.w);
