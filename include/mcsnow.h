@@ -133,7 +133,11 @@ typedef struct {
                                   wave with at most this many live photons
                                   (0..32) walks them cooperatively, 64 / L
                                   lanes per photon stepping it in parallel;
-                                  0 = off.  Default 16.  Polar cases
+                                  0 = off (records bit-reproducible; with at
+                                  most 4 waves per SIMD a dry wave with <= 16
+                                  photons builds their next steps' Philox and
+                                  angles in its idle lanes, records
+                                  unchanged).  Default 16.  Polar cases
                                   (mcs_set_polar): a wave with at most
                                   min(coop_live, 16) photons builds their next
                                   steps' state-independent part in its idle

@@ -382,6 +382,102 @@ __device__ __forceinline__ bool hot_walk_step(W &w, const KParams &P)
     else return w.step(P);
 }
 
+// Tail mode of the reproducible HG walk (coop_live = 0).  Once the queue is
+// empty and a wave holds at most HG_TAIL photons, the wave works in rounds as
+// the polarised tail does: the G = HG_PK / 2^ceil(log2 L) lanes of group g
+// build the packets of the next G steps of the group's photon (Philox block,
+// free path, cos theta, the azimuth's sine and cosine: hg_packet), then every
+// photon's own lane walks those G steps through hg_move -- rotation, move and
+// event test, ~25 instructions instead of the whole step -- until the step in
+// which any photon of the wave has an event or reaches the step cap, or the
+// last packet (a wave-uniform exit).  The packet and the move use the
+// expressions of NativeWalkerT::step and the event its hg_event, so records
+// equal the one-lane walk's bit for bit.  LDS (the dynamic area, which holds
+// no cooperative tables at coop_live = 0): HG_TAIL owner lanes and
+// HG_PK_SLOTS packets per wave.
+constexpr int HG_TAIL = 16;
+constexpr int HG_PK_LOG = 6;
+constexpr int HG_PK = 1 << HG_PK_LOG;
+constexpr int HG_PK_SLOTS = HG_PK + 1;   // the last group's prefetch may read one past
+template <int WPE>
+constexpr bool HG_TAIL_IN = WPE <= 4;    // the builds that carry the packet tail
+// live photons at which a wave with the queue dry leaves the hot loop for its
+// tail: the cooperative tail's threshold, or the packet tail's at coop_live 0
+template <int WPE>
+__device__ __forceinline__ int tail_live(const KParams &P)
+{
+    if constexpr (HG_TAIL_IN<WPE>) return P.coop_live > 0 ? P.coop_live : HG_TAIL;
+    else return P.coop_live;
+}
+// (a template, so that only the builds that carry it instantiate it: another
+// caller of the walker's non-inlined helpers changes how the others inline)
+template <int WPE>
+__device__ void hg_tail(NativeWalker &w, int &state, const KParams &P, int wave)
+{
+    extern __shared__ __align__(16) CoopEntry coop_tab[];
+    int *const owners = reinterpret_cast<int *>(coop_tab);
+    HgPacket *const pk = reinterpret_cast<HgPacket *>(owners + (blockDim.x >> 6) * HG_TAIL) + wave * HG_PK_SLOTS;
+    int *const own = owners + wave * HG_TAIL;
+    const int lane = lane_id();
+    for (;;) {
+        const unsigned long long live = __ballot(state == LIVE);
+        if (live == 0ull) break;
+        const int L = __popcll(live);
+        const int lg = (L <= 1) ? 0 : 32 - __clz(L - 1);
+        const int G = HG_PK >> lg;                 // >= 2 (L <= HG_TAIL)
+        if (state == LIVE) own[lane_rank(live)] = lane;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int g = lane >> (HG_PK_LOG - lg);    // >= L for lanes >= HG_PK
+        const int o = own[g < L ? g : 0];
+        const uint32_t ck0 = bperm(o, w.pre.ck0), h0 = bperm(o, w.pre.h0), l0 = bperm(o, w.pre.l0);
+        const uint32_t io = bperm(o, w.i);
+        const float ca = bperm(o, w.a), cb = bperm(o, w.b), ccc = bperm(o, w.cc), cd = bperm(o, w.d), ce = bperm(o, w.e);
+        const bool iso = bperm(o, (uint32_t)w.iso) != 0u;
+        if (g < L) {
+            const PhiloxPre pre{ck0, h0, l0};
+            const u32x4 cur = philox_step(pre, io + 1u + (uint32_t)(lane & (G - 1)), P.key0, P.key1);
+            HgPacket q;
+            NativeWalker::hg_packet(cur, ca, cb, ccc, cd, ce, iso, q);
+            pk[lane] = q;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (state == LIVE) {
+            // photons in the tail have i < max_steps: the first step is taken
+            const HgPacket *const mine = pk + lane_rank(live) * G;
+            uint32_t cap = P.max_steps;
+            asm volatile("" : "+v"(cap));
+            HgPacket a = mine[0];
+            int k = 0;
+            for (;;) {
+                const HgPacket b = mine[k + 1];
+                bool ev = w.hg_move(a);
+                ++k;
+                if (__ballot(ev || w.i >= cap) != 0ull || k >= G) break;
+                a = mine[k + 1];
+                ev = w.hg_move(b);
+                ++k;
+                if (__ballot(ev || w.i >= cap) != 0ull || k >= G) break;
+            }
+            // the last step's event test, formed again from the state it left
+            const HgPacket &last = mine[k - 1];
+            const bool ev = w.z > 0.f || w.z < -w.tau_tot || last.w < w.t_abs_all;
+            bool d = false;
+            if (ev) {
+                w.r = philox_step(w.pre, w.i + 1u, P.key0, P.key1);   // the next step's block (re-emission)
+                d = w.hg_event(P, last.dtau, last.w);
+            }
+            if (d || w.i >= P.max_steps) {
+                if (!d) w.cond = 0;
+                state = DONE;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+}
+
 // Wave issue priority from the oldest live photon (i = its step count, 0 for
 // other lanes): long walks set the tail of a launch, so their waves win issue
 // slots over young ones.
@@ -467,6 +563,17 @@ k_walk(const KParams P)
                 }
                 break;
             }
+            // reproducible mode (no cooperative tail): the packet tail mode,
+            // in the 128-VGPR build (in the smaller ones its code would add
+            // spills to the step loop; they keep the one-lane drain)
+            if constexpr (HG_TAIL_IN<WPE>) {
+                if (P.coop_live == 0 && lds_load(&bs.exhausted) && __popcll(live) <= HG_TAIL) {
+                    __builtin_amdgcn_s_setprio(3);
+                    hg_tail<WPE>(w, state, P, wave);
+                    retire<W, DIRECT>(bs, P, w, state, my_slot, my_gid);
+                    break;
+                }
+            }
         }
         constexpr bool PTAIL = W::STOKES && !W::COOP && sizeof(typename W::Real) == 4;
         const int ptail = PTAIL ? min(P.coop_live, 16) : 0;
@@ -482,7 +589,7 @@ k_walk(const KParams P)
         }
         // leave the hot loop once `thresh` lanes are not live (with the queue
         // empty: once the cooperative tail can take over)
-        const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - P.coop_live : 64 - ptail)
+        const int thresh = lds_load(&bs.exhausted) ? (W::COOP ? 64 - tail_live<WPE>(P) : 64 - ptail)
                                                    : min(64, (64 - __popcll(live)) + (int)P.refill_min);
         // ---------------- hot: step every live photon
         // Two steps per loop turn, the exit test after the second: the next
@@ -1373,11 +1480,17 @@ static int enqueue_walk(mcs_ctx *c, int mode, const KParams &P, uint32_t n_chunk
                 hipLaunchKernelGGL((k_walk<PolarWalker, 3>), dim3(grid), dim3(block), lds, c->stream, P);
         }
     } else if (mode == MCS_NATIVE) {
-        const size_t tab = (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int));
         // the instance whose register budget fits the shape's waves per SIMD:
         // <= 4 (1024-thread blocks, one per CU) 128 VGPRs, 5 96, more 80 (the
         // 6-wave build's spills sit outside the step loop)
         const int waves_per_simd = (c->launch.blocks_per_cu * (block / 64) + 3) / 4;
+        // cooperative tables, or at coop_live 0 the packet tail's owners and
+        // packets (in the build that carries it)
+        const size_t tab = c->launch.coop_live > 0
+                               ? (size_t)(block / 64) * c->launch.coop_live * (sizeof(CoopEntry) + sizeof(int))
+                           : waves_per_simd <= 4
+                               ? (size_t)(block / 64) * (HG_TAIL * sizeof(int) + HG_PK_SLOTS * sizeof(HgPacket))
+                               : 0;
         if (waves_per_simd > 5 && P.rec_direct)
             hipLaunchKernelGGL((k_walk<NativeWalker, 6, true>), dim3(grid), dim3(block), tab, c->stream, P);
         else if (waves_per_simd > 5)

@@ -62,6 +62,19 @@ struct alignas(16) PolarPacket {
 };
 static_assert(sizeof(PolarPacket) == 80, "PolarPacket: five 16-byte LDS loads");
 
+// The state-independent part of a step of the HG walk: free path, absorption
+// word, cos and sin theta, the azimuth's sine and cosine (hg_packet, from the
+// step's Philox block with the expressions of NativeWalkerT::step, so that a
+// step formed from a packet built in another lane rounds alike).  32 bytes,
+// two 16-byte LDS loads.
+struct alignas(16) HgPacket {
+    float dtau;
+    uint32_t w;
+    float cost, sint, sinp, cosp;
+    float pad0, pad1;
+};
+static_assert(sizeof(HgPacket) == 32, "HgPacket: two 16-byte LDS loads");
+
 // sin and cos of phi and of 2 phi for 2 phi = r2 revolutions (one
 // transcendental pair; no implicit contraction, so every caller rounds alike)
 __device__ __forceinline__ void phi_trig(float r2, float &sinp, float &cosp, float &c2r, float &s2r)
@@ -389,9 +402,16 @@ struct NativeWalkerT {
         z = fmaf(dtau, muz, z);
         tau_acc += dtau;
         if (!(z > 0.f || z < -tau_tot || cur.w < t_abs_all || (CAP_IN_STEP && i >= P.max_steps))) return false;
-        // ---- rare: exits, absorption and the step cap.  The depth passes an
-        // empty asm so that the tests below are issued here, not hoisted into
-        // every step.
+        return hg_event(P, dtau, cur.w);
+    }
+
+    // The rare end of a step (its free path dtau and absorption word w):
+    // exits, absorption and the step cap; true when the walk has ended,
+    // false after a Lambertian re-emission below the cap (r must hold the
+    // next step's block).  The depth passes an empty asm so that the tests
+    // are issued here, not hoisted into every step.
+    __device__ __forceinline__ bool hg_event(const KParams &P, float dtau, uint32_t w)
+    {
         float ze = z;
         asm volatile("" : "+v"(ze));
         if (ze > 0.f) {                              // mc3D:1390-1397
@@ -416,8 +436,51 @@ struct NativeWalkerT {
         // absorbed: one uniform decides species and absorption jointly, with
         // P(cond 5) = P_imp (1 - omega_imp), P(cond 4) = (1 - P_imp)(1 - omega_ice)
         // (mc3D:1374-1383, 1461-1466); otherwise the step cap (condition 0)
-        cond = (cur.w < t_abs5) ? 5 : (CAP_IN_STEP && cur.w >= t_abs_all) ? 0 : 4;
+        cond = (w < t_abs5) ? 5 : (CAP_IN_STEP && w >= t_abs_all) ? 0 : 4;
         return true;
+    }
+
+    // Packet of a step of the HG walk from its Philox block and the band's
+    // constants (the expressions of step(); tail mode of the reproducible walk)
+    __device__ __forceinline__ static void hg_packet(const u32x4 &cur, float a, float b, float cc, float d, float e,
+                                                     bool iso, HgPacket &o)
+    {
+        o.dtau = free_path(cur);
+        o.w = cur.w;
+        const float xf = (float)cur.x;
+        const float rev = u01f(cur.y);
+        const float t = cc * __builtin_amdgcn_rcpf(fmaf(e, xf, d));
+        o.cost = iso ? fmaf(-0x1p-31f, xf, 1.f) : fminf(fmaxf(a * fmaf(-t, t, b), -1.f), 1.f);
+        o.sint = __builtin_amdgcn_sqrtf(fmaf(-o.cost, o.cost, 1.f));
+        o.sinp = __builtin_amdgcn_sinf(rev);
+        o.cosp = __builtin_amdgcn_cosf(rev);
+        o.pad0 = o.pad1 = 0.f;
+    }
+
+    // The move of a step from its packet (the expressions of step()); true
+    // when an event follows (exit through either face, or absorbed)
+    __device__ __forceinline__ bool hg_move(const HgPacket &k)
+    {
+        ++i;
+        const float cost = k.cost, sint = k.sint, sinp = k.sinp, cosp = k.cosp;
+        const float s2 = fmaf(-muz, muz, 1.f);
+        const float st = sint * __builtin_amdgcn_rsqf(s2);
+        const float mz0 = muz;
+        const float gx = fmaf(st, fmaf(mux * muz, cosp, -muy * sinp), mux * cost);
+        const float gy = fmaf(st, fmaf(muy * muz, cosp, mux * sinp), muy * cost);
+        muz = fmaf(-s2 * st, cosp, muz * cost);
+        mux = gx; muy = gy;
+        if (__builtin_expect(__ballot(s2 < 1e-7f) != 0ull, 0)) {
+            if (s2 < 1e-7f) {
+                const float sg = mz0 > 0.f ? 1.f : -1.f;
+                mux = sint * cosp;
+                muy = sg * sint * sinp;
+                muz = sg * cost;
+            }
+        }
+        z = fmaf(k.dtau, muz, z);
+        tau_acc += k.dtau;
+        return z > 0.f || z < -tau_tot || k.w < t_abs_all;
     }
 
     // ids from the Philox words: ck0 = case ^ k0, l0 = lo(M0 * photon) with
