@@ -34,6 +34,24 @@ from .gpu import Band, Case, GpuWalker, PolarTables
 from .parallelize import Parallel
 
 TWO_PIE = 2 * np.pi
+
+
+def wavelength_bins(wvls):
+    """(uniq, inverse) of the photons' wavelengths, equal to
+    ``np.unique(wvls, return_inverse=True)``.  Wavelengths on the 0.01 um grid
+    (the ``np.around(..., 2)`` draws of mc3D:1519-1520) are binned by their
+    integer hundredths in O(n); any other input goes through np.unique."""
+    w = np.asarray(wvls, dtype=np.float64).ravel()
+    if w.size:
+        m = np.rint(w * 100.0)
+        lo, hi = m.min(), m.max()
+        if np.isfinite(lo) and np.isfinite(hi) and hi - lo < 65536 and np.array_equal(m / 100.0, w):
+            k = (m - lo).astype(np.int64)
+            present = np.flatnonzero(np.bincount(k))
+            lut = np.zeros(int(hi - lo) + 1, dtype=np.int64)
+            lut[present] = np.arange(present.size)
+            return (present + lo) / 100.0, lut[k]
+    return np.unique(w, return_inverse=True)
 _DEFAULTS = {'tau_tot': 1000000.0, 'imp_cnc': 0.0, 'rho_snw': 300.0, 'rho_ice': 917.0,
              'flg_crt': 1, 'flg_3D': 999, 'output_dir': 'monte_carlo_results',
              'optics_dir': 'inputdata', 'fi_imp': 'mie_sot_ChC90_dns_1317.nc'}
@@ -216,22 +234,29 @@ class MonteCarlo(object):
         par_wvls = Parallel(wvls)
         wvls = par_wvls.working_set
 
+        # The reference evaluates the optics per photon (mc3D:1522-1612); they
+        # depend on the wavelength alone, so here they are evaluated once per
+        # wavelength bin (uniq) and the photons carry their bin (band).
         pm = None
         if shape == 'sphere':
-            ssa_ice, ext_ice, g = self.get_optical_properties(wvls, rds_snw)
+            uniq, band = wavelength_bins(wvls)
+            self._check_bins(uniq)
+            ssa_ice, ext_ice, g = self.get_optical_properties(uniq, rds_snw)
         else:                                              # mc3D:1525-1545
             a = optics.aspherical_ssps(wvls, rds_snw, wvl0, self.optics_dir, shape, roughness,
                                        bool(self.HG), self.rho_ice)
             wvls = a['wvls']                               # nearest table wavelengths
             par_wvls.working_set = wvls
-            ssa_ice, ext_ice, g = a['ssa'], a['ext_cff_mss'], a['g']
+            uniq, first, band = np.unique(wvls, return_index=True, return_inverse=True)
+            self._check_bins(uniq)
+            ssa_ice, ext_ice, g = a['ssa'][first], a['ext_cff_mss'][first], a['g'][first]
             self.snow_effective_radius = a['snow_effective_radius']
             pm = a['pm']
-        ssa_imp, ext_imp = self.get_impurity_optics(wvls)
+        ssa_imp, ext_imp = self.get_impurity_optics(uniq)
         if test:                                           # mc3D:1553-1573
             for name in ('ssa_ice', 'ext_cff_mss_ice', 'g', 'ssa_imp', 'ext_cff_mss_imp'):
                 if name in self.__dict__:
-                    val = self.__dict__[name] * np.ones(n_photon)
+                    val = self.__dict__[name] * np.ones(uniq.size)
                     if name == 'ssa_ice': ssa_ice = val
                     elif name == 'ext_cff_mss_ice': ext_ice = val
                     elif name == 'g': g = val
@@ -243,19 +268,16 @@ class MonteCarlo(object):
             p_ext_imp = (imp_cnc * ext_imp) / (imp_cnc * ext_imp + (1 - imp_cnc) * ext_ice)
         p_ext_imp = np.nan_to_num(p_ext_imp)
         tau_tot = self.tau_tot
-        if self.depth_m is not None:
-            tau_tot = float(self.depth_m * ext_cff_mss[0] * self.rho_snw)
+        if self.depth_m is not None:                       # the first photon's extinction
+            tau_tot = float(self.depth_m * ext_cff_mss[band[0]] * self.rho_snw)
         self.tau_tot_used = tau_tot
-        self.snow_depth = tau_tot / (ext_cff_mss * self.rho_snw)           # mc3D:1612
+        self.snow_depth = (tau_tot / (ext_cff_mss * self.rho_snw))[band]  # per photon, mc3D:1612
 
-        uniq, first, band = np.unique(wvls, return_index=True, return_inverse=True)
-        if uniq.size > 256:
-            raise ValueError('%d distinct photon wavelengths: at most 256 bins of 0.01 um per run'
-                             % uniq.size)
-        bands = [Band(wvl=float(uniq[j]), g=float(g[first[j]]), ssa_ice=float(ssa_ice[first[j]]),
-                      ext_cff=float(ext_cff_mss[first[j]] * self.rho_snw),
-                      ssa_imp=float(ssa_imp[first[j]]), p_ext_imp=float(p_ext_imp[first[j]]),
-                      weight=float(np.sum(band == j))) for j in range(uniq.size)]
+        weight = np.bincount(band, minlength=uniq.size)
+        bands = [Band(wvl=float(uniq[j]), g=float(g[j]), ssa_ice=float(ssa_ice[j]),
+                      ext_cff=float(ext_cff_mss[j] * self.rho_snw),
+                      ssa_imp=float(ssa_imp[j]), p_ext_imp=float(p_ext_imp[j]),
+                      weight=float(weight[j])) for j in range(uniq.size)]
         polar = None
         if pm is not None:                                 # mc3D:819-883, 1132
             x = pm.interp_arrays(float(uniq[0]))[0]
@@ -284,6 +306,12 @@ class MonteCarlo(object):
         self.write_records(output_file, all_answers)
         print('%s' % output_file)
         return output_file
+
+    @staticmethod
+    def _check_bins(uniq):
+        if uniq.size > 256:
+            raise ValueError('%d distinct photon wavelengths: at most 256 bins of 0.01 um per run'
+                             % uniq.size)
 
     def calculate_albedo(self, answers):
         """Directional-hemispherical reflectance sum(wvn | cond 1) / sum(wvn)

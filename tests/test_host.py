@@ -209,3 +209,22 @@ def test_spectral_albedo_table(tmp_path):
     np.testing.assert_allclose(rows[1][4], (1 + 1 / 0.9 + 1 / 1.1) / q, rtol=1e-14)
     # r = 100, 1.3 um: one reflected, one direct transmission (condition 3 counts as non-absorbed)
     np.testing.assert_allclose(rows[2][2:], (1.3, 0.5, 1.0), rtol=1e-15)
+
+
+def test_wavelength_bins_equal_numpy_unique():
+    """run() bins the photons' wavelengths by integer hundredths; the result
+    equals np.unique(return_inverse=True) for draws on the 0.01 um grid (as
+    mc3D:1519-1520 rounds them), for off-grid and non-finite values (which go
+    through np.unique), a single value and no photons."""
+    from monte_carlompi_amd.monte_carlo3D import wavelength_bins
+    rng = np.random.default_rng(4)
+    inputs = [np.around(rng.normal(1.3, 0.085 / 2.355, 100000), 2),
+              np.around(rng.normal(0.5, 0.2, 50000), 2),              # includes negative draws
+              np.around(rng.normal(1.3, 1e-12, 1000), 2),
+              rng.uniform(0.3, 2.5, 1000),                            # off the grid
+              np.array([1.3, np.nan, 1.31]), np.array([0.01]), np.array([], dtype=float)]
+    for w in inputs:
+        u, inv = wavelength_bins(w)
+        u0, inv0 = np.unique(w, return_inverse=True)
+        np.testing.assert_array_equal(u, u0)
+        np.testing.assert_array_equal(inv, inv0.ravel())
