@@ -228,3 +228,67 @@ def test_wavelength_bins_equal_numpy_unique():
         u0, inv0 = np.unique(w, return_inverse=True)
         np.testing.assert_array_equal(u, u0)
         np.testing.assert_array_equal(inv, inv0.ravel())
+
+
+class _CaptureWalker:
+    """Stands in for GpuWalker in run(): keeps the case and the photon bands it
+    is given and returns empty records (no device work)."""
+    def set_cases(self, cases):
+        self.cases = cases
+
+    def walk(self, mode, seed, records, photon_band=None, replay=None):
+        self.photon_band = np.array(photon_band)
+
+    def records(self):
+        n = self.cases[0].n_photons
+        return {k: np.zeros(n, t) for k, t in (('condition', np.int8), ('band', np.uint8), ('n_scat', np.int64),
+                                              ('theta_n', float), ('phi_n', float), ('path_length', float),
+                                              ('wvn', float), ('snow_depth', float))}
+
+    def tallies(self):
+        return np.zeros((1, 656))
+
+    def stats(self):
+        return {}
+
+
+def test_run_bins_photon_wavelengths_and_optics_per_bin(tmp_path, monkeypatch):
+    """run() evaluates the optics once per 0.01 um wavelength bin: the case's
+    bands, photon weights and per-photon band indices, tau_tot from the snow
+    depth and the per-photon snow_depth equal the per-photon evaluation of
+    mc3D:1515-1612 (np.unique over the photons' wavelengths, a lookup in the
+    ice table for every photon)."""
+    from scipy.io import netcdf_file
+    monkeypatch.setattr('sys.argv', ['x'])
+    ice = [c for c in CASES if c['name'] == 'spectral_imp'][0]['ice']
+    path = optics.ice_file(str(tmp_path), 100)
+    os.makedirs(os.path.dirname(path))
+    f = netcdf_file(path, 'w')
+    f.createDimension('wvl', ice['wvl'].size)
+    for k, v in ice.items():
+        f.createVariable(k, 'd', ('wvl',))[:] = v
+    f.close()
+    mc = MonteCarlo(output_dir=str(tmp_path), optics_dir=str(tmp_path), depth_m=0.4, seed=5)
+    cap = _CaptureWalker()
+    mc._walker = cap
+    n, wvl0, fwhm = 20000, 1.3, 0.085
+    np.random.seed(11)
+    mc.run(n, wvl0, fwhm, 100, theta_0=30.0, write=False)
+    np.random.seed(11)
+    wvls = np.around(np.random.normal(loc=wvl0, scale=fwhm / 2.355, size=n), decimals=2)
+    uniq, first, band = np.unique(wvls, return_index=True, return_inverse=True)
+    t = optics.SpectralTable.from_netcdf(path)
+    ssa_p, ext_p, g_p = (np.array([t.lookup(k, w) for w in wvls]) for k in ('ss_alb', 'ext_cff_mss', 'asm_prm'))
+    ssa, ext, g = ssa_p[first], ext_p[first], g_p[first]
+    assert np.unique(ext).size > 5                         # the optics do vary over the bins
+    case = cap.cases[0]
+    assert case.n_photons == n and len(case.bands) == uniq.size > 5
+    np.testing.assert_array_equal(cap.photon_band, band)
+    np.testing.assert_array_equal([b.wvl for b in case.bands], uniq)
+    np.testing.assert_array_equal([b.weight for b in case.bands], np.bincount(band))
+    np.testing.assert_array_equal([b.ext_cff for b in case.bands], ext * mc.rho_snw)
+    np.testing.assert_array_equal([b.ssa_ice for b in case.bands], ssa)
+    np.testing.assert_array_equal([b.g for b in case.bands], g)
+    tau = float(0.4 * ext_p[0] * mc.rho_snw)
+    assert case.tau_tot == tau
+    np.testing.assert_array_equal(mc.snow_depth, tau / (ext_p * mc.rho_snw))
