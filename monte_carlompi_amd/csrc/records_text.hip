@@ -2,14 +2,21 @@
 // (plus ' %r %r %r' of Q_n, U_n, V_n for aspherical grains), floats in
 // Python's shortest round-trip repr -- the text the reference writes for
 // numpy float64 scalars before numpy 2.  Host code only; rows are formatted
-// by up to 16 threads and written in photon order.
+// in blocks by up to 16 threads, and each thread writes its block at its
+// place in the file (pwrite) as soon as the blocks before it are formatted.
+#include <fcntl.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <atomic>
 #include <charconv>
 #include <cmath>
+#include <condition_variable>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -143,24 +150,50 @@ int mcs_write_records_text(const char *path, const char *header, const mcs_text_
     if ((r->q == nullptr) != (r->u == nullptr) || (r->q == nullptr) != (r->v == nullptr)) return MCS_ERR_ARG;
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const int64_t T = std::max<int64_t>(1, std::min<int64_t>({(int64_t)std::min(hw, 16u), (n + 8191) / 8192}));
-    std::vector<std::string> parts((size_t)T);
+    // blocks of 8192 rows, taken in order; a block's file offset is known once
+    // every block before it is formatted (sizes in order under the lock)
+    const int64_t B = 8192, K = (n + B - 1) / B;
+    const int fd = open(path, O_WRONLY | O_CREAT | O_TRUNC, 0666);
+    if (fd < 0) return MCS_ERR_ARG;
+    std::atomic<bool> ok{true};
+    const size_t hlen = header ? strlen(header) : 0;
+    if (hlen && pwrite(fd, header, hlen, 0) != (ssize_t)hlen) ok = false;
+    std::atomic<int64_t> next{0};
+    std::mutex mu;
+    std::condition_variable cv;
+    int64_t placed = 0;                      // blocks whose offsets are known
+    int64_t end = (int64_t)hlen;             // file offset after block placed - 1
+    auto work = [&] {
+        std::string text;
+        for (;;) {
+            const int64_t b = next.fetch_add(1);
+            if (b >= K) break;
+            text.clear();
+            format_rows(r, b * B, std::min(n, (b + 1) * B), text);
+            int64_t off;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return placed == b; });
+                off = end;
+                end += (int64_t)text.size();
+                ++placed;
+            }
+            cv.notify_all();
+            size_t done = 0;
+            while (done < text.size()) {
+                const ssize_t w = pwrite(fd, text.data() + done, text.size() - done, (off_t)(off + (int64_t)done));
+                if (w <= 0) { ok = false; break; }
+                done += (size_t)w;
+            }
+        }
+    };
     {
         std::vector<std::thread> th;
-        const int64_t per = (n + T - 1) / T;
-        for (int64_t t = 1; t < T; ++t) {
-            const int64_t a = std::min(n, t * per), b = std::min(n, a + per);
-            th.emplace_back([=, &parts] { format_rows(r, a, b, parts[(size_t)t]); });
-        }
-        format_rows(r, 0, std::min(n, per), parts[0]);
+        for (int64_t t = 1; t < T; ++t) th.emplace_back(work);
+        work();
         for (auto &x : th) x.join();
     }
-    FILE *f = fopen(path, "w");
-    if (!f) return MCS_ERR_ARG;
-    bool ok = true;
-    if (header) ok = fputs(header, f) >= 0;
-    for (const auto &p : parts)
-        if (ok && !p.empty()) ok = fwrite(p.data(), 1, p.size(), f) == p.size();
-    if (fclose(f) != 0) ok = false;
+    if (close(fd) != 0) ok = false;
     return ok ? MCS_OK : MCS_ERR_ARG;
 }
 
