@@ -344,8 +344,9 @@ def e2e_block(w, n):
     """The drop-in path end to end: MonteCarlo.run of the headline case
     (1e6 photons; wavelengths drawn and binned, tables uploaded, walk, record
     read-back, record file written), timed by the wall clock, and the same
-    without the file.  The file write (Python's shortest repr of every
-    float, the reference's numpy-1 text) dominates."""
+    without the file.  numpy's normal draws of the wavelengths and the file
+    write (Python's shortest repr of every float, the reference's numpy-1
+    text) take most of it."""
     import contextlib
     import io
     import sys
