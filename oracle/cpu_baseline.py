@@ -1,5 +1,5 @@
-"""CPU baseline: the reference's parallel photon loop restated (TEST /
-BASELINE INFRASTRUCTURE ONLY; bench.py's cpu_baseline leg).
+"""CPU baseline: the reference's parallel photon loop restated (test and
+CPU-baseline infrastructure only; bench.py's cpu_baseline leg).
 
 Runs oracle/walk.py -- the reference's per-step structure: five numpy draws
 per step, numpy scalar math, np.append trajectory growth -- as P worker
