@@ -214,8 +214,10 @@ typedef struct {
  * per photon '%d %r %r %r %d %r %r\n' (condition, wvn, theta_n, phi_n,
  * n_scat, path_length, snow_depth; aspherical: then ' %r %r %r' of Q_n, U_n,
  * V_n before the newline), each float as Python's shortest round-trip repr
- * (numpy float64's %r before numpy 2).  Formats on up to 16 host threads;
- * MCS_ERR_ARG for bad arguments or a file that cannot be written. */
+ * (numpy float64's %r before numpy 2).  Formats 8192-row blocks on up to 16
+ * host threads, each block written at its offset in the file (pwrite);
+ * MCS_ERR_ARG for bad arguments or a file that cannot be written (the file
+ * may then hold part of the text). */
 int mcs_write_records_text(const char *path, const char *header, const mcs_text_records *records,
                            int64_t n_photons);
 
