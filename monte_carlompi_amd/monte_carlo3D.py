@@ -43,8 +43,11 @@ def wavelength_bins(wvls):
     integer hundredths in O(n); any other input goes through np.unique."""
     w = np.asarray(wvls, dtype=np.float64).ravel()
     if w.size:
-        m = np.rint(w * 100.0)
+        m = w * 100.0
+        np.rint(m, out=m)
         lo, hi = m.min(), m.max()
+        if lo == hi and np.all(w == lo / 100.0):           # one bin (a narrow distribution)
+            return w[:1].copy(), np.zeros(w.size, dtype=np.int64)
         if np.isfinite(lo) and np.isfinite(hi) and hi - lo < 65536 and np.array_equal(m / 100.0, w):
             k = (m - lo).astype(np.int64)
             present = np.flatnonzero(np.bincount(k))
