@@ -233,7 +233,8 @@ class MonteCarlo(object):
         self.initial_stokes_params = stokes_params
 
         scale = half_width / 2.355                         # mc3D:1516
-        wvls = np.around(np.random.normal(loc=wvl0, scale=scale, size=(n_photon)), decimals=2)
+        wvls = np.random.normal(loc=wvl0, scale=scale, size=(n_photon))   # mc3D:1519-1520
+        np.around(wvls, decimals=2, out=wvls)
         par_wvls = Parallel(wvls)
         wvls = par_wvls.working_set
 
